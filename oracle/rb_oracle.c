@@ -1,0 +1,1219 @@
+/*
+ * rb_oracle.c -- CPU FP64 restatement of the reference's Rayleigh-Benard MRT-LBM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker and the CPU baseline); the product never calls it.
+ *
+ * Every routine names the reference routine it restates (file:line, paths relative to the
+ * reference repository root).  The reference keeps its state in module globals; here it lives
+ * in `orc_state`, with the same arrays and the same layout:
+ *   f(0:q-1, nx, ny, nz)                 population index fastest, interior only
+ *   f_post(0:q-1, 0:nx+1, 0:ny+1, 0:nz+1) one-cell halo whose ghosts stay zero
+ * and the same eight-sweep step: collision, streaming, bounceback, macro, collisionT,
+ * streamingT, bouncebackT, macroT (2DRB/3DRB.F90:225-243).
+ *
+ * Build with -ffp-contract=off: the reference's x86-64 build has no fused multiply-add, and
+ * expressions below keep the reference's evaluation order.
+ */
+#include "rb_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ----------------------------------------------------------------------------------------- */
+/* Lattice tables.  D3Q19 order, opp and weights: 2DRB/3DRB.F90:656-678.                     */
+/* D2Q9 order: 2DRB/2DRB.F90:188-190; weights 2DRB/2DRB.F90:506-524.                        */
+/* ----------------------------------------------------------------------------------------- */
+static const int EX3[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int EY3[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+static const int EZ3[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+static const int OPP3[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+static const int OPP7[7] = {0, 2, 1, 4, 3, 6, 5};
+
+static const int EX2[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const int EY2[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+
+struct orc_state {
+    orc_cfg c;
+    orc_params p;
+    int qf, qt;
+    long n;                      /* interior nodes */
+    long itc;
+    double omega[19], omegaT[7];
+    double *xp, *yp, *zp;        /* 0..n+1 node coordinates / L (3DRB.F90:451-470) */
+    double *f, *fpost, *g, *gpost;
+    double *rho, *u, *v, *w, *T;
+    double *up, *vp, *wp, *Tp;
+    double *bx, *by, *bz;
+    double *Fx, *Fy;             /* 2D only: forces stored by collision (2DRB.F90:718-719) */
+};
+
+/* ----------------------------------------------------------------------------------------- */
+/* Parameters: 2DRB/3DRB.F90:73-123 (3D) and 2DRB/2DRB.F90:61-115 (2D).                      */
+/* ----------------------------------------------------------------------------------------- */
+int orc_params_compute(const orc_cfg *c, orc_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->L = c->side_heated ? (double)c->nx : (double)c->ny;
+    p->Tref = 0.5 * (c->thot + c->tcold);
+    p->tauf = 0.5 + c->mach * p->L * sqrt(3.0 * c->prandtl / c->rayleigh);
+    p->viscosity = (p->tauf - 0.5) / 3.0;
+    p->diffusivity = p->viscosity / c->prandtl;
+    p->heat_flux_scale = p->L / p->diffusivity;
+    double gbeta1 = c->rayleigh * p->viscosity * p->diffusivity / p->L;
+    p->gbeta = gbeta1 / p->L / p->L;
+    p->time_unit = sqrt(p->L / p->gbeta);
+    p->velocity_unit = sqrt(p->gbeta * p->L);
+    p->Se = 1.0 / p->tauf;
+    p->Seps = 1.0 / p->tauf;
+    p->Snu = 1.0 / p->tauf;
+    p->Spi = 1.0 / p->tauf;
+    p->Sq = 8.0 * (2.0 * p->tauf - 1.0) / (8.0 * p->tauf - 1.0);
+    p->Sm = 8.0 * (2.0 * p->tauf - 1.0) / (8.0 * p->tauf - 1.0);
+    if (c->dims == 3) {
+        const double cs2T = 0.25;
+        p->taug = 0.5 + p->diffusivity / cs2T;           /* 3DRB.F90:121 */
+    } else {
+        p->taug = 0.5 + (p->tauf - 0.5) / c->prandtl;    /* 2DRB.F90:111 */
+    }
+    p->Qnu = 1.0;
+    p->Qk = 1.0 / p->taug;
+    p->SG = 1.0 - 0.5 * p->Qk;                           /* 3DRB.F90:1113, 2DRB.F90:915 */
+    return 0;
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* Compensated (Neumaier) summation used by every global reduction.                          */
+/* ----------------------------------------------------------------------------------------- */
+typedef struct { double s, c; } ksum;
+static inline void ks_add(ksum *a, double x)
+{
+    double t = a->s + x;
+    if (fabs(a->s) >= fabs(x)) a->c += (a->s - t) + x;
+    else a->c += (x - t) + a->s;
+    a->s = t;
+}
+static inline void ks_merge(ksum *a, ksum b)
+{
+    ks_add(a, b.s);
+    ks_add(a, b.c);
+}
+static inline double ks_val(ksum a) { return a.s + a.c; }
+
+/* ----------------------------------------------------------------------------------------- */
+/* Indexing helpers (0-based interior i in [0,nx); haloed index shifts by one).               */
+/* ----------------------------------------------------------------------------------------- */
+#define NX (s->c.nx)
+#define NY (s->c.ny)
+#define NZ (s->c.nz)
+static inline long IDX(const orc_state *s, long i, long j, long k)
+{
+    return i + (long)s->c.nx * (j + (long)s->c.ny * k);
+}
+static inline long HDX(const orc_state *s, long i, long j, long k)
+{
+    return (i + 1) + (long)(s->c.nx + 2) * ((j + 1) + (long)(s->c.ny + 2) * (k + 1));
+}
+
+static void *xcalloc(size_t n, size_t sz)
+{
+    void *p = calloc(n ? n : 1, sz);
+    return p;
+}
+
+orc_state *orc_create(const orc_cfg *cfg)
+{
+    if (cfg->dims != 2 && cfg->dims != 3) return NULL;
+    if (cfg->nx < 4 || cfg->ny < 4) return NULL;
+    if (cfg->dims == 3 && cfg->nz < 4) return NULL;
+    orc_state *s = (orc_state *)xcalloc(1, sizeof(orc_state));
+    s->c = *cfg;
+    if (s->c.dims == 2) s->c.nz = 1;
+    orc_params_compute(&s->c, &s->p);
+    s->qf = (s->c.dims == 3) ? 19 : 9;
+    s->qt = (s->c.dims == 3) ? 7 : 5;
+    s->n = (long)s->c.nx * s->c.ny * s->c.nz;
+    long nh = (long)(s->c.nx + 2) * (s->c.ny + 2) * (s->c.dims == 3 ? (s->c.nz + 2) : 1);
+    s->xp = (double *)xcalloc(s->c.nx + 2, sizeof(double));
+    s->yp = (double *)xcalloc(s->c.ny + 2, sizeof(double));
+    s->zp = (double *)xcalloc(s->c.nz + 2, sizeof(double));
+    s->f = (double *)xcalloc((size_t)s->n * s->qf, sizeof(double));
+    s->fpost = (double *)xcalloc((size_t)nh * s->qf, sizeof(double));
+    s->g = (double *)xcalloc((size_t)s->n * s->qt, sizeof(double));
+    s->gpost = (double *)xcalloc((size_t)nh * s->qt, sizeof(double));
+    double **macro[] = {&s->rho, &s->u, &s->v, &s->w, &s->T, &s->up, &s->vp, &s->wp,
+                        &s->Tp, &s->bx, &s->by, &s->bz, &s->Fx, &s->Fy};
+    for (unsigned m = 0; m < sizeof(macro) / sizeof(macro[0]); ++m)
+        *macro[m] = (double *)xcalloc((size_t)s->n, sizeof(double));
+    if (!s->f || !s->fpost || !s->g || !s->gpost || !s->Fy) {
+        orc_destroy(s);
+        return NULL;
+    }
+    if (s->c.dims == 3) {
+        /* 3DRB.F90:662-678 (non-legacy thermal weights) */
+        s->omega[0] = 1.0 / 3.0;
+        for (int a = 1; a <= 6; ++a) s->omega[a] = 1.0 / 18.0;
+        for (int a = 7; a <= 18; ++a) s->omega[a] = 1.0 / 36.0;
+        s->omegaT[0] = 1.0 / 4.0;
+        for (int a = 1; a <= 6; ++a) s->omegaT[a] = 1.0 / 8.0;
+    } else {
+        /* 2DRB.F90:506-524 */
+        s->omega[0] = 4.0 / 9.0;
+        for (int a = 1; a <= 4; ++a) s->omega[a] = 1.0 / 9.0;
+        for (int a = 5; a <= 8; ++a) s->omega[a] = 1.0 / 36.0;
+        s->omegaT[0] = 1.0 / 3.0;
+        for (int a = 1; a <= 4; ++a) s->omegaT[a] = 1.0 / 6.0;
+    }
+    return s;
+}
+
+void orc_destroy(orc_state *s)
+{
+    if (!s) return;
+    double *arr[] = {s->xp, s->yp, s->zp, s->f, s->fpost, s->g, s->gpost, s->rho, s->u,
+                     s->v, s->w, s->T, s->up, s->vp, s->wp, s->Tp, s->bx, s->by, s->bz,
+                     s->Fx, s->Fy};
+    for (unsigned m = 0; m < sizeof(arr) / sizeof(arr[0]); ++m) free(arr[m]);
+    free(s);
+}
+
+long orc_itc(const orc_state *s) { return s->itc; }
+
+/* ----------------------------------------------------------------------------------------- */
+/* Seeded noise: SplitMix64 finaliser of (seed, instance, global linear index).              */
+/* ----------------------------------------------------------------------------------------- */
+static inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double orc_noise(uint64_t seed, uint64_t instance, uint64_t idx)
+{
+    uint64_t key = seed + 0x9E3779B97F4A7C15ULL * (idx + 1ULL + (instance << 40));
+    double r = (double)(mix64(key) >> 11) * 0x1.0p-53;
+    return 2.0 * r - 1.0;
+}
+
+/* ----------------------------------------------------------------------------------------- */
+/* initial, exact branch: 3DRB.F90:437-470 (coordinates), :517-600 (fields);                 */
+/* 2DRB.F90:461-490, :540-608.                                                               */
+/* ----------------------------------------------------------------------------------------- */
+static void coords(double *xp, int n, double L)
+{
+    xp[0] = 0.0;
+    xp[n + 1] = (double)n;
+    for (int i = 1; i <= n; ++i) xp[i] = (double)i - 0.5;
+    for (int i = 0; i <= n + 1; ++i) xp[i] = xp[i] / L;
+}
+
+void orc_init_exact(orc_state *s, const double *T0)
+{
+    const orc_cfg *c = &s->c;
+    const double L = s->p.L;
+    const double pi = acos(-1.0);
+    s->itc = 0;
+    coords(s->xp, c->nx, L);
+    coords(s->yp, c->ny, L);
+    if (c->dims == 3) coords(s->zp, c->nz, L);
+    memset(s->fpost, 0, sizeof(double) * (size_t)s->qf * (c->nx + 2) * (c->ny + 2) *
+                             (c->dims == 3 ? (c->nz + 2) : 1));
+    memset(s->gpost, 0, sizeof(double) * (size_t)s->qt * (c->nx + 2) * (c->ny + 2) *
+                            (c->dims == 3 ? (c->nz + 2) : 1));
+    for (long n = 0; n < s->n; ++n) {
+        s->rho[n] = 1.0;
+        s->u[n] = s->v[n] = s->w[n] = 0.0;
+        s->T[n] = 0.0;
+        s->bx[n] = s->by[n] = s->bz[n] = 0.0;
+        s->up[n] = s->vp[n] = s->wp[n] = s->Tp[n] = 0.0;
+        s->Fx[n] = s->Fy[n] = 0.0;
+    }
+    for (int k = 0; k < c->nz; ++k)
+        for (int j = 0; j < c->ny; ++j)
+            for (int i = 0; i < c->nx; ++i) {
+                long n = IDX(s, i, j, k);
+                double T = 0.0;
+                if (T0) {
+                    T = T0[n];
+                } else {
+                    if (c->temp_x == ORC_T_CONST)   /* VerticalWallsConstT */
+                        T = c->thot + (s->xp[i + 1] - s->xp[0]) / (s->xp[c->nx + 1] - s->xp[0]) *
+                                          (c->tcold - c->thot);
+                    if (c->temp_y == ORC_T_CONST) { /* HorizontalWallsConstT */
+                        T = c->thot + (s->yp[j + 1] - s->yp[0]) / (s->yp[c->ny + 1] - s->yp[0]) *
+                                          (c->tcold - c->thot);
+                        if (!c->side_heated) {
+                            /* RB perturbation: 3D if Ra <= 1e4 (3DRB.F90:542-558),
+                             * 2D if Ra < 1e4 (2DRB.F90:568-581). */
+                            double amp = 1.0e-3 * (c->thot - c->tcold);
+                            double xLen = s->xp[c->nx + 1], yLen = s->yp[c->ny + 1];
+                            if (c->dims == 3 && c->rayleigh <= 1.0e4)
+                                T = T + amp * cos(2.0 * pi * s->xp[i + 1] / xLen) *
+                                            sin(pi * s->yp[j + 1] / yLen);
+                            if (c->dims == 2 && c->rayleigh < 1.0e4)
+                                T = T + amp * sin(2.0 * pi * s->xp[i + 1] / xLen) *
+                                            sin(pi * s->yp[j + 1] / yLen);
+                        }
+                    }
+                }
+                s->T[n] = T;
+            }
+    /* f = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2); g = wT T (1 + c e.u); B_prev = u T. */
+    for (long n = 0; n < s->n; ++n) {
+        double u = s->u[n], v = s->v[n], w = s->w[n], rho = s->rho[n], T = s->T[n];
+        if (c->dims == 3) {
+            double u2 = u * u + v * v + w * w;
+            for (int a = 0; a < 19; ++a) {
+                double eu = (double)EX3[a] * u + (double)EY3[a] * v + (double)EZ3[a] * w;
+                s->f[n * 19 + a] =
+                    s->omega[a] * rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * u2);
+            }
+            for (int a = 0; a < 7; ++a) {
+                double eu = (double)EX3[a] * u + (double)EY3[a] * v + (double)EZ3[a] * w;
+                s->g[n * 7 + a] = s->omegaT[a] * T * (1.0 + 4.0 * eu);
+            }
+        } else {
+            double us2 = u * u + v * v;
+            for (int a = 0; a < 9; ++a) {
+                double un = u * EX2[a] + v * EY2[a];
+                s->f[n * 9 + a] =
+                    rho * s->omega[a] * (1.0 + 3.0 * un + 4.5 * un * un - 1.5 * us2);
+            }
+            for (int a = 0; a < 5; ++a) {
+                double un = u * EX2[a] + v * EY2[a];
+                s->g[n * 5 + a] = s->omegaT[a] * T * (1.0 + 3.0 * un);
+            }
+        }
+        if (c->use_g) {
+            s->bx[n] = u * T;
+            s->by[n] = v * T;
+            s->bz[n] = w * T;
+        }
+    }
+}
+
+void orc_init_seeded(orc_state *s, uint64_t seed, uint64_t instance, double amp)
+{
+    orc_init_exact(s, NULL);
+    const double scale = amp * (s->c.thot - s->c.tcold);
+    for (long n = 0; n < s->n; ++n) {
+        double r = orc_noise(seed, instance, (uint64_t)n);
+        double noise = scale * r;
+        s->T[n] = s->T[n] + noise;
+        for (int a = 0; a < s->qt; ++a) s->g[n * s->qt + a] = s->omegaT[a] * s->T[n];
+    }
+}
+
+/* ========================================================================================= */
+/* 3D: D3Q19 MRT flow + D3Q7 MRT temperature                                                 */
+/* ========================================================================================= */
+
+/* collision, 3DRB.F90:689-949 */
+static void collision3d(orc_state *s)
+{
+    const orc_params *p = &s->p;
+    const double gBeta = p->gbeta, Tref = p->Tref;
+    double sr[19];
+    sr[0] = 0.0; sr[1] = p->Se; sr[2] = p->Seps; sr[3] = 0.0; sr[4] = p->Sq; sr[5] = 0.0;
+    sr[6] = p->Sq; sr[7] = 0.0; sr[8] = p->Sq; sr[9] = p->Snu; sr[10] = p->Spi;
+    sr[11] = p->Snu; sr[12] = p->Spi; sr[13] = p->Snu; sr[14] = p->Snu; sr[15] = p->Snu;
+    sr[16] = p->Sm; sr[17] = p->Sm; sr[18] = p->Sm;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const long n = IDX(s, i, j, k);
+                const double *F = s->f + n * 19;
+                double m[19], meq[19], mp[19], src[19];
+                const double rhoLoc = s->rho[n], uLoc = s->u[n], vLoc = s->v[n], wLoc = s->w[n];
+                const double u2 = uLoc * uLoc + vLoc * vLoc + wLoc * wLoc;
+
+                m[0] = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8] + F[9] +
+                       F[10] + F[11] + F[12] + F[13] + F[14] + F[15] + F[16] + F[17] + F[18];
+                m[1] = -30.0 * F[0] - 11.0 * (F[1] + F[2] + F[3] + F[4] + F[5] + F[6]) +
+                       8.0 * (F[7] + F[8] + F[9] + F[10] + F[11] + F[12] + F[13] + F[14] +
+                              F[15] + F[16] + F[17] + F[18]);
+                m[2] = 12.0 * F[0] - 4.0 * (F[1] + F[2] + F[3] + F[4] + F[5] + F[6]) +
+                       (F[7] + F[8] + F[9] + F[10] + F[11] + F[12] + F[13] + F[14] + F[15] +
+                        F[16] + F[17] + F[18]);
+                m[3] = F[1] - F[2] + F[7] - F[8] + F[9] - F[10] + F[11] - F[12] + F[13] - F[14];
+                m[4] = -4.0 * F[1] + 4.0 * F[2] + F[7] - F[8] + F[9] - F[10] + F[11] - F[12] +
+                       F[13] - F[14];
+                m[5] = F[3] - F[4] + F[7] + F[8] - F[9] - F[10] + F[15] - F[16] + F[17] - F[18];
+                m[6] = -4.0 * F[3] + 4.0 * F[4] + F[7] + F[8] - F[9] - F[10] + F[15] - F[16] +
+                       F[17] - F[18];
+                m[7] = F[5] - F[6] + F[11] + F[12] - F[13] - F[14] + F[15] + F[16] - F[17] - F[18];
+                m[8] = -4.0 * F[5] + 4.0 * F[6] + F[11] + F[12] - F[13] - F[14] + F[15] + F[16] -
+                       F[17] - F[18];
+                m[9] = 2.0 * (F[1] + F[2]) - (F[3] + F[4] + F[5] + F[6]) +
+                       (F[7] + F[8] + F[9] + F[10] + F[11] + F[12] + F[13] + F[14]) -
+                       2.0 * (F[15] + F[16] + F[17] + F[18]);
+                m[10] = -4.0 * (F[1] + F[2]) + 2.0 * (F[3] + F[4] + F[5] + F[6]) +
+                        (F[7] + F[8] + F[9] + F[10] + F[11] + F[12] + F[13] + F[14]) -
+                        2.0 * (F[15] + F[16] + F[17] + F[18]);
+                m[11] = (F[3] + F[4]) - (F[5] + F[6]) + (F[7] + F[8] + F[9] + F[10]) -
+                        (F[11] + F[12] + F[13] + F[14]);
+                m[12] = -2.0 * (F[3] + F[4]) + 2.0 * (F[5] + F[6]) +
+                        (F[7] + F[8] + F[9] + F[10]) - (F[11] + F[12] + F[13] + F[14]);
+                m[13] = F[7] - F[8] - F[9] + F[10];
+                m[14] = F[15] - F[16] - F[17] + F[18];
+                m[15] = F[11] - F[12] - F[13] + F[14];
+                m[16] = F[7] - F[8] + F[9] - F[10] - F[11] + F[12] - F[13] + F[14];
+                m[17] = -F[7] - F[8] + F[9] + F[10] + F[15] - F[16] + F[17] - F[18];
+                m[18] = F[11] + F[12] - F[13] - F[14] - F[15] - F[16] + F[17] + F[18];
+
+                meq[0] = rhoLoc;
+                meq[1] = rhoLoc * (-11.0 + 19.0 * u2);
+                meq[2] = rhoLoc * (3.0 - 11.0 * u2 / 2.0);
+                meq[3] = rhoLoc * uLoc;
+                meq[4] = -2.0 * rhoLoc * uLoc / 3.0;
+                meq[5] = rhoLoc * vLoc;
+                meq[6] = -2.0 * rhoLoc * vLoc / 3.0;
+                meq[7] = rhoLoc * wLoc;
+                meq[8] = -2.0 * rhoLoc * wLoc / 3.0;
+                meq[9] = rhoLoc * (2.0 * uLoc * uLoc - vLoc * vLoc - wLoc * wLoc);
+                meq[10] = -0.5 * meq[9];
+                meq[11] = rhoLoc * (vLoc * vLoc - wLoc * wLoc);
+                meq[12] = -0.5 * meq[11];
+                meq[13] = rhoLoc * uLoc * vLoc;
+                meq[14] = rhoLoc * vLoc * wLoc;
+                meq[15] = rhoLoc * uLoc * wLoc;
+                meq[16] = 0.0;
+                meq[17] = 0.0;
+                meq[18] = 0.0;
+
+                const double FxLoc = 0.0;
+                const double FyLoc = rhoLoc * gBeta * (s->T[n] - Tref);
+                const double FzLoc = 0.0;
+                const double uDotF = uLoc * FxLoc + vLoc * FyLoc + wLoc * FzLoc;
+                src[0] = 0.0;
+                src[1] = (1.0 - 0.5 * sr[1]) * 38.0 * uDotF;
+                src[2] = (1.0 - 0.5 * sr[2]) * (-11.0) * uDotF;
+                src[3] = (1.0 - 0.5 * sr[3]) * FxLoc;
+                src[4] = (1.0 - 0.5 * sr[4]) * (-2.0 / 3.0) * FxLoc;
+                src[5] = (1.0 - 0.5 * sr[5]) * FyLoc;
+                src[6] = (1.0 - 0.5 * sr[6]) * (-2.0 / 3.0) * FyLoc;
+                src[7] = (1.0 - 0.5 * sr[7]) * FzLoc;
+                src[8] = (1.0 - 0.5 * sr[8]) * (-2.0 / 3.0) * FzLoc;
+                src[9] = (1.0 - 0.5 * sr[9]) *
+                         (4.0 * uLoc * FxLoc - 2.0 * vLoc * FyLoc - 2.0 * wLoc * FzLoc);
+                src[10] = (1.0 - 0.5 * sr[10]) * (-2.0 * uLoc * FxLoc + vLoc * FyLoc + wLoc * FzLoc);
+                src[11] = (1.0 - 0.5 * sr[11]) * (2.0 * vLoc * FyLoc - 2.0 * wLoc * FzLoc);
+                src[12] = (1.0 - 0.5 * sr[12]) * (-vLoc * FyLoc + wLoc * FzLoc);
+                src[13] = (1.0 - 0.5 * sr[13]) * (uLoc * FyLoc + vLoc * FxLoc);
+                src[14] = (1.0 - 0.5 * sr[14]) * (vLoc * FzLoc + wLoc * FyLoc);
+                src[15] = (1.0 - 0.5 * sr[15]) * (uLoc * FzLoc + wLoc * FxLoc);
+                src[16] = 0.0;
+                src[17] = 0.0;
+                src[18] = 0.0;
+
+                for (int a = 0; a < 19; ++a) mp[a] = m[a] - sr[a] * (m[a] - meq[a]) + src[a];
+
+                double *P = s->fpost + HDX(s, i, j, k) * 19;
+                P[0] = mp[0] / 19.0 - 5.0 * mp[1] / 399.0 + mp[2] / 21.0;
+                P[1] = mp[0] / 19.0 - 11.0 * mp[1] / 2394.0 - mp[2] / 63.0 + mp[3] / 10.0 -
+                       mp[4] / 10.0 + mp[9] / 18.0 - mp[10] / 18.0;
+                P[2] = mp[0] / 19.0 - 11.0 * mp[1] / 2394.0 - mp[2] / 63.0 - mp[3] / 10.0 +
+                       mp[4] / 10.0 + mp[9] / 18.0 - mp[10] / 18.0;
+                P[3] = mp[0] / 19.0 - 11.0 * mp[1] / 2394.0 - mp[2] / 63.0 + mp[5] / 10.0 -
+                       mp[6] / 10.0 - mp[9] / 36.0 + mp[10] / 36.0 + mp[11] / 12.0 -
+                       mp[12] / 12.0;
+                P[4] = mp[0] / 19.0 - 11.0 * mp[1] / 2394.0 - mp[2] / 63.0 - mp[5] / 10.0 +
+                       mp[6] / 10.0 - mp[9] / 36.0 + mp[10] / 36.0 + mp[11] / 12.0 -
+                       mp[12] / 12.0;
+                P[5] = mp[0] / 19.0 - 11.0 * mp[1] / 2394.0 - mp[2] / 63.0 + mp[7] / 10.0 -
+                       mp[8] / 10.0 - mp[9] / 36.0 + mp[10] / 36.0 - mp[11] / 12.0 +
+                       mp[12] / 12.0;
+                P[6] = mp[0] / 19.0 - 11.0 * mp[1] / 2394.0 - mp[2] / 63.0 - mp[7] / 10.0 +
+                       mp[8] / 10.0 - mp[9] / 36.0 + mp[10] / 36.0 - mp[11] / 12.0 +
+                       mp[12] / 12.0;
+                P[7] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 + mp[3] / 10.0 +
+                       mp[4] / 40.0 + mp[5] / 10.0 + mp[6] / 40.0 + mp[9] / 36.0 +
+                       mp[10] / 72.0 + mp[11] / 12.0 + mp[12] / 24.0 + mp[13] / 4.0 +
+                       mp[16] / 8.0 - mp[17] / 8.0;
+                P[8] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 - mp[3] / 10.0 -
+                       mp[4] / 40.0 + mp[5] / 10.0 + mp[6] / 40.0 + mp[9] / 36.0 +
+                       mp[10] / 72.0 + mp[11] / 12.0 + mp[12] / 24.0 - mp[13] / 4.0 -
+                       mp[16] / 8.0 - mp[17] / 8.0;
+                P[9] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 + mp[3] / 10.0 +
+                       mp[4] / 40.0 - mp[5] / 10.0 - mp[6] / 40.0 + mp[9] / 36.0 +
+                       mp[10] / 72.0 + mp[11] / 12.0 + mp[12] / 24.0 - mp[13] / 4.0 +
+                       mp[16] / 8.0 + mp[17] / 8.0;
+                P[10] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 - mp[3] / 10.0 -
+                        mp[4] / 40.0 - mp[5] / 10.0 - mp[6] / 40.0 + mp[9] / 36.0 +
+                        mp[10] / 72.0 + mp[11] / 12.0 + mp[12] / 24.0 + mp[13] / 4.0 -
+                        mp[16] / 8.0 + mp[17] / 8.0;
+                P[11] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 + mp[3] / 10.0 +
+                        mp[4] / 40.0 + mp[7] / 10.0 + mp[8] / 40.0 + mp[9] / 36.0 +
+                        mp[10] / 72.0 - mp[11] / 12.0 - mp[12] / 24.0 + mp[15] / 4.0 -
+                        mp[16] / 8.0 + mp[18] / 8.0;
+                P[12] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 - mp[3] / 10.0 -
+                        mp[4] / 40.0 + mp[7] / 10.0 + mp[8] / 40.0 + mp[9] / 36.0 +
+                        mp[10] / 72.0 - mp[11] / 12.0 - mp[12] / 24.0 - mp[15] / 4.0 +
+                        mp[16] / 8.0 + mp[18] / 8.0;
+                P[13] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 + mp[3] / 10.0 +
+                        mp[4] / 40.0 - mp[7] / 10.0 - mp[8] / 40.0 + mp[9] / 36.0 +
+                        mp[10] / 72.0 - mp[11] / 12.0 - mp[12] / 24.0 - mp[15] / 4.0 -
+                        mp[16] / 8.0 - mp[18] / 8.0;
+                P[14] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 - mp[3] / 10.0 -
+                        mp[4] / 40.0 - mp[7] / 10.0 - mp[8] / 40.0 + mp[9] / 36.0 +
+                        mp[10] / 72.0 - mp[11] / 12.0 - mp[12] / 24.0 + mp[15] / 4.0 +
+                        mp[16] / 8.0 - mp[18] / 8.0;
+                P[15] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 + mp[5] / 10.0 +
+                        mp[6] / 40.0 + mp[7] / 10.0 + mp[8] / 40.0 - mp[9] / 18.0 -
+                        mp[10] / 36.0 + mp[14] / 4.0 + mp[17] / 8.0 - mp[18] / 8.0;
+                P[16] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 - mp[5] / 10.0 -
+                        mp[6] / 40.0 + mp[7] / 10.0 + mp[8] / 40.0 - mp[9] / 18.0 -
+                        mp[10] / 36.0 - mp[14] / 4.0 - mp[17] / 8.0 - mp[18] / 8.0;
+                P[17] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 + mp[5] / 10.0 +
+                        mp[6] / 40.0 - mp[7] / 10.0 - mp[8] / 40.0 - mp[9] / 18.0 -
+                        mp[10] / 36.0 - mp[14] / 4.0 + mp[17] / 8.0 + mp[18] / 8.0;
+                P[18] = mp[0] / 19.0 + 4.0 * mp[1] / 1197.0 + mp[2] / 252.0 - mp[5] / 10.0 -
+                        mp[6] / 40.0 - mp[7] / 10.0 - mp[8] / 40.0 - mp[9] / 18.0 -
+                        mp[10] / 36.0 + mp[14] / 4.0 - mp[17] / 8.0 + mp[18] / 8.0;
+            }
+}
+
+/* streaming, 3DRB.F90:957-979 and streamingT, 3DRB.F90:1209-1232 (pull from the haloed
+ * post-collision array; ghost reads return the zeros the wall sweeps then overwrite). */
+static void streaming3d(orc_state *s, double *dst, const double *src, int q)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i)
+                for (int a = 0; a < q; ++a)
+                    dst[IDX(s, i, j, k) * q + a] =
+                        src[HDX(s, i - EX3[a], j - EY3[a], k - EZ3[a]) * q + a];
+}
+
+/* bounceback, 3DRB.F90:987-1058.  Sweep order: x periodic, x no-slip, y no-slip, z periodic,
+ * z no-slip.  The periodic copies keep the destination's transverse indices. */
+static void bounceback3d(orc_state *s)
+{
+    double *f = s->f;
+    const double *P = s->fpost;
+    if (s->c.vel_x == ORC_VEL_PERIODIC) {
+        for (int k = 0; k < NZ; ++k)
+            for (int j = 0; j < NY; ++j)
+                for (int a = 0; a < 19; ++a) {
+                    if (EX3[a] == 1) f[IDX(s, 0, j, k) * 19 + a] = P[HDX(s, NX - 1, j, k) * 19 + a];
+                    if (EX3[a] == -1) f[IDX(s, NX - 1, j, k) * 19 + a] = P[HDX(s, 0, j, k) * 19 + a];
+                }
+    } else {
+        for (int k = 0; k < NZ; ++k)
+            for (int j = 0; j < NY; ++j)
+                for (int a = 0; a < 19; ++a) {
+                    if (EX3[a] == 1) f[IDX(s, 0, j, k) * 19 + a] = P[HDX(s, 0, j, k) * 19 + OPP3[a]];
+                    if (EX3[a] == -1)
+                        f[IDX(s, NX - 1, j, k) * 19 + a] = P[HDX(s, NX - 1, j, k) * 19 + OPP3[a]];
+                }
+    }
+    for (int k = 0; k < NZ; ++k)
+        for (int i = 0; i < NX; ++i)
+            for (int a = 0; a < 19; ++a) {
+                if (EY3[a] == 1) f[IDX(s, i, 0, k) * 19 + a] = P[HDX(s, i, 0, k) * 19 + OPP3[a]];
+                if (EY3[a] == -1)
+                    f[IDX(s, i, NY - 1, k) * 19 + a] = P[HDX(s, i, NY - 1, k) * 19 + OPP3[a]];
+            }
+    if (s->c.vel_z == ORC_VEL_PERIODIC) {
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i)
+                for (int a = 0; a < 19; ++a) {
+                    if (EZ3[a] == 1) f[IDX(s, i, j, 0) * 19 + a] = P[HDX(s, i, j, NZ - 1) * 19 + a];
+                    if (EZ3[a] == -1) f[IDX(s, i, j, NZ - 1) * 19 + a] = P[HDX(s, i, j, 0) * 19 + a];
+                }
+    } else {
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i)
+                for (int a = 0; a < 19; ++a) {
+                    if (EZ3[a] == 1) f[IDX(s, i, j, 0) * 19 + a] = P[HDX(s, i, j, 0) * 19 + OPP3[a]];
+                    if (EZ3[a] == -1)
+                        f[IDX(s, i, j, NZ - 1) * 19 + a] = P[HDX(s, i, j, NZ - 1) * 19 + OPP3[a]];
+                }
+    }
+}
+
+/* macro, 3DRB.F90:1066-1098: the half force uses the new rho and the old T. */
+static void macro3d(orc_state *s)
+{
+    const double gBeta = s->p.gbeta, Tref = s->p.Tref;
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < s->n; ++n) {
+        const double *F = s->f + n * 19;
+        double rho = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8] + F[9] +
+                     F[10] + F[11] + F[12] + F[13] + F[14] + F[15] + F[16] + F[17] + F[18];
+        s->rho[n] = rho;
+        double FyLoc = rho * gBeta * (s->T[n] - Tref);
+        s->u[n] = (F[1] - F[2] + F[7] - F[8] + F[9] - F[10] + F[11] - F[12] + F[13] - F[14]) / rho;
+        s->v[n] = (F[3] - F[4] + F[7] + F[8] - F[9] - F[10] + F[15] - F[16] + F[17] - F[18] +
+                   0.5 * FyLoc) / rho;
+        s->w[n] = (F[5] - F[6] + F[11] + F[12] - F[13] - F[14] + F[15] + F[16] - F[17] - F[18]) / rho;
+    }
+}
+
+/* collisionT, 3DRB.F90:1106-1198 (non-legacy branch). */
+static void collisionT3d(orc_state *s)
+{
+    const double Qk = s->p.Qk, Qnu = s->p.Qnu, SG = s->p.SG;
+    const int useG = s->c.use_g;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const long n = IDX(s, i, j, k);
+                const double *G = s->g + n * 7;
+                double nm[7], neq[7], q[7], np[7];
+                const double Tl = s->T[n];
+                const double Bx = s->u[n] * Tl, By = s->v[n] * Tl, Bz = s->w[n] * Tl;
+                double dBx = 0.0, dBy = 0.0, dBz = 0.0;
+                if (useG) {
+                    dBx = Bx - s->bx[n];
+                    dBy = By - s->by[n];
+                    dBz = Bz - s->bz[n];
+                    s->bx[n] = Bx;
+                    s->by[n] = By;
+                    s->bz[n] = Bz;
+                }
+                nm[0] = G[0] + G[1] + G[2] + G[3] + G[4] + G[5] + G[6];
+                nm[1] = G[1] - G[2];
+                nm[2] = G[3] - G[4];
+                nm[3] = G[5] - G[6];
+                nm[4] = -6.0 * G[0] + G[1] + G[2] + G[3] + G[4] + G[5] + G[6];
+                nm[5] = 2.0 * G[1] + 2.0 * G[2] - G[3] - G[4] - G[5] - G[6];
+                nm[6] = G[3] + G[4] - G[5] - G[6];
+                neq[0] = Tl;
+                neq[1] = Bx;
+                neq[2] = By;
+                neq[3] = Bz;
+                neq[4] = -0.75 * Tl;
+                neq[5] = 0.0;
+                neq[6] = 0.0;
+                q[0] = 0.0; q[1] = Qk; q[2] = Qk; q[3] = Qk; q[4] = Qnu; q[5] = Qnu; q[6] = Qnu;
+                np[0] = nm[0] - q[0] * (nm[0] - neq[0]);
+                np[1] = nm[1] - q[1] * (nm[1] - neq[1]) + SG * dBx;
+                np[2] = nm[2] - q[2] * (nm[2] - neq[2]) + SG * dBy;
+                np[3] = nm[3] - q[3] * (nm[3] - neq[3]) + SG * dBz;
+                np[4] = nm[4] - q[4] * (nm[4] - neq[4]);
+                np[5] = nm[5] - q[5] * (nm[5] - neq[5]);
+                np[6] = nm[6] - q[6] * (nm[6] - neq[6]);
+                double *P = s->gpost + HDX(s, i, j, k) * 7;
+                P[0] = np[0] / 7.0 - np[4] / 7.0;
+                P[1] = np[0] / 7.0 + np[1] / 2.0 + np[4] / 42.0 + np[5] / 6.0;
+                P[2] = np[0] / 7.0 - np[1] / 2.0 + np[4] / 42.0 + np[5] / 6.0;
+                P[3] = np[0] / 7.0 + np[2] / 2.0 + np[4] / 42.0 - np[5] / 12.0 + np[6] / 4.0;
+                P[4] = np[0] / 7.0 - np[2] / 2.0 + np[4] / 42.0 - np[5] / 12.0 + np[6] / 4.0;
+                P[5] = np[0] / 7.0 + np[3] / 2.0 + np[4] / 42.0 - np[5] / 12.0 - np[6] / 4.0;
+                P[6] = np[0] / 7.0 - np[3] / 2.0 + np[4] / 42.0 - np[5] / 12.0 - np[6] / 4.0;
+            }
+}
+
+/* bouncebackT, 3DRB.F90:1243-1351 (non-legacy branch).  Sweep order: x periodic, x const-T,
+ * x adiabatic, y adiabatic, y const-T, z periodic, z adiabatic.  Hot wall at the low index. */
+static void bouncebackT3d(orc_state *s)
+{
+    double *g = s->g;
+    const double *P = s->gpost;
+    const double *wT = s->omegaT;
+    const double Th = s->c.thot, Tc = s->c.tcold;
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int a = 0; a < 7; ++a) {
+                long lo = IDX(s, 0, j, k) * 7 + a, hi = IDX(s, NX - 1, j, k) * 7 + a;
+                if (s->c.temp_x == ORC_T_PERIODIC) {
+                    if (EX3[a] == 1) g[lo] = P[HDX(s, NX - 1, j, k) * 7 + a];
+                    if (EX3[a] == -1) g[hi] = P[HDX(s, 0, j, k) * 7 + a];
+                } else if (s->c.temp_x == ORC_T_CONST) {
+                    if (EX3[a] == 1) g[lo] = -P[HDX(s, 0, j, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Th;
+                    if (EX3[a] == -1) g[hi] = -P[HDX(s, NX - 1, j, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Tc;
+                } else {
+                    if (EX3[a] == 1) g[lo] = P[HDX(s, 0, j, k) * 7 + OPP7[a]];
+                    if (EX3[a] == -1) g[hi] = P[HDX(s, NX - 1, j, k) * 7 + OPP7[a]];
+                }
+            }
+    for (int k = 0; k < NZ; ++k)
+        for (int i = 0; i < NX; ++i)
+            for (int a = 0; a < 7; ++a) {
+                long lo = IDX(s, i, 0, k) * 7 + a, hi = IDX(s, i, NY - 1, k) * 7 + a;
+                if (s->c.temp_y == ORC_T_ADIABATIC) {
+                    if (EY3[a] == 1) g[lo] = P[HDX(s, i, 0, k) * 7 + OPP7[a]];
+                    if (EY3[a] == -1) g[hi] = P[HDX(s, i, NY - 1, k) * 7 + OPP7[a]];
+                } else if (s->c.temp_y == ORC_T_CONST) {
+                    if (EY3[a] == 1) g[lo] = -P[HDX(s, i, 0, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Th;
+                    if (EY3[a] == -1) g[hi] = -P[HDX(s, i, NY - 1, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Tc;
+                }
+            }
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i)
+            for (int a = 0; a < 7; ++a) {
+                long lo = IDX(s, i, j, 0) * 7 + a, hi = IDX(s, i, j, NZ - 1) * 7 + a;
+                if (s->c.temp_z == ORC_T_PERIODIC) {
+                    if (EZ3[a] == 1) g[lo] = P[HDX(s, i, j, NZ - 1) * 7 + a];
+                    if (EZ3[a] == -1) g[hi] = P[HDX(s, i, j, 0) * 7 + a];
+                } else {
+                    if (EZ3[a] == 1) g[lo] = P[HDX(s, i, j, 0) * 7 + OPP7[a]];
+                    if (EZ3[a] == -1) g[hi] = P[HDX(s, i, j, NZ - 1) * 7 + OPP7[a]];
+                }
+            }
+}
+
+/* macroT, 3DRB.F90:1362-1381 */
+static void macroT3d(orc_state *s)
+{
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < s->n; ++n) {
+        const double *G = s->g + n * 7;
+        s->T[n] = G[0] + G[1] + G[2] + G[3] + G[4] + G[5] + G[6];
+    }
+}
+
+/* ========================================================================================= */
+/* 2D: D2Q9 MRT flow + D2Q5 MRT temperature                                                  */
+/* ========================================================================================= */
+
+/* collision, 2DRB.F90:675-766.  Stores Fx, Fy (old rho, old T) for macro. */
+static void collision2d(orc_state *s)
+{
+    const orc_params *p = &s->p;
+    const double gBeta = p->gbeta, Tref = p->Tref, Snu = p->Snu, Sq = p->Sq;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
+            const long n = IDX(s, i, j, 0);
+            const double *F = s->f + n * 9;
+            double m[9], meq[9], mp[9], sr[9], src[9];
+            const double rho = s->rho[n], u = s->u[n], v = s->v[n];
+            m[0] = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8];
+            m[1] = -4.0 * F[0] - F[1] - F[2] - F[3] - F[4] + 2.0 * (F[5] + F[6] + F[7] + F[8]);
+            m[2] = 4.0 * F[0] - 2.0 * (F[1] + F[2] + F[3] + F[4]) + F[5] + F[6] + F[7] + F[8];
+            m[3] = F[1] - F[3] + F[5] - F[6] - F[7] + F[8];
+            m[4] = -2.0 * F[1] + 2.0 * F[3] + F[5] - F[6] - F[7] + F[8];
+            m[5] = F[2] - F[4] + F[5] + F[6] - F[7] - F[8];
+            m[6] = -2.0 * F[2] + 2.0 * F[4] + F[5] + F[6] - F[7] - F[8];
+            m[7] = F[1] - F[2] + F[3] - F[4];
+            m[8] = F[5] - F[6] + F[7] - F[8];
+
+            meq[0] = rho;
+            meq[1] = rho * (-2.0 + 3.0 * (u * u + v * v));
+            meq[2] = rho * (1.0 - 3.0 * (u * u + v * v));
+            meq[3] = rho * u;
+            meq[4] = -rho * u;
+            meq[5] = rho * v;
+            meq[6] = -rho * v;
+            meq[7] = rho * (u * u - v * v);
+            meq[8] = rho * (u * v);
+
+            sr[0] = 0.0; sr[1] = Snu; sr[2] = Snu; sr[3] = 0.0; sr[4] = Sq; sr[5] = 0.0;
+            sr[6] = Sq; sr[7] = Snu; sr[8] = Snu;
+
+            const double Fx = 0.0;
+            const double Fy = rho * gBeta * (s->T[n] - Tref);
+            s->Fx[n] = Fx;
+            s->Fy[n] = Fy;
+
+            src[0] = 0.0;
+            src[1] = (6.0 - 3.0 * sr[1]) * (u * Fx + v * Fy);
+            src[2] = -(6.0 - 3.0 * sr[2]) * (u * Fx + v * Fy);
+            src[3] = (1.0 - 0.5 * sr[3]) * Fx;
+            src[4] = -(1.0 - 0.5 * sr[4]) * Fx;
+            src[5] = (1.0 - 0.5 * sr[5]) * Fy;
+            src[6] = -(1.0 - 0.5 * sr[6]) * Fy;
+            src[7] = (2.0 - sr[7]) * (u * Fx - v * Fy);
+            src[8] = (1.0 - 0.5 * sr[8]) * (u * Fy + v * Fx);
+
+            for (int a = 0; a < 9; ++a) mp[a] = m[a] - sr[a] * (m[a] - meq[a]) + src[a];
+
+            double *P = s->fpost + HDX(s, i, j, -1) * 9;
+            P[0] = mp[0] / 9.0 - mp[1] / 9.0 + mp[2] / 9.0;
+            P[1] = mp[0] / 9.0 - mp[1] / 36.0 - mp[2] / 18.0 + mp[3] / 6.0 - mp[4] / 6.0 + mp[7] / 4.0;
+            P[2] = mp[0] / 9.0 - mp[1] / 36.0 - mp[2] / 18.0 + mp[5] / 6.0 - mp[6] / 6.0 - mp[7] / 4.0;
+            P[3] = mp[0] / 9.0 - mp[1] / 36.0 - mp[2] / 18.0 - mp[3] / 6.0 + mp[4] / 6.0 + mp[7] / 4.0;
+            P[4] = mp[0] / 9.0 - mp[1] / 36.0 - mp[2] / 18.0 - mp[5] / 6.0 + mp[6] / 6.0 - mp[7] / 4.0;
+            P[5] = mp[0] / 9.0 + mp[1] / 18.0 + mp[2] / 36.0 + mp[3] / 6.0 + mp[4] / 12.0 +
+                   mp[5] / 6.0 + mp[6] / 12.0 + mp[8] / 4.0;
+            P[6] = mp[0] / 9.0 + mp[1] / 18.0 + mp[2] / 36.0 - mp[3] / 6.0 - mp[4] / 12.0 +
+                   mp[5] / 6.0 + mp[6] / 12.0 - mp[8] / 4.0;
+            P[7] = mp[0] / 9.0 + mp[1] / 18.0 + mp[2] / 36.0 - mp[3] / 6.0 - mp[4] / 12.0 -
+                   mp[5] / 6.0 - mp[6] / 12.0 + mp[8] / 4.0;
+            P[8] = mp[0] / 9.0 + mp[1] / 18.0 + mp[2] / 36.0 + mp[3] / 6.0 + mp[4] / 12.0 -
+                   mp[5] / 6.0 - mp[6] / 12.0 - mp[8] / 4.0;
+        }
+}
+
+/* streaming, 2DRB.F90:777-798; streamingT, 2DRB.F90:993-1014 */
+static void streaming2d(orc_state *s, double *dst, const double *src, int q)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i)
+            for (int a = 0; a < q; ++a)
+                dst[IDX(s, i, j, 0) * q + a] = src[HDX(s, i - EX2[a], j - EY2[a], -1) * q + a];
+}
+
+/* bounceback, 2DRB.F90:811-866: x periodic (same row) or x no-slip, then y no-slip. */
+static void bounceback2d(orc_state *s)
+{
+    double *f = s->f;
+    const double *P = s->fpost;
+#define F2(i, j, a) f[IDX(s, (i), (j), 0) * 9 + (a)]
+#define P2(i, j, a) P[HDX(s, (i), (j), -1) * 9 + (a)]
+    for (int j = 0; j < NY; ++j) {
+        if (s->c.vel_x == ORC_VEL_PERIODIC) {
+            F2(0, j, 1) = P2(NX - 1, j, 1);
+            F2(0, j, 5) = P2(NX - 1, j, 5);
+            F2(0, j, 8) = P2(NX - 1, j, 8);
+            F2(NX - 1, j, 3) = P2(0, j, 3);
+            F2(NX - 1, j, 6) = P2(0, j, 6);
+            F2(NX - 1, j, 7) = P2(0, j, 7);
+        } else {
+            F2(0, j, 1) = P2(0, j, 3);
+            F2(0, j, 5) = P2(0, j, 7);
+            F2(0, j, 8) = P2(0, j, 6);
+            F2(NX - 1, j, 3) = P2(NX - 1, j, 1);
+            F2(NX - 1, j, 6) = P2(NX - 1, j, 8);
+            F2(NX - 1, j, 7) = P2(NX - 1, j, 5);
+        }
+    }
+    for (int i = 0; i < NX; ++i) {
+        F2(i, 0, 2) = P2(i, 0, 4);
+        F2(i, 0, 5) = P2(i, 0, 7);
+        F2(i, 0, 6) = P2(i, 0, 8);
+        F2(i, NY - 1, 4) = P2(i, NY - 1, 2);
+        F2(i, NY - 1, 7) = P2(i, NY - 1, 5);
+        F2(i, NY - 1, 8) = P2(i, NY - 1, 6);
+    }
+#undef F2
+#undef P2
+}
+
+/* macro, 2DRB.F90:878-894: half force from the Fx, Fy stored by collision. */
+static void macro2d(orc_state *s)
+{
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < s->n; ++n) {
+        const double *F = s->f + n * 9;
+        double rho = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8];
+        s->rho[n] = rho;
+        s->u[n] = (F[1] - F[3] + F[5] - F[6] - F[7] + F[8] + 0.5 * s->Fx[n]) / rho;
+        s->v[n] = (F[2] - F[4] + F[5] + F[6] - F[7] - F[8] + 0.5 * s->Fy[n]) / rho;
+    }
+}
+
+/* collisionT, 2DRB.F90:906-980 (non-legacy branch). */
+static void collisionT2d(orc_state *s)
+{
+    const double Qk = s->p.Qk, Qnu = s->p.Qnu, SG = s->p.SG;
+    const int useG = s->c.use_g;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < NY; ++j)
+        for (int i = 0; i < NX; ++i) {
+            const long n = IDX(s, i, j, 0);
+            const double *G = s->g + n * 5;
+            double nm[5], neq[5], q[5], np[5];
+            const double Tl = s->T[n];
+            const double Bx = s->u[n] * Tl, By = s->v[n] * Tl;
+            double dBx = 0.0, dBy = 0.0;
+            if (useG) {
+                dBx = Bx - s->bx[n];
+                dBy = By - s->by[n];
+                s->bx[n] = Bx;
+                s->by[n] = By;
+            }
+            nm[0] = G[0] + G[1] + G[2] + G[3] + G[4];
+            nm[1] = G[1] - G[3];
+            nm[2] = G[2] - G[4];
+            nm[3] = -4.0 * G[0] + G[1] + G[2] + G[3] + G[4];
+            nm[4] = G[1] - G[2] + G[3] - G[4];
+            neq[0] = Tl;
+            neq[1] = Tl * s->u[n];
+            neq[2] = Tl * s->v[n];
+            neq[3] = Tl * (-2.0 / 3.0);
+            neq[4] = 0.0;
+            q[0] = 0.0; q[1] = Qk; q[2] = Qk; q[3] = Qnu; q[4] = Qnu;
+            np[0] = nm[0] - q[0] * (nm[0] - neq[0]);
+            np[1] = nm[1] - q[1] * (nm[1] - neq[1]) + SG * dBx;
+            np[2] = nm[2] - q[2] * (nm[2] - neq[2]) + SG * dBy;
+            np[3] = nm[3] - q[3] * (nm[3] - neq[3]);
+            np[4] = nm[4] - q[4] * (nm[4] - neq[4]);
+            double *P = s->gpost + HDX(s, i, j, -1) * 5;
+            P[0] = 0.2 * np[0] - 0.2 * np[3];
+            P[1] = 0.2 * np[0] + 0.5 * np[1] + 0.05 * np[3] + 0.25 * np[4];
+            P[2] = 0.2 * np[0] + 0.5 * np[2] + 0.05 * np[3] - 0.25 * np[4];
+            P[3] = 0.2 * np[0] - 0.5 * np[1] + 0.05 * np[3] + 0.25 * np[4];
+            P[4] = 0.2 * np[0] - 0.5 * np[2] + 0.05 * np[3] - 0.25 * np[4];
+        }
+}
+
+/* bouncebackT, 2DRB.F90:1026-1107 (non-legacy branch). */
+static void bouncebackT2d(orc_state *s)
+{
+    double *g = s->g;
+    const double *P = s->gpost;
+    const double *wT = s->omegaT;
+    const double Th = s->c.thot, Tc = s->c.tcold;
+#define G2(i, j, a) g[IDX(s, (i), (j), 0) * 5 + (a)]
+#define Q2(i, j, a) P[HDX(s, (i), (j), -1) * 5 + (a)]
+    for (int j = 0; j < NY; ++j) {
+        if (s->c.temp_x == ORC_T_PERIODIC) {
+            G2(0, j, 1) = Q2(NX - 1, j, 1);
+            G2(NX - 1, j, 3) = Q2(0, j, 3);
+        } else if (s->c.temp_x == ORC_T_CONST) {
+            G2(0, j, 1) = -Q2(0, j, 3) + 2.0 * wT[3] * Th;
+            G2(NX - 1, j, 3) = -Q2(NX - 1, j, 1) + 2.0 * wT[1] * Tc;
+        } else {
+            G2(0, j, 1) = Q2(0, j, 3);
+            G2(NX - 1, j, 3) = Q2(NX - 1, j, 1);
+        }
+    }
+    for (int i = 0; i < NX; ++i) {
+        if (s->c.temp_y == ORC_T_ADIABATIC) {
+            G2(i, 0, 2) = Q2(i, 0, 4);
+            G2(i, NY - 1, 4) = Q2(i, NY - 1, 2);
+        } else if (s->c.temp_y == ORC_T_CONST) {
+            G2(i, 0, 2) = -Q2(i, 0, 4) + 2.0 * wT[4] * Th;
+            G2(i, NY - 1, 4) = -Q2(i, NY - 1, 2) + 2.0 * wT[2] * Tc;
+        }
+    }
+#undef G2
+#undef Q2
+}
+
+/* macroT, 2DRB.F90:1120-1134 */
+static void macroT2d(orc_state *s)
+{
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < s->n; ++n) {
+        const double *G = s->g + n * 5;
+        s->T[n] = G[0] + G[1] + G[2] + G[3] + G[4];
+    }
+}
+
+/* One pass of the reference time loop body (3DRB.F90:227-243, 2DRB.F90:238-254). */
+void orc_step(orc_state *s, int nsteps)
+{
+    for (int it = 0; it < nsteps; ++it) {
+        s->itc += 1;
+        if (s->c.dims == 3) {
+            collision3d(s);
+            streaming3d(s, s->f, s->fpost, 19);
+            bounceback3d(s);
+            macro3d(s);
+            collisionT3d(s);
+            streaming3d(s, s->g, s->gpost, 7);
+            bouncebackT3d(s);
+            macroT3d(s);
+        } else {
+            collision2d(s);
+            streaming2d(s, s->f, s->fpost, 9);
+            bounceback2d(s);
+            macro2d(s);
+            collisionT2d(s);
+            streaming2d(s, s->g, s->gpost, 5);
+            bouncebackT2d(s);
+            macroT2d(s);
+        }
+    }
+}
+
+void orc_get(const orc_state *s, double *f, double *g, double *rho, double *u, double *v,
+             double *w, double *T, double *bx, double *by, double *bz)
+{
+    size_t n = (size_t)s->n;
+    if (f) memcpy(f, s->f, n * s->qf * sizeof(double));
+    if (g) memcpy(g, s->g, n * s->qt * sizeof(double));
+    if (rho) memcpy(rho, s->rho, n * sizeof(double));
+    if (u) memcpy(u, s->u, n * sizeof(double));
+    if (v) memcpy(v, s->v, n * sizeof(double));
+    if (w) memcpy(w, s->w, n * sizeof(double));
+    if (T) memcpy(T, s->T, n * sizeof(double));
+    if (bx) memcpy(bx, s->bx, n * sizeof(double));
+    if (by) memcpy(by, s->by, n * sizeof(double));
+    if (bz) memcpy(bz, s->bz, n * sizeof(double));
+}
+
+/* reconstruct_macro_from_fg, 3DRB.F90:1392-1463 and 2DRB.F90:1144-1200: macroT, then rho,
+ * momentum and three fixed-point passes of u = (sum e f + F/2)/rho, then B_prev = u T. */
+int orc_set_fg(orc_state *s, const double *f, const double *g)
+{
+    const double gBeta = s->p.gbeta, Tref = s->p.Tref;
+    memcpy(s->f, f, (size_t)s->n * s->qf * sizeof(double));
+    memcpy(s->g, g, (size_t)s->n * s->qt * sizeof(double));
+    if (s->c.dims == 3) macroT3d(s);
+    else macroT2d(s);
+    int bad = 0;
+    for (long n = 0; n < s->n; ++n) {
+        const double *F = s->f + n * s->qf;
+        double rho, momx, momy, momz = 0.0;
+        if (s->c.dims == 3) {
+            rho = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8] + F[9] + F[10] +
+                  F[11] + F[12] + F[13] + F[14] + F[15] + F[16] + F[17] + F[18];
+            momx = F[1] - F[2] + F[7] - F[8] + F[9] - F[10] + F[11] - F[12] + F[13] - F[14];
+            momy = F[3] - F[4] + F[7] + F[8] - F[9] - F[10] + F[15] - F[16] + F[17] - F[18];
+            momz = F[5] - F[6] + F[11] + F[12] - F[13] - F[14] + F[15] + F[16] - F[17] - F[18];
+        } else {
+            rho = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8];
+            momx = F[1] - F[3] + F[5] - F[6] - F[7] + F[8];
+            momy = F[2] - F[4] + F[5] + F[6] - F[7] - F[8];
+        }
+        s->rho[n] = rho;
+        if (rho > 0.0) {
+            s->u[n] = momx / rho;
+            s->v[n] = momy / rho;
+            s->w[n] = momz / rho;
+            for (int iter = 0; iter < 3; ++iter) {
+                double FxLoc = 0.0, FyLoc = rho * gBeta * (s->T[n] - Tref), FzLoc = 0.0;
+                s->u[n] = (momx + 0.5 * FxLoc) / rho;
+                s->v[n] = (momy + 0.5 * FyLoc) / rho;
+                s->w[n] = (momz + 0.5 * FzLoc) / rho;
+                if (s->c.dims == 2) {
+                    s->Fx[n] = FxLoc;
+                    s->Fy[n] = FyLoc;
+                }
+            }
+        } else {
+            bad = 1;
+            s->u[n] = s->v[n] = s->w[n] = 0.0;
+            s->Fx[n] = s->Fy[n] = 0.0;
+        }
+        if (s->c.use_g) {
+            s->bx[n] = s->u[n] * s->T[n];
+            s->by[n] = s->v[n] * s->T[n];
+            s->bz[n] = s->w[n] * s->T[n];
+        } else if (s->c.dims == 3) {
+            s->bx[n] = s->by[n] = s->bz[n] = 0.0;
+        }
+    }
+    return bad ? -1 : 0;
+}
+
+/* check, 3DRB.F90:1475-1528 (3D keeps a 1e-30 guard; 2D, 2DRB.F90:1213-1260, has none). */
+void orc_check(orc_state *s, double *error_u, double *error_t)
+{
+    ksum e1 = {0, 0}, e2 = {0, 0}, e5 = {0, 0}, e6 = {0, 0};
+    for (int k = 0; k < NZ; ++k) {
+        ksum p1 = {0, 0}, p2 = {0, 0}, p5 = {0, 0}, p6 = {0, 0};
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                long n = IDX(s, i, j, k);
+                double du = s->u[n] - s->up[n], dv = s->v[n] - s->vp[n], dw = s->w[n] - s->wp[n];
+                double a = du * du + dv * dv, b = s->u[n] * s->u[n] + s->v[n] * s->v[n];
+                if (s->c.dims == 3) {
+                    a = a + dw * dw;
+                    b = b + s->w[n] * s->w[n];
+                }
+                ks_add(&p1, a);
+                ks_add(&p2, b);
+                ks_add(&p5, fabs(s->T[n] - s->Tp[n]));
+                ks_add(&p6, fabs(s->T[n]));
+                s->up[n] = s->u[n];
+                s->vp[n] = s->v[n];
+                s->wp[n] = s->w[n];
+                s->Tp[n] = s->T[n];
+            }
+        ks_merge(&e1, p1);
+        ks_merge(&e2, p2);
+        ks_merge(&e5, p5);
+        ks_merge(&e6, p6);
+    }
+    double error1 = ks_val(e1), error2 = ks_val(e2), error5 = ks_val(e5), error6 = ks_val(e6);
+    if (s->c.dims == 3) {
+        *error_u = (error2 > 1.0e-30) ? sqrt(error1) / sqrt(error2) : sqrt(error1);
+        *error_t = (error6 > 1.0e-30) ? error5 / error6 : error5;
+    } else {
+        *error_u = sqrt(error1) / sqrt(error2);
+        *error_t = error5 / error6;
+    }
+}
+
+/* fit_adiabatic_wall_T4, 3DRB.F90:2212-2241: least-squares T = a + b (y - y0)^2, returns a. */
+void orc_fit_adiabatic_wall_T4(double y0, const double *y, const double *tt, double *t_wall)
+{
+    double sq[4], S0 = 4.0, S1 = 0.0, S2 = 0.0, B0 = 0.0, B1 = 0.0;
+    for (int k = 0; k < 4; ++k) sq[k] = (y[k] - y0) * (y[k] - y0);
+    for (int k = 0; k < 4; ++k) {
+        S1 = S1 + sq[k];
+        S2 = S2 + sq[k] * sq[k];
+        B0 = B0 + tt[k];
+        B1 = B1 + tt[k] * sq[k];
+    }
+    double D = S0 * S2 - S1 * S1;
+    *t_wall = (B0 * S2 - B1 * S1) / D;
+}
+
+/* fit_parabola_ls5, 3DRB.F90:2252-2334: 5-point least-squares parabola, vertex if interior. */
+void orc_fit_parabola_ls5(const double *y, const double *f, int mode, double *fstar,
+                          double *ystar)
+{
+    const double epsD = 1.0e-20, epsA = 1.0e-14;
+    int kbest = 0;
+    for (int k = 1; k < 5; ++k) {
+        if (mode == 1) {
+            if (f[k] > f[kbest]) kbest = k;
+        } else {
+            if (f[k] < f[kbest]) kbest = k;
+        }
+    }
+    double S0 = 0, S1 = 0, S2 = 0, S3 = 0, S4 = 0, F0 = 0, F1 = 0, F2 = 0;
+    for (int k = 0; k < 5; ++k) {
+        S0 = S0 + 1.0;
+        S1 = S1 + y[k];
+        S2 = S2 + y[k] * y[k];
+        S3 = S3 + y[k] * y[k] * y[k];
+        S4 = S4 + y[k] * y[k] * y[k] * y[k];
+        F0 = F0 + f[k];
+        F1 = F1 + f[k] * y[k];
+        F2 = F2 + f[k] * y[k] * y[k];
+    }
+    double D = S0 * (S2 * S4 - S3 * S3) - S1 * (S1 * S4 - S2 * S3) + S2 * (S1 * S3 - S2 * S2);
+    double DA = F0 * (S2 * S4 - S3 * S3) - S1 * (F1 * S4 - S3 * F2) + S2 * (F1 * S3 - S2 * F2);
+    double DB = S0 * (F1 * S4 - S3 * F2) - F0 * (S1 * S4 - S2 * S3) + S2 * (S1 * F2 - F1 * S2);
+    double DC = S0 * (S2 * F2 - F1 * S3) - S1 * (S1 * F2 - F1 * S2) + F0 * (S1 * S3 - S2 * S2);
+    if (fabs(D) <= epsD) {
+        *ystar = y[kbest];
+        *fstar = f[kbest];
+        return;
+    }
+    double A = DA / D, B = DB / D, C = DC / D;
+    double ymin = y[0], ymax = y[0];
+    for (int k = 1; k < 5; ++k) {
+        if (y[k] < ymin) ymin = y[k];
+        if (y[k] > ymax) ymax = y[k];
+    }
+    if (fabs(C) <= epsA || (mode == 1 && C >= 0.0) || (mode == -1 && C <= 0.0)) {
+        *ystar = y[kbest];
+        *fstar = f[kbest];
+        return;
+    }
+    double ys = -B / (2.0 * C);
+    if (ys < ymin || ys > ymax) {
+        *ystar = y[kbest];
+        *fstar = f[kbest];
+        return;
+    }
+    *ystar = ys;
+    *fstar = A + B * ys + C * ys * ys;
+}
+
+/* Diagnostics: calNuRe (3DRB.F90:1712-1778, 2DRB.F90:1610-1675), RBcalc_Nu_global
+ * (3DRB.F90:1839-1878, 2DRB.F90:2251-2297), RBcalc_Nu_wall_avg (3DRB.F90:2049-2201,
+ * 2DRB.F90:2308-2473), plus mass, heat content and kinetic energy. */
+void orc_diagnostics(const orc_state *s, orc_diag *d)
+{
+    const orc_params *p = &s->p;
+    const double Th = s->c.thot, Tc = s->c.tcold, Tref = p->Tref, L = p->L;
+    const double dy = 1.0 / L, deltaT = Th - Tc, coef = L / p->diffusivity;
+    const double N = (double)s->n;
+    const int nx = NX, ny = NY, nz = NZ;
+    memset(d, 0, sizeof(*d));
+
+    ksum snu = {0, 0}, sre = {0, 0}, sqy = {0, 0}, sm = {0, 0}, sh = {0, 0}, ske = {0, 0};
+    for (int k = 0; k < nz; ++k) {
+        ksum a = {0, 0}, b = {0, 0}, c = {0, 0}, m = {0, 0}, h = {0, 0}, e = {0, 0};
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                long n = IDX(s, i, j, k);
+                double u = s->u[n], v = s->v[n], w = s->w[n], T = s->T[n];
+                ks_add(&a, v * T);
+                double q2 = u * u + v * v;
+                if (s->c.dims == 3) q2 = q2 + w * w;
+                ks_add(&b, q2);
+                double dTdy;
+                if (j == 0) dTdy = (3.0 * T + s->T[IDX(s, i, 1, k)] - 4.0 * Th) / (3.0 * dy);
+                else if (j == ny - 1)
+                    dTdy = (4.0 * Tc - 3.0 * T - s->T[IDX(s, i, ny - 2, k)]) / (3.0 * dy);
+                else dTdy = (s->T[IDX(s, i, j + 1, k)] - s->T[IDX(s, i, j - 1, k)]) / (2.0 * dy);
+                ks_add(&c, coef * v * (T - Tref) - dTdy);
+                ks_add(&m, s->rho[n]);
+                ks_add(&h, T);
+                ks_add(&e, 0.5 * s->rho[n] * q2);
+            }
+        ks_merge(&snu, a);
+        ks_merge(&sre, b);
+        ks_merge(&sqy, c);
+        ks_merge(&sm, m);
+        ks_merge(&sh, h);
+        ks_merge(&ske, e);
+    }
+    d->nu_vol = ks_val(snu) / N * L / p->diffusivity + 1.0;
+    d->re_vol = sqrt(ks_val(sre) / N) * L / p->viscosity;
+    d->nu_global = (ks_val(sqy) / N) / deltaT;
+    d->mass = ks_val(sm);
+    d->heat = ks_val(sh);
+    d->kinetic = ks_val(ske);
+
+    /* Hot-wall profile: per-x averages over z of T(i,1,k) and of the wall flux. */
+    double *Nu_ext = (double *)calloc((size_t)nx + 2, sizeof(double));
+    double *Tbot = (double *)calloc((size_t)nx + 2, sizeof(double));
+    ksum shot = {0, 0}, scold = {0, 0}, smid = {0, 0};
+    for (int i = 0; i < nx; ++i) {
+        ksum tb = {0, 0}, nb = {0, 0};
+        for (int k = 0; k < nz; ++k) {
+            ks_add(&tb, s->T[IDX(s, i, 0, k)]);
+            double qy = (8.0 * Th - 9.0 * s->T[IDX(s, i, 0, k)] + s->T[IDX(s, i, 1, k)]) / (3.0 * dy);
+            ks_add(&nb, qy / deltaT);
+            double qc = (-8.0 * Tc + 9.0 * s->T[IDX(s, i, ny - 1, k)] - s->T[IDX(s, i, ny - 2, k)]) /
+                        (3.0 * dy);
+            ks_add(&scold, qc / deltaT);
+            double mid;
+            if (ny % 2 == 1) {
+                int jm = (ny + 1) / 2 - 1;
+                long nmid = IDX(s, i, jm, k);
+                mid = (coef * s->v[nmid] * (s->T[nmid] - Tref) -
+                       (s->T[IDX(s, i, jm + 1, k)] - s->T[IDX(s, i, jm - 1, k)]) / (2.0 * dy)) /
+                      deltaT;
+            } else {
+                int jb = ny / 2 - 1, jt = jb + 1;
+                long nb_ = IDX(s, i, jb, k), nt_ = IDX(s, i, jt, k);
+                mid = (coef * 0.5 * (s->v[nb_] * (s->T[nb_] - Tref) + s->v[nt_] * (s->T[nt_] - Tref)) +
+                       (s->T[nb_] - s->T[nt_]) / dy) /
+                      deltaT;
+            }
+            ks_add(&smid, mid);
+        }
+        Tbot[i + 1] = ks_val(tb) / (double)nz;
+        Nu_ext[i + 1] = ks_val(nb) / (double)nz;
+        ks_add(&shot, Nu_ext[i + 1]);
+    }
+    d->nu_hot = ks_val(shot) / (double)nx;
+    d->nu_cold = ks_val(scold) / ((double)nx * nz);
+    d->nu_middle = ks_val(smid) / ((double)nx * nz);
+
+    const double *xp = s->xp;
+    double xfit[4], Tfit[4], Twl, Twr;
+    for (int m = 0; m < 4; ++m) {
+        xfit[m] = xp[1 + m];
+        Tfit[m] = Tbot[1 + m];
+    }
+    orc_fit_adiabatic_wall_T4(0.0, xfit, Tfit, &Twl);
+    Nu_ext[0] = (2.0 * (Th - Twl) / dy) / deltaT;
+    for (int m = 0; m < 4; ++m) {
+        xfit[m] = xp[nx - 3 + m];
+        Tfit[m] = Tbot[nx - 3 + m];
+    }
+    orc_fit_adiabatic_wall_T4(xp[nx + 1], xfit, Tfit, &Twr);
+    Nu_ext[nx + 1] = (2.0 * (Th - Twr) / dy) / deltaT;
+
+    int imax = 0, imin = 0;
+    double vmax = Nu_ext[0], vmin = Nu_ext[0];
+    for (int i = 1; i <= nx + 1; ++i) {
+        if (Nu_ext[i] > vmax) { vmax = Nu_ext[i]; imax = i; }
+        if (Nu_ext[i] < vmin) { vmin = Nu_ext[i]; imin = i; }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        int ic = pass == 0 ? imax : imin;
+        int ii[5];
+        if (ic <= 2) for (int m = 0; m < 5; ++m) ii[m] = m;
+        else if (ic >= nx - 1) for (int m = 0; m < 5; ++m) ii[m] = nx - 3 + m;
+        else for (int m = 0; m < 5; ++m) ii[m] = ic - 2 + m;
+        double xk[5], fk[5], fs, xs;
+        for (int m = 0; m < 5; ++m) {
+            xk[m] = xp[ii[m]];
+            fk[m] = Nu_ext[ii[m]];
+        }
+        orc_fit_parabola_ls5(xk, fk, pass == 0 ? 1 : -1, &fs, &xs);
+        if (pass == 0) { d->nu_hot_max = fs; d->nu_hot_max_pos = xs; }
+        else { d->nu_hot_min = fs; d->nu_hot_min_pos = xs; }
+    }
+    free(Nu_ext);
+    free(Tbot);
+}

@@ -1,0 +1,226 @@
+"""Pins the CPU oracle (oracle/rb_oracle.c) before it is trusted as the parity checker.
+
+The reference ships no tests, fixtures or golden vectors for this path, and it cannot be built
+in this image (its driver calls GNU Fortran's `ctime` extension, which flang's runtime does not
+provide).  The oracle is therefore pinned by:
+  * an independent matrix-form model of the same scheme (tests/mrt_model.py) on small,
+    non-cubic grids, for every wall combination the reference's switches allow;
+  * closed forms: the dimensionless groups, the least-squares fits against numpy.polyfit;
+  * invariants: mass and heat conservation in closed cavities, the rest state as a fixed point,
+    the conduction state reaching Nu = 1 on every Nusselt definition.
+"""
+import numpy as np
+import pytest
+
+import orc
+from mrt_model import Model, basis_d2q9, basis_d3q19, E2, E3
+
+
+def _rand_T0(cfg, seed, amp=0.3):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-amp, amp, size=cfg.shape)
+
+
+def _pair(cfg, T0):
+    o = orc.Oracle(cfg)
+    o.init_exact(T0)
+    m = Model(cfg.dims, cfg.nx, cfg.ny, cfg.nz, cfg.rayleigh, cfg.prandtl, cfg.mach, cfg.thot,
+              cfg.tcold, cfg.vel_x, cfg.vel_z, cfg.temp_x, cfg.temp_y, cfg.temp_z,
+              bool(cfg.use_g), bool(cfg.side_heated))
+    m.init(T0)
+    return o, m
+
+
+def _compare(o, m, tol):
+    s = o.state()
+    np.testing.assert_allclose(s["f"], m.f, rtol=0, atol=tol)
+    np.testing.assert_allclose(s["g"], m.g, rtol=0, atol=tol)
+    np.testing.assert_allclose(s["T"], m.T, rtol=0, atol=tol)
+    np.testing.assert_allclose(s["rho"], m.rho, rtol=0, atol=tol)
+    np.testing.assert_allclose(s["u"], m.u[..., 0], rtol=0, atol=tol)
+    np.testing.assert_allclose(s["v"], m.u[..., 1], rtol=0, atol=tol)
+    if o.cfg.dims == 3:
+        np.testing.assert_allclose(s["w"], m.u[..., 2], rtol=0, atol=tol)
+    np.testing.assert_allclose(s["bx"], m.B[..., 0], rtol=0, atol=tol)
+    np.testing.assert_allclose(s["by"], m.B[..., 1], rtol=0, atol=tol)
+
+
+# --------------------------------------------------------------------------------------------
+# closed forms
+# --------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dims,n,ra", [(3, 128, 1e6), (3, 512, 1e8), (3, 1024, 1e10), (2, 129, 1e5)])
+def test_params_match_physical_definitions(dims, n, ra):
+    cfg = orc.Config(dims=dims, nx=2 * n - 1 if dims == 2 else n, ny=n, nz=n, rayleigh=ra)
+    p = orc.params(cfg)
+    q = Model(dims, cfg.nx, cfg.ny, cfg.nz, ra).p
+    for k in ("tauf", "viscosity", "diffusivity", "gbeta", "taug", "Qk", "Sq"):
+        assert p[k] == pytest.approx(q[k], rel=1e-12), k
+    # free-fall velocity = Ma * c_s, and Ra, Pr recovered from the lattice transport coefficients
+    assert p["velocity_unit"] == pytest.approx(0.1 / np.sqrt(3), rel=1e-12)
+    assert p["gbeta"] * p["L"] ** 3 / (p["viscosity"] * p["diffusivity"]) == pytest.approx(ra, rel=1e-12)
+    assert p["viscosity"] / p["diffusivity"] == pytest.approx(0.71, rel=1e-14)
+
+
+def test_params_published_values():
+    # tau_f at 512^3, Ra=1e8 and at 1024^3, Ra=1e10 (values quoted with the workload definitions)
+    p = orc.params(orc.Config(nx=512, ny=512, nz=512, rayleigh=1e8))
+    assert p["tauf"] == pytest.approx(0.507472, abs=5e-7)
+    assert p["Qk"] == pytest.approx(1.94540, abs=5e-5)
+    p = orc.params(orc.Config(nx=1024, ny=1024, nz=1024, rayleigh=1e10))
+    assert p["tauf"] == pytest.approx(0.501494, abs=5e-7)
+
+
+def test_moment_bases_orthogonal():
+    for M in (basis_d3q19(E3), basis_d2q9(E2)):
+        G = M @ M.T
+        assert np.allclose(G, np.diag(np.diag(G)))
+
+
+def test_fits_against_numpy_least_squares():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        y = np.sort(rng.uniform(0, 1, 4))
+        t = rng.normal(size=4)
+        y0 = rng.uniform(-0.5, 0.0)
+        # model T = a + b (y - y0)^2, least squares; T_wall = a
+        A = np.stack([np.ones(4), (y - y0) ** 2], axis=1)
+        a = np.linalg.lstsq(A, t, rcond=None)[0][0]
+        assert orc.fit_adiabatic_wall_T4(y0, y, t) == pytest.approx(a, rel=1e-10, abs=1e-12)
+    for mode in (1, -1):
+        for _ in range(20):
+            y = np.sort(rng.uniform(0, 1, 5))
+            c2 = -mode * rng.uniform(0.5, 2.0)
+            yv = rng.uniform(y[1], y[3])
+            f = c2 * (y - yv) ** 2 + 3.0 + 1e-3 * rng.normal(size=5)
+            C, B, A0 = np.polyfit(y, f, 2)
+            fs, ys = orc.fit_parabola_ls5(y, f, mode)
+            yref = -B / (2 * C)
+            if y.min() <= yref <= y.max():
+                assert ys == pytest.approx(yref, rel=1e-9, abs=1e-12)
+                assert fs == pytest.approx(A0 + B * yref + C * yref ** 2, rel=1e-9)
+    # wrong-curvature parabola falls back to the best sample
+    y = np.linspace(0, 1, 5)
+    f = (y - 0.5) ** 2
+    assert orc.fit_parabola_ls5(y, f, 1) == (f[0], y[0])
+
+
+def test_noise_is_uniform_and_deterministic():
+    vals = np.array([orc.noise(20261203, 0, i) for i in range(20000)])
+    assert vals.min() >= -1.0 and vals.max() < 1.0
+    assert abs(vals.mean()) < 0.02
+    assert abs(vals.var() - 1 / 3) < 0.01
+    assert orc.noise(1, 0, 5) == orc.noise(1, 0, 5)
+    assert orc.noise(1, 0, 5) != orc.noise(1, 1, 5)
+
+
+# --------------------------------------------------------------------------------------------
+# oracle vs independent model
+# --------------------------------------------------------------------------------------------
+CASES_3D = {
+    "rb_noslip": dict(),
+    "periodic_xz": dict(vel_x=1, vel_z=1, temp_x=2, temp_z=2),
+    "side_heated": dict(side_heated=1, temp_x=0, temp_y=1, temp_z=1),
+    "periodic_x_walls_z": dict(vel_x=1, temp_x=2),
+    "no_m1g": dict(use_g=0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES_3D))
+def test_oracle_matches_model_3d(name):
+    cfg = orc.Config(dims=3, nx=7, ny=6, nz=5, rayleigh=1e5, mach=0.3, **CASES_3D[name])
+    o, m = _pair(cfg, _rand_T0(cfg, 11))
+    for n in (1, 2, 7):
+        o.step(n)
+        m.step(n)
+        _compare(o, m, 2e-13)
+
+
+CASES_2D = {
+    "rb_periodic": dict(vel_x=1, temp_x=2),
+    "rb_walls": dict(vel_x=0, temp_x=1),
+    "side_heated": dict(side_heated=1, vel_x=0, temp_x=0, temp_y=1),
+    "no_m1g": dict(vel_x=1, temp_x=2, use_g=0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES_2D))
+def test_oracle_matches_model_2d(name):
+    cfg = orc.Config(dims=2, nx=9, ny=6, rayleigh=1e5, mach=0.3, thot=1.0, tcold=0.0,
+                     **CASES_2D[name])
+    o, m = _pair(cfg, _rand_T0(cfg, 5) + 0.5)
+    for n in (1, 2, 7):
+        o.step(n)
+        m.step(n)
+        _compare(o, m, 2e-13)
+
+
+# --------------------------------------------------------------------------------------------
+# invariants
+# --------------------------------------------------------------------------------------------
+def test_mass_conserved_closed_cavity():
+    cfg = orc.Config(dims=3, nx=8, ny=7, nz=6, rayleigh=1e5, mach=0.3)
+    o = orc.Oracle(cfg)
+    o.init_exact(_rand_T0(cfg, 2))
+    m0 = o.diagnostics()["mass"]
+    o.step(50)
+    assert o.diagnostics()["mass"] == pytest.approx(m0, rel=1e-13)
+
+
+def test_heat_conserved_adiabatic_box():
+    cfg = orc.Config(dims=3, nx=8, ny=7, nz=6, rayleigh=1e5, mach=0.3, temp_y=orc.T_ADIABATIC)
+    o = orc.Oracle(cfg)
+    o.init_exact(_rand_T0(cfg, 4))
+    h0 = o.diagnostics()["heat"]
+    o.step(50)
+    assert o.diagnostics()["heat"] == pytest.approx(h0, rel=1e-12, abs=1e-12)
+
+
+def test_rest_state_is_fixed_point():
+    cfg = orc.Config(dims=3, nx=6, ny=6, nz=6, temp_y=orc.T_ADIABATIC)
+    o = orc.Oracle(cfg)
+    o.init_exact(np.zeros(cfg.shape))          # T = Tref everywhere: no buoyancy
+    s0 = o.state()
+    o.step(20)
+    s1 = o.state()
+    np.testing.assert_allclose(s1["f"], s0["f"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(s1["g"], s0["g"], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("dims,periodic", [(2, 0), (2, 1), (3, 0), (3, 1)])
+def test_conduction_state_gives_unit_nusselt(dims, periodic):
+    # Far below the onset of convection the linear profile between the anti-bounce-back walls
+    # is the steady state and every Nusselt definition reads 1.  The buoyancy-driven spurious
+    # velocity of the discrete hydrostatic state scales with g*beta = Ma^2/(3L), so a small Mach
+    # number (at fixed tau_f) leaves Nu - 1 at the 1e-8 level.
+    cfg = orc.Config(dims=dims, nx=8, ny=12, nz=6, rayleigh=0.01, mach=0.001,
+                     vel_x=periodic, temp_x=2 if periodic else 1)
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    o.step(12000)
+    d = o.diagnostics()
+    for k in ("nu_hot", "nu_cold", "nu_global", "nu_middle", "nu_vol"):
+        assert d[k] == pytest.approx(1.0, abs=5e-8), (k, d[k])
+    assert d["re_vol"] < 2e-6
+
+
+def test_restart_reconstruction_roundtrip():
+    cfg = orc.Config(dims=3, nx=7, ny=6, nz=5, rayleigh=1e5, mach=0.3)
+    o = orc.Oracle(cfg)
+    o.init_exact(_rand_T0(cfg, 9))
+    o.step(5)
+    s = o.state()
+    r = orc.Oracle(cfg)
+    r.init_exact(np.zeros(cfg.shape))
+    assert r.set_fg(s["f"], s["g"]) == 0
+    t = r.state()
+    np.testing.assert_array_equal(t["T"], s["T"])
+    np.testing.assert_array_equal(t["rho"], s["rho"])
+    # The reload rebuilds u with the half force of the reloaded T (3DRB.F90:1428-1435), not
+    # with the previous step's T as the in-run macro does.
+    p = orc.params(cfg)
+    jy = (s["f"] * np.array([e[1] for e in E3])).sum(axis=-1)
+    v_expect = (jy + 0.5 * (s["rho"] * p["gbeta"] * (s["T"] - p["Tref"]))) / s["rho"]
+    np.testing.assert_allclose(t["v"], v_expect, rtol=0, atol=1e-15)
+    f = s["f"].copy()
+    f[0, 0, 0, :] = -1.0
+    assert r.set_fg(f, s["g"]) == -1
