@@ -1,0 +1,610 @@
+// lbm_api.hip -- the C ABI of include/lbm_rb.h: handle life cycle, stepping, diagnostics,
+// gather/scatter.  Host code only; the kernels live in lbm_kernels.hip.
+#include "lbm_internal.h"
+#include "../../include/lbm_rb.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+using namespace rblbm;
+
+struct lbm_handle {
+    lbm_config cfg;
+    lbm_params p;
+    KParams kp;
+    int bx;                 // threads per block of the step kernels
+    int qf, qt;
+    uint64_t N;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    double *F[2], *G[2], *Bp, *aux[2], *mac, *prev;
+    double *red;            // reduction partials (device)
+    double *wall;           // per-x wall profiles (device, 2 * nx)
+    int *flag;              // device error flag
+    unsigned red_blocks;
+    int fcur, gcur, fy;     // buffer indices, see k*_ kernels
+    int phase;              // -1: not initialised, 0: plain state, 1: pipelined
+    int first_thermal;      // next thermal pass reads the plain g / T
+    int64_t itc;
+    char err[256];
+};
+
+static int fail(lbm_handle *h, int code, const char *fmt, ...)
+{
+    if (h) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(h->err, sizeof(h->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((h), LBM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+static void compute_params(const lbm_config *c, lbm_params *p)
+{
+    // module commondata3d, 2DRB/3DRB.F90:73-123 (2D: 2DRB/2DRB.F90:61-115)
+    memset(p, 0, sizeof(*p));
+    p->L = c->side_heated ? (double)c->nx : (double)c->ny;
+    p->tref = 0.5 * (c->thot + c->tcold);
+    p->tauf = 0.5 + c->mach * p->L * sqrt(3.0 * c->prandtl / c->rayleigh);
+    p->viscosity = (p->tauf - 0.5) / 3.0;
+    p->diffusivity = p->viscosity / c->prandtl;
+    const double gbeta1 = c->rayleigh * p->viscosity * p->diffusivity / p->L;
+    p->gbeta = gbeta1 / p->L / p->L;
+    p->time_unit = sqrt(p->L / p->gbeta);
+    p->velocity_unit = sqrt(p->gbeta * p->L);
+    p->s_e = p->s_eps = p->s_nu = p->s_pi = 1.0 / p->tauf;
+    p->s_q = p->s_m = 8.0 * (2.0 * p->tauf - 1.0) / (8.0 * p->tauf - 1.0);
+    p->taug = c->dims == 3 ? 0.5 + p->diffusivity / 0.25 : 0.5 + (p->tauf - 0.5) / c->prandtl;
+    p->q_nu = 1.0;
+    p->q_k = 1.0 / p->taug;
+    p->sg = 1.0 - 0.5 * p->q_k;
+}
+
+extern "C" const char *lbm_error_string(int code)
+{
+    switch (code) {
+    case LBM_OK: return "ok";
+    case LBM_ERR_ARG: return "invalid argument";
+    case LBM_ERR_HIP: return "HIP runtime error";
+    case LBM_ERR_NOMEM: return "out of memory";
+    case LBM_ERR_STATE: return "call out of order";
+    case LBM_ERR_UNSUPPORTED: return "unsupported configuration";
+    case LBM_ERR_RHO: return "non-positive density in restart data";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char *lbm_last_error(lbm_handle *h) { return h ? h->err : "null handle"; }
+
+static void release(lbm_handle *h)
+{
+    double **bufs[] = {&h->F[0], &h->F[1], &h->G[0], &h->G[1], &h->Bp, &h->aux[0],
+                       &h->aux[1], &h->mac, &h->prev, &h->red, &h->wall};
+    for (auto b : bufs)
+        if (*b) { (void)hipFree(*b); *b = nullptr; }
+    if (h->flag) { (void)hipFree(h->flag); h->flag = nullptr; }
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    h->ev0 = h->ev1 = nullptr;
+    h->stream = nullptr;
+}
+
+extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
+{
+    if (!cfg || !out) return LBM_ERR_ARG;
+    *out = nullptr;
+    lbm_config c = *cfg;
+    if (c.dims != 2 && c.dims != 3) return LBM_ERR_ARG;
+    if (c.dims == 2) c.nz = 1;
+    if (c.nx < 4 || c.ny < 4 || (c.dims == 3 && c.nz < 4)) return LBM_ERR_ARG;
+    if (c.nx > 65536 || c.ny > 65536 || c.nz > 65536) return LBM_ERR_ARG;
+    if (!(c.rayleigh > 0) || !(c.prandtl > 0) || !(c.mach > 0)) return LBM_ERR_ARG;
+    if (c.vel_x < 0 || c.vel_x > 1 || c.vel_z < 0 || c.vel_z > 1) return LBM_ERR_ARG;
+    if (c.temp_x < 0 || c.temp_x > 2 || c.temp_z < 0 || c.temp_z > 2) return LBM_ERR_ARG;
+    if (c.temp_y < 0 || c.temp_y > 1) return LBM_ERR_ARG;  // no periodic option for y walls
+    if (c.dims == 3 && c.temp_z == LBM_T_CONST) return LBM_ERR_UNSUPPORTED;
+    if (c.precision != LBM_FP64) return LBM_ERR_UNSUPPORTED;
+
+    lbm_handle *h = (lbm_handle *)calloc(1, sizeof(lbm_handle));
+    if (!h) return LBM_ERR_NOMEM;
+    h->cfg = c;
+    compute_params(&c, &h->p);
+    h->qf = c.dims == 3 ? 19 : 9;
+    h->qt = c.dims == 3 ? 7 : 5;
+    h->N = (uint64_t)c.nx * c.ny * c.nz;
+    h->phase = -1;
+    if ((uint64_t)c.nx * c.ny * c.nz >= (1ULL << 32)) { free(h); return LBM_ERR_ARG; }
+
+    KParams &k = h->kp;
+    k.nx = c.nx; k.ny = c.ny; k.nz = c.nz;
+    k.vel_x = c.vel_x; k.vel_z = c.dims == 3 ? c.vel_z : 0;
+    k.temp_x = c.temp_x; k.temp_y = c.temp_y; k.temp_z = c.dims == 3 ? c.temp_z : 1;
+    k.use_g = c.use_g ? 1 : 0;
+    k.N = h->N;
+    k.gbeta = h->p.gbeta; k.tref = h->p.tref; k.thot = c.thot; k.tcold = c.tcold;
+    k.sv = 1.0 / h->p.tauf; k.sq = h->p.s_q;
+    k.qk = h->p.q_k; k.qnu = h->p.q_nu; k.sg = h->p.sg;
+    h->bx = c.nx >= 256 ? 256 : ((c.nx + 63) / 64) * 64;
+    k.nxb = (c.nx + h->bx - 1) / h->bx;
+    const uint64_t nb = (uint64_t)k.nxb * c.ny * c.nz;
+    if (nb >= (1ULL << 31)) { free(h); return LBM_ERR_ARG; }
+    k.nblocks = (unsigned)nb;
+
+    int rc = LBM_OK;
+    hipError_t e = hipSetDevice(c.device);
+    if (e != hipSuccess) { free(h); return LBM_ERR_HIP; }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        release(h);
+        free(h);
+        return LBM_ERR_HIP;
+    }
+    const size_t N = h->N;
+    struct { double **p; size_t n; } plan[] = {
+        {&h->F[0], N * h->qf}, {&h->F[1], N * h->qf}, {&h->G[0], N * h->qt},
+        {&h->G[1], N * h->qt}, {&h->Bp, N * 3},       {&h->aux[0], N},
+        {&h->aux[1], c.dims == 2 ? N : 1},            {&h->mac, N * 5},
+        {&h->wall, (size_t)2 * c.nx},
+    };
+    for (auto &b : plan) {
+        if (hipMalloc((void **)b.p, b.n * sizeof(double)) != hipSuccess) { rc = LBM_ERR_NOMEM; break; }
+    }
+    h->red_blocks = 1024;
+    if (rc == LBM_OK) {
+        size_t nred = (size_t)2 * 8 * (h->red_blocks > (unsigned)c.nx ? h->red_blocks : c.nx);
+        if (hipMalloc((void **)&h->red, nred * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&h->flag, sizeof(int)) != hipSuccess)
+            rc = LBM_ERR_NOMEM;
+    }
+    if (rc != LBM_OK) {
+        release(h);
+        free(h);
+        return rc;
+    }
+    *out = h;
+    return LBM_OK;
+}
+
+extern "C" int lbm_destroy(lbm_handle *h)
+{
+    if (!h) return LBM_ERR_ARG;
+    (void)hipSetDevice(h->cfg.device);
+    (void)hipStreamSynchronize(h->stream);
+    release(h);
+    free(h);
+    return LBM_OK;
+}
+
+static int do_init(lbm_handle *h, const double *T0, uint64_t seed, double amp, bool seeded)
+{
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const lbm_config &c = h->cfg;
+    InitParams ip{};
+    ip.use_T0 = T0 != nullptr;
+    ip.L = h->p.L;
+    ip.pi = acos(-1.0);
+    ip.perturb_amp = 1.0e-3 * (c.thot - c.tcold);
+    ip.rb_perturb = 0;
+    if (!c.side_heated && c.temp_y == LBM_T_CONST) {
+        if (c.dims == 3 && c.rayleigh <= 1.0e4) ip.rb_perturb = 3;   // 2DRB/3DRB.F90:542
+        if (c.dims == 2 && c.rayleigh < 1.0e4) ip.rb_perturb = 2;    // 2DRB/2DRB.F90:568
+    }
+    ip.seed = seed;
+    ip.noise_scale = seeded ? amp * (c.thot - c.tcold) : 0.0;
+    if (T0) HIPCHK(h, hipMemcpyAsync(h->mac + 4 * h->N, T0, h->N * sizeof(double),
+                                     hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, launch_init(h->kp, h->bx, h->stream, ip, h->F[0], h->G[0], h->Bp, h->mac, h->qf, h->qt));
+    if (h->prev) HIPCHK(h, hipMemsetAsync(h->prev, 0, 4 * h->N * sizeof(double), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->phase = 0;
+    h->itc = 0;
+    return LBM_OK;
+}
+
+extern "C" int lbm_init_exact(lbm_handle *h, const double *T0)
+{
+    if (!h) return LBM_ERR_ARG;
+    return do_init(h, T0, 0, 0.0, false);
+}
+
+extern "C" int lbm_init_seeded(lbm_handle *h, uint64_t seed, double amp)
+{
+    if (!h || !(amp >= 0.0)) return LBM_ERR_ARG;
+    return do_init(h, nullptr, seed, amp, true);
+}
+
+static int enqueue_steps(lbm_handle *h, int32_t nsteps)
+{
+    const bool d3 = h->cfg.dims == 3;
+    if (h->phase == 0) {
+        // flow collision of step 1 from the plain state (f^1 in F[0], macros in mac)
+        HIPCHK(h, launch_prologue(h->kp, h->bx, h->stream, h->F[0], h->F[1], h->mac, h->aux[1], d3));
+        h->fcur = 1;
+        h->gcur = 0;
+        h->fy = 1;
+        h->first_thermal = 1;
+        h->phase = 1;
+    }
+    for (int32_t n = 0; n < nsteps; ++n) {
+        const double *Fc = h->F[h->fcur];
+        double *Fn = h->F[h->fcur ^ 1];
+        const double *Gi = h->G[h->gcur];
+        double *Go = h->G[h->gcur ^ 1];
+        if (d3) {
+            HIPCHK(h, launch_step3(h->kp, h->bx, h->stream, h->first_thermal, Fc, Fn, Gi, Go, h->Bp,
+                                   h->aux[0], h->mac));
+        } else {
+            HIPCHK(h, launch_step2(h->kp, h->bx, h->stream, h->first_thermal, Fc, Fn, Gi, Go, h->Bp,
+                                   h->aux[h->fy], h->aux[h->fy ^ 1], h->mac));
+            h->fy ^= 1;
+        }
+        h->first_thermal = 0;
+        h->fcur ^= 1;
+        h->gcur ^= 1;
+        h->itc += 1;
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_step(lbm_handle *h, int32_t nsteps)
+{
+    if (!h || nsteps < 0) return LBM_ERR_ARG;
+    if (h->phase < 0) return fail(h, LBM_ERR_STATE, "lbm_step before initialisation");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    return enqueue_steps(h, nsteps);
+}
+
+extern "C" int lbm_step_timed(lbm_handle *h, int32_t nsteps, double *ms)
+{
+    if (!h || nsteps < 0 || !ms) return LBM_ERR_ARG;
+    if (h->phase < 0) return fail(h, LBM_ERR_STATE, "lbm_step before initialisation");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    if (h->phase == 0) {            // keep the one-off prologue out of the timed region
+        int rc = enqueue_steps(h, 0);
+        if (rc) return rc;
+    }
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    int rc = enqueue_steps(h, nsteps);
+    if (rc) return rc;
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float t = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&t, h->ev0, h->ev1));
+    *ms = t;
+    return LBM_OK;
+}
+
+extern "C" int lbm_sync(lbm_handle *h)
+{
+    if (!h) return LBM_ERR_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_itc(lbm_handle *h, int64_t *itc)
+{
+    if (!h || !itc) return LBM_ERR_ARG;
+    *itc = h->itc;
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_params(lbm_handle *h, lbm_params *p)
+{
+    if (!h || !p) return LBM_ERR_ARG;
+    *p = h->p;
+    return LBM_OK;
+}
+
+extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
+{
+    if (!h || !bytes) return LBM_ERR_ARG;
+    const double s_pop = 8.0, s_aux = 8.0;
+    const int q = h->qf + h->qt, naux = h->cfg.dims == 3 ? 3 : 2;
+    *bytes = 2.0 * q * s_pop + 2.0 * naux * s_aux;
+    return LBM_OK;
+}
+
+// Bring the macro planes up to date with the current state (phase 1) or leave the plain ones.
+static int refresh_macros(lbm_handle *h)
+{
+    if (h->phase < 0) return fail(h, LBM_ERR_STATE, "no state: call lbm_init_* first");
+    if (h->phase == 1) {
+        const bool d3 = h->cfg.dims == 3;
+        const double *aux = d3 ? h->aux[0] : h->aux[h->fy ^ 1];
+        HIPCHK(h, launch_state(h->kp, h->bx, h->stream, d3, h->F[h->fcur ^ 1], h->G[h->gcur], aux,
+                               h->mac, nullptr, nullptr, 0, 0));
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_macros(lbm_handle *h, double *rho, double *u, double *v, double *w, double *T)
+{
+    if (!h) return LBM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc = refresh_macros(h);
+    if (rc) return rc;
+    double *dst[5] = {rho, u, v, w, T};
+    for (int q = 0; q < 5; ++q)
+        if (dst[q])
+            HIPCHK(h, hipMemcpyAsync(dst[q], h->mac + q * h->N, h->N * sizeof(double),
+                                     hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+// Compensated host combination of per-block (sum, carry) pairs.
+struct HostKS {
+    double s = 0, c = 0;
+    void add(double x)
+    {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    double val() const { return s + c; }
+};
+
+static void combine(const std::vector<double> &part, unsigned blocks, int nq, double *out)
+{
+    for (int q = 0; q < nq; ++q) {
+        HostKS k;
+        for (unsigned b = 0; b < blocks; ++b) {
+            k.add(part[(b * nq + q) * 2]);
+            k.add(part[(b * nq + q) * 2 + 1]);
+        }
+        out[q] = k.val();
+    }
+}
+
+extern "C" int lbm_check(lbm_handle *h, double *error_u, double *error_t)
+{
+    if (!h || !error_u || !error_t) return LBM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    if (!h->prev) {
+        if (hipMalloc((void **)&h->prev, 4 * h->N * sizeof(double)) != hipSuccess)
+            return fail(h, LBM_ERR_NOMEM, "prev arrays");
+        HIPCHK(h, hipMemsetAsync(h->prev, 0, 4 * h->N * sizeof(double), h->stream));
+    }
+    int rc = refresh_macros(h);
+    if (rc) return rc;
+    const unsigned nb = h->red_blocks;
+    HIPCHK(h, launch_reduce_check(h->kp, h->stream, h->mac, h->prev, h->red, nb));
+    std::vector<double> part((size_t)nb * 4 * 2);
+    HIPCHK(h, hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double s[4];
+    combine(part, nb, 4, s);
+    if (h->cfg.dims == 3) {  // 2DRB/3DRB.F90:1512-1521
+        *error_u = s[1] > 1.0e-30 ? sqrt(s[0]) / sqrt(s[1]) : sqrt(s[0]);
+        *error_t = s[3] > 1.0e-30 ? s[2] / s[3] : s[2];
+    } else {                 // 2DRB/2DRB.F90:1244-1245 (no guard)
+        *error_u = sqrt(s[0]) / sqrt(s[1]);
+        *error_t = s[2] / s[3];
+    }
+    return LBM_OK;
+}
+
+// fit_adiabatic_wall_T4, 2DRB/3DRB.F90:2212-2241
+static double fit_wall_T4(double y0, const double *y, const double *t)
+{
+    double sq[4], S1 = 0, S2 = 0, B0 = 0, B1 = 0;
+    const double S0 = 4.0;
+    for (int k = 0; k < 4; ++k) sq[k] = (y[k] - y0) * (y[k] - y0);
+    for (int k = 0; k < 4; ++k) {
+        S1 += sq[k];
+        S2 += sq[k] * sq[k];
+        B0 += t[k];
+        B1 += t[k] * sq[k];
+    }
+    return (B0 * S2 - B1 * S1) / (S0 * S2 - S1 * S1);
+}
+
+// fit_parabola_ls5, 2DRB/3DRB.F90:2252-2334
+static void fit_parabola5(const double *y, const double *f, int mode, double *fs, double *ys)
+{
+    int kb = 0;
+    for (int k = 1; k < 5; ++k)
+        if ((mode == 1 && f[k] > f[kb]) || (mode != 1 && f[k] < f[kb])) kb = k;
+    double S0 = 0, S1 = 0, S2 = 0, S3 = 0, S4 = 0, F0 = 0, F1 = 0, F2 = 0;
+    for (int k = 0; k < 5; ++k) {
+        S0 += 1.0;
+        S1 += y[k];
+        S2 += y[k] * y[k];
+        S3 += y[k] * y[k] * y[k];
+        S4 += y[k] * y[k] * y[k] * y[k];
+        F0 += f[k];
+        F1 += f[k] * y[k];
+        F2 += f[k] * y[k] * y[k];
+    }
+    const double D = S0 * (S2 * S4 - S3 * S3) - S1 * (S1 * S4 - S2 * S3) + S2 * (S1 * S3 - S2 * S2);
+    const double DA = F0 * (S2 * S4 - S3 * S3) - S1 * (F1 * S4 - S3 * F2) + S2 * (F1 * S3 - S2 * F2);
+    const double DB = S0 * (F1 * S4 - S3 * F2) - F0 * (S1 * S4 - S2 * S3) + S2 * (S1 * F2 - F1 * S2);
+    const double DC = S0 * (S2 * F2 - F1 * S3) - S1 * (S1 * F2 - F1 * S2) + F0 * (S1 * S3 - S2 * S2);
+    *ys = y[kb];
+    *fs = f[kb];
+    if (fabs(D) <= 1.0e-20) return;
+    const double A = DA / D, B = DB / D, C = DC / D;
+    double ymin = y[0], ymax = y[0];
+    for (int k = 1; k < 5; ++k) {
+        ymin = y[k] < ymin ? y[k] : ymin;
+        ymax = y[k] > ymax ? y[k] : ymax;
+    }
+    if (fabs(C) <= 1.0e-14 || (mode == 1 && C >= 0.0) || (mode == -1 && C <= 0.0)) return;
+    const double yst = -B / (2.0 * C);
+    if (yst < ymin || yst > ymax) return;
+    *ys = yst;
+    *fs = A + B * yst + C * yst * yst;
+}
+
+extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
+{
+    if (!h || !d) return LBM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc = refresh_macros(h);
+    if (rc) return rc;
+    const lbm_config &c = h->cfg;
+    const double L = h->p.L, dy = 1.0 / L, coef = L / h->p.diffusivity;
+    const double dT = c.thot - c.tcold, N = (double)h->N;
+    const int nx = c.nx;
+    const unsigned nb = h->red_blocks;
+    memset(d, 0, sizeof(*d));
+
+    HIPCHK(h, launch_reduce_volume(h->kp, h->stream, h->mac, coef, dy, h->red, nb));
+    std::vector<double> part((size_t)nb * 6 * 2);
+    HIPCHK(h, hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double s[6];
+    combine(part, nb, 6, s);
+    d->nu_vol = s[0] / N * L / h->p.diffusivity + 1.0;
+    d->re_vol = sqrt(s[1] / N) * L / h->p.viscosity;
+    d->nu_global = (s[2] / N) / dT;
+    d->mass = s[3];
+    d->heat = s[4];
+    d->kinetic = s[5];
+
+    HIPCHK(h, launch_reduce_walls(h->kp, h->stream, h->mac, coef, dy, h->wall, h->wall + nx, h->red));
+    std::vector<double> wpart((size_t)nx * 4 * 2), prof((size_t)2 * nx);
+    HIPCHK(h, hipMemcpyAsync(wpart.data(), h->red, wpart.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(prof.data(), h->wall, prof.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HostKS hot, cold, mid;
+    for (int i = 0; i < nx; ++i) {
+        hot.add(prof[nx + i]);
+        cold.add(wpart[(i * 4 + 2) * 2]);
+        cold.add(wpart[(i * 4 + 2) * 2 + 1]);
+        mid.add(wpart[(i * 4 + 3) * 2]);
+        mid.add(wpart[(i * 4 + 3) * 2 + 1]);
+    }
+    d->nu_hot = hot.val() / (double)nx;
+    d->nu_cold = cold.val() / ((double)nx * c.nz);
+    d->nu_middle = mid.val() / ((double)nx * c.nz);
+
+    // Corner extension and 5-point fits of the hot-wall profile (2DRB/3DRB.F90:2089-2150).
+    std::vector<double> xp(nx + 2), ext(nx + 2);
+    xp[0] = 0.0;
+    xp[nx + 1] = (double)nx / L;
+    for (int i = 1; i <= nx; ++i) xp[i] = ((double)i - 0.5) / L;
+    for (int i = 0; i < nx; ++i) ext[i + 1] = prof[nx + i];
+    const double *tb = prof.data();
+    ext[0] = (2.0 * (c.thot - fit_wall_T4(0.0, &xp[1], tb)) / dy) / dT;
+    ext[nx + 1] = (2.0 * (c.thot - fit_wall_T4(xp[nx + 1], &xp[nx - 3], tb + nx - 4)) / dy) / dT;
+    int imax = 0, imin = 0;
+    for (int i = 1; i <= nx + 1; ++i) {
+        if (ext[i] > ext[imax]) imax = i;
+        if (ext[i] < ext[imin]) imin = i;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ic = pass == 0 ? imax : imin;
+        const int i0 = ic <= 2 ? 0 : (ic >= nx - 1 ? nx - 3 : ic - 2);
+        double fs, ys;
+        fit_parabola5(&xp[i0], &ext[i0], pass == 0 ? 1 : -1, &fs, &ys);
+        if (pass == 0) { d->nu_hot_max = fs; d->nu_hot_max_pos = ys; }
+        else { d->nu_hot_min = fs; d->nu_hot_min_pos = ys; }
+    }
+    return LBM_OK;
+}
+
+// Chunk of z-planes whose AoS f and g fit a staging buffer of about 256 MiB.
+static int chunk_planes(const lbm_handle *h)
+{
+    const size_t plane = (size_t)h->cfg.nx * h->cfg.ny * (h->qf + h->qt) * sizeof(double);
+    size_t n = ((size_t)256 << 20) / (plane ? plane : 1);
+    if (n < 1) n = 1;
+    if (n > (size_t)h->cfg.nz) n = h->cfg.nz;
+    return (int)n;
+}
+
+extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
+{
+    if (!h) return LBM_ERR_ARG;
+    if (h->phase < 0) return fail(h, LBM_ERR_STATE, "no state: call lbm_init_* first");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const bool d3 = h->cfg.dims == 3;
+    const int nk = chunk_planes(h);
+    const size_t plane = (size_t)h->cfg.nx * h->cfg.ny;
+    double *stage = nullptr;
+    if (f || g) {
+        if (hipMalloc((void **)&stage, (size_t)nk * plane * (h->qf + h->qt) * sizeof(double)) != hipSuccess)
+            return fail(h, LBM_ERR_NOMEM, "gather staging");
+    }
+    double *fo = stage, *go = stage ? stage + (size_t)nk * plane * h->qf : nullptr;
+    int rc = LBM_OK;
+    for (int k0 = 0; stage && k0 < h->cfg.nz && rc == LBM_OK; k0 += nk) {
+        const int n = (k0 + nk <= h->cfg.nz) ? nk : h->cfg.nz - k0;
+        const uint64_t c0 = (uint64_t)k0 * plane, cnt = (uint64_t)n * plane;
+        hipError_t e;
+        if (h->phase == 0) {
+            e = launch_transpose(false, h->stream, h->F[0], fo, h->qf, h->N, c0, cnt);
+            if (e == hipSuccess) e = launch_transpose(false, h->stream, h->G[0], go, h->qt, h->N, c0, cnt);
+        } else {
+            const double *aux = d3 ? h->aux[0] : h->aux[h->fy ^ 1];
+            e = launch_state(h->kp, h->bx, h->stream, d3, h->F[h->fcur ^ 1], h->G[h->gcur], aux,
+                             nullptr, fo, go, k0, n);
+        }
+        if (e == hipSuccess && f)
+            e = hipMemcpyAsync(f + c0 * h->qf, fo, cnt * h->qf * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess && g)
+            e = hipMemcpyAsync(g + c0 * h->qt, go, cnt * h->qt * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(h, LBM_ERR_HIP, "gather: %s", hipGetErrorString(e));
+    }
+    if (stage) (void)hipFree(stage);
+    if (rc) return rc;
+    if (bprev) {
+        HIPCHK(h, hipMemcpyAsync(bprev, h->Bp, 3 * h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_scatter(lbm_handle *h, const double *f, const double *g)
+{
+    if (!h || !f || !g) return LBM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const int nk = chunk_planes(h);
+    const size_t plane = (size_t)h->cfg.nx * h->cfg.ny;
+    double *stage = nullptr;
+    if (hipMalloc((void **)&stage, (size_t)nk * plane * (h->qf + h->qt) * sizeof(double)) != hipSuccess)
+        return fail(h, LBM_ERR_NOMEM, "scatter staging");
+    double *fo = stage, *go = stage + (size_t)nk * plane * h->qf;
+    int rc = LBM_OK;
+    for (int k0 = 0; k0 < h->cfg.nz && rc == LBM_OK; k0 += nk) {
+        const int n = (k0 + nk <= h->cfg.nz) ? nk : h->cfg.nz - k0;
+        const uint64_t c0 = (uint64_t)k0 * plane, cnt = (uint64_t)n * plane;
+        hipError_t e = hipMemcpyAsync(fo, f + c0 * h->qf, cnt * h->qf * sizeof(double), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(go, g + c0 * h->qt, cnt * h->qt * sizeof(double), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = launch_transpose(true, h->stream, fo, h->F[0], h->qf, h->N, c0, cnt);
+        if (e == hipSuccess) e = launch_transpose(true, h->stream, go, h->G[0], h->qt, h->N, c0, cnt);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(h, LBM_ERR_HIP, "scatter: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(stage);
+    if (rc) return rc;
+    HIPCHK(h, hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
+    HIPCHK(h, launch_reconstruct(h->kp, h->stream, h->F[0], h->G[0], h->Bp, h->mac, h->qf, h->qt, h->flag));
+    int bad = 0;
+    HIPCHK(h, hipMemcpyAsync(&bad, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->phase = 0;
+    if (bad) return fail(h, LBM_ERR_RHO, "non-positive rho found during restart reconstruction");
+    return LBM_OK;
+}

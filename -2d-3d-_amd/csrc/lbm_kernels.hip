@@ -1,0 +1,751 @@
+// lbm_kernels.hip -- gfx950 kernels of the thermal MRT-LBM Rayleigh-Benard step (FP64 form).
+//
+// Data layout in HBM (structure of arrays, x fastest):
+//   F[2]  flow populations,   F[b][a*N + c]    19 (2D: 9) planes of N nodes, ping-pong
+//   G[2]  thermal populations, G[b][a*N + c]   7 (2D: 5) planes, ping-pong
+//   Bp    M1G history B_prev = u T, 3 planes
+//   aux   3D: T of the previous step (half force of `macro`); 2D: two planes holding the
+//         stored y force of the two latest flow collisions (2DRB/2DRB.F90:718-719, :887-888)
+//   mac   rho, u, v, w, T planes (initial state, then diagnostics output)
+// with c = i + nx (j + ny k).  Storage holds POST-collision populations; every kernel pulls
+// f(a, x) = f_post(a, x - e_a) and applies the walls inline (the reference's streaming +
+// bounceback sweeps, 2DRB/3DRB.F90:957-1058 and :1209-1351).
+//
+// One reference time step n (collision .. macroT, 2DRB/3DRB.F90:227-243) is two launches:
+//   A(n): pull f^{n+1} from F (f_post^n) -> rho, u^{n+1} with the half force of T^n;
+//         pull g^n from G (g_post^{n-1}) -> T^n; thermal collision -> g_post^n; B_prev.
+//   B(n): pull f^{n+1} again; pull-sum g_post^n -> T^{n+1}; flow collision of step n+1
+//         -> f_post^{n+1}.
+// The flow collision of step n+1 needs T^{n+1} at the node, which needs the thermal
+// collision at its six neighbours, which needs u^{n+1} there: two neighbour hops per step,
+// hence two passes.  A "prologue" launch performs the flow collision of step 1 from the plain
+// initial state, and the first A launch reads g^1 and T^1 unpulled from that state.
+#include "lbm_collide.h"
+#include "lbm_internal.h"
+
+namespace rblbm {
+
+// ---------------------------------------------------------------------------------------
+// Pull with walls.
+// ---------------------------------------------------------------------------------------
+// Flow lattice: the wall sweeps of `bounceback` overwrite in the order x, y, z, so the value
+// of a population whose source lies outside is decided by the LAST outside axis:
+//   z outside: periodic -> f_post(a, i, j, wrap k) (same i, j); no-slip -> f_post(opp a, x)
+//   y outside: no-slip  -> f_post(opp a, x)
+//   x outside: periodic -> f_post(a, wrap i, j, k) (same j, k); no-slip -> f_post(opp a, x)
+template <class L>
+__device__ __forceinline__ void pull_flow(const double *__restrict__ P, uint64_t N, int i, int j,
+                                          int k, uint64_t c, bool border, const KParams &kp,
+                                          double *f)
+{
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+    if (!border) {
+#pragma unroll
+        for (int a = 0; a < L::Q; ++a)
+            f[a] = P[a * N + c - (L::ex[a] + L::ey[a] * sy + L::ez[a] * sz)];
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < L::Q; ++a) {
+        const int si = i - L::ex[a], sj = j - L::ey[a], sk = k - L::ez[a];
+        const bool ox = (unsigned)si >= (unsigned)kp.nx;
+        const bool oy = (unsigned)sj >= (unsigned)kp.ny;
+        const bool oz = (unsigned)sk >= (unsigned)kp.nz;
+        uint64_t src;
+        int pa = a;
+        if (oz) {
+            if (kp.vel_z == VEL_PERIODIC) src = c + (uint64_t)(L::ez[a] == 1 ? kp.nz - 1 - k : -k) * sz;
+            else { src = c; pa = L::opp[a]; }
+        } else if (oy) {
+            src = c;
+            pa = L::opp[a];
+        } else if (ox) {
+            if (kp.vel_x == VEL_PERIODIC) src = c + (uint64_t)(int64_t)(L::ex[a] == 1 ? kp.nx - 1 - i : -i);
+            else { src = c; pa = L::opp[a]; }
+        } else {
+            src = c - (int64_t)(L::ex[a] + L::ey[a] * sy + L::ez[a] * sz);
+        }
+        f[a] = P[pa * N + src];
+    }
+}
+
+// Thermal lattice (one non-zero component per direction, so only one wall applies):
+//   periodic -> g_post(a, wrap); constant T -> -g_post(opp a, x) + 2 wT_a T_wall (hot wall at
+//   the low index); adiabatic -> g_post(opp a, x).
+template <class L>
+__device__ __forceinline__ void pull_heat(const double *__restrict__ P, uint64_t N, int i, int j,
+                                          int k, uint64_t c, bool border, const KParams &kp,
+                                          double *g)
+{
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+    if (!border) {
+#pragma unroll
+        for (int a = 0; a < L::Q; ++a)
+            g[a] = P[a * N + c - (L::ex[a] + L::ey[a] * sy + L::ez[a] * sz)];
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < L::Q; ++a) {
+        const int ex = L::ex[a], ey = L::ey[a], ez = L::ez[a];
+        const int si = i - ex, sj = j - ey, sk = k - ez;
+        int kind = -1, s = 0;
+        int64_t wrap = 0;
+        if ((unsigned)si >= (unsigned)kp.nx) {
+            kind = kp.temp_x; s = ex;
+            wrap = ex == 1 ? kp.nx - 1 - i : -i;
+        } else if ((unsigned)sj >= (unsigned)kp.ny) {
+            kind = kp.temp_y; s = ey;
+            wrap = (int64_t)(ey == 1 ? kp.ny - 1 - j : -j) * sy;
+        } else if ((unsigned)sk >= (unsigned)kp.nz) {
+            kind = kp.temp_z; s = ez;
+            wrap = (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz;
+        }
+        const double wa = (L::Q == 7) ? w7(a) : w5(a);
+        if (kind < 0) g[a] = P[a * N + c - (int64_t)(ex + ey * sy + ez * sz)];
+        else if (kind == T_PERIODIC) g[a] = P[a * N + c + wrap];
+        else if (kind == T_CONST) g[a] = -P[L::opp[a] * N + c] + 2.0 * wa * (s > 0 ? kp.thot : kp.tcold);
+        else g[a] = P[L::opp[a] * N + c];
+    }
+}
+
+__device__ __forceinline__ bool on_border(int i, int j, int k, const KParams &kp)
+{
+    return i == 0 || j == 0 || k == 0 || i == kp.nx - 1 || j == kp.ny - 1 || k == kp.nz - 1;
+}
+
+// Node coordinates of this thread; false if the thread is past the end of its row.
+__device__ __forceinline__ bool node_of(const KParams &kp, int &i, int &j, int &k, uint64_t &c)
+{
+    const unsigned lb = xcd_remap(blockIdx.x, kp.nblocks);
+    const unsigned xb = lb % (unsigned)kp.nxb;
+    const unsigned row = lb / (unsigned)kp.nxb;
+    j = (int)(row % (unsigned)kp.ny);
+    k = (int)(row / (unsigned)kp.ny);
+    i = (int)(xb * blockDim.x + threadIdx.x);
+    c = (uint64_t)i + (uint64_t)kp.nx * ((uint64_t)j + (uint64_t)kp.ny * (uint64_t)k);
+    return i < kp.nx;
+}
+
+__device__ __forceinline__ double sum19(const double *f)
+{
+    return f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] +
+           f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+}
+// Momentum sums in the reference's term order (macro, 2DRB/3DRB.F90:1079-1090).
+__device__ __forceinline__ double jx19(const double *f)
+{
+    return f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
+}
+__device__ __forceinline__ double jy19(const double *f)
+{
+    return f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18];
+}
+__device__ __forceinline__ double jz19(const double *f)
+{
+    return f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18];
+}
+
+// ---------------------------------------------------------------------------------------
+// 3D step kernels
+// ---------------------------------------------------------------------------------------
+// Flow collision of step 1 from the plain initial state (f^1 unpulled in Fin, macros in mac).
+__global__ void __launch_bounds__(256) k3_prologue(KParams kp, const double *__restrict__ Fin,
+                                                   double *__restrict__ Fout,
+                                                   const double *__restrict__ mac)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    double f[19];
+#pragma unroll
+    for (int a = 0; a < 19; ++a) f[a] = Fin[a * N + c];
+    collide_d3q19(f, f, mac[c], mac[N + c], mac[2 * N + c], mac[3 * N + c], mac[4 * N + c], kp);
+#pragma unroll
+    for (int a = 0; a < 19; ++a) Fout[a * N + c] = f[a];
+}
+
+// Pass A of step n.  FIRST: g^n and T^n come unpulled from the initial state.
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k3_thermal(KParams kp, const double *__restrict__ Fcur,
+                                                  const double *__restrict__ Gin,
+                                                  double *__restrict__ Gout,
+                                                  double *__restrict__ Bp,
+                                                  double *__restrict__ aux,
+                                                  const double *__restrict__ mac)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border(i, j, k, kp);
+    double g[7], Tn;
+    if (FIRST) {
+#pragma unroll
+        for (int a = 0; a < 7; ++a) g[a] = Gin[a * N + c];
+        Tn = mac[4 * N + c];
+    } else {
+        pull_heat<D3Q7>(Gin, N, i, j, k, c, border, kp, g);
+        Tn = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
+    }
+    double f[19];
+    pull_flow<D3Q19>(Fcur, N, i, j, k, c, border, kp, f);
+    const double rho = sum19(f);
+    const double Fy = rho * kp.gbeta * (Tn - kp.tref);
+    const double u = jx19(f) / rho;
+    const double v = (jy19(f) + 0.5 * Fy) / rho;
+    const double w = jz19(f) / rho;
+    double bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
+    double p[7];
+    collide_d3q7(g, p, Tn, u, v, w, bp, kp);
+#pragma unroll
+    for (int a = 0; a < 7; ++a) Gout[a * N + c] = p[a];
+    if (kp.use_g) {
+        Bp[c] = bp[0];
+        Bp[N + c] = bp[1];
+        Bp[2 * N + c] = bp[2];
+    }
+    aux[c] = Tn;
+}
+
+// Pass B of step n: flow collision of step n+1.
+__global__ void __launch_bounds__(256) k3_flow(KParams kp, const double *__restrict__ Fcur,
+                                               double *__restrict__ Fnext,
+                                               const double *__restrict__ Gpost,
+                                               const double *__restrict__ aux)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border(i, j, k, kp);
+    double g[7];
+    pull_heat<D3Q7>(Gpost, N, i, j, k, c, border, kp, g);
+    const double T1 = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
+    double f[19];
+    pull_flow<D3Q19>(Fcur, N, i, j, k, c, border, kp, f);
+    const double rho = sum19(f);
+    const double Fy = rho * kp.gbeta * (aux[c] - kp.tref);
+    const double u = jx19(f) / rho;
+    const double v = (jy19(f) + 0.5 * Fy) / rho;
+    const double w = jz19(f) / rho;
+    collide_d3q19(f, f, rho, u, v, w, T1, kp);
+#pragma unroll
+    for (int a = 0; a < 19; ++a) Fnext[a * N + c] = f[a];
+}
+
+// Reference state after the last step (f^{n+1}, g^{n+1}, macros) from the pipelined buffers.
+// Writes macros to mac (5 planes) and, if fo/go are non-null, AoS populations for the nodes
+// of planes [k0, k0 + nk) into fo/go (node index relative to plane k0).
+__global__ void __launch_bounds__(256) k3_state(KParams kp, const double *__restrict__ Fprev,
+                                                const double *__restrict__ Gpost,
+                                                const double *__restrict__ aux,
+                                                double *__restrict__ mac, double *__restrict__ fo,
+                                                double *__restrict__ go, int k0, int nk)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border(i, j, k, kp);
+    double g[7], f[19];
+    pull_heat<D3Q7>(Gpost, N, i, j, k, c, border, kp, g);
+    pull_flow<D3Q19>(Fprev, N, i, j, k, c, border, kp, f);
+    const double T = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
+    const double rho = sum19(f);
+    const double Fy = rho * kp.gbeta * (aux[c] - kp.tref);
+    if (mac) {
+        mac[c] = rho;
+        mac[N + c] = jx19(f) / rho;
+        mac[2 * N + c] = (jy19(f) + 0.5 * Fy) / rho;
+        mac[3 * N + c] = jz19(f) / rho;
+        mac[4 * N + c] = T;
+    }
+    if (fo && k >= k0 && k < k0 + nk) {
+        const uint64_t r = c - (uint64_t)k0 * kp.nx * kp.ny;
+#pragma unroll
+        for (int a = 0; a < 19; ++a) fo[r * 19 + a] = f[a];
+#pragma unroll
+        for (int a = 0; a < 7; ++a) go[r * 7 + a] = g[a];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// 2D step kernels (nz = 1).  Same two-pass structure; the half force of `macro` is the force
+// stored by the preceding flow collision (old rho, old T), kept in aux planes.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k2_prologue(KParams kp, const double *__restrict__ Fin,
+                                                   double *__restrict__ Fout,
+                                                   const double *__restrict__ mac,
+                                                   double *__restrict__ fy_out)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    double f[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) f[a] = Fin[a * N + c];
+    fy_out[c] = collide_d2q9(f, f, mac[c], mac[N + c], mac[2 * N + c], mac[4 * N + c], kp);
+#pragma unroll
+    for (int a = 0; a < 9; ++a) Fout[a * N + c] = f[a];
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k2_thermal(KParams kp, const double *__restrict__ Fcur,
+                                                  const double *__restrict__ Gin,
+                                                  double *__restrict__ Gout,
+                                                  double *__restrict__ Bp,
+                                                  const double *__restrict__ fy_cur,
+                                                  const double *__restrict__ mac)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border(i, j, k, kp);
+    double g[5], Tn;
+    if (FIRST) {
+#pragma unroll
+        for (int a = 0; a < 5; ++a) g[a] = Gin[a * N + c];
+        Tn = mac[4 * N + c];
+    } else {
+        pull_heat<D2Q5>(Gin, N, i, j, k, c, border, kp, g);
+        Tn = g[0] + g[1] + g[2] + g[3] + g[4];
+    }
+    double f[9];
+    pull_flow<D2Q9>(Fcur, N, i, j, k, c, border, kp, f);
+    const double rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    const double u = (f[1] - f[3] + f[5] - f[6] - f[7] + f[8] + 0.5 * 0.0) / rho;
+    const double v = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8] + 0.5 * fy_cur[c]) / rho;
+    double bp[2] = {Bp[c], Bp[N + c]};
+    double p[5];
+    collide_d2q5(g, p, Tn, u, v, bp, kp);
+#pragma unroll
+    for (int a = 0; a < 5; ++a) Gout[a * N + c] = p[a];
+    if (kp.use_g) {
+        Bp[c] = bp[0];
+        Bp[N + c] = bp[1];
+    }
+}
+
+__global__ void __launch_bounds__(256) k2_flow(KParams kp, const double *__restrict__ Fcur,
+                                               double *__restrict__ Fnext,
+                                               const double *__restrict__ Gpost,
+                                               const double *__restrict__ fy_cur,
+                                               double *__restrict__ fy_next)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border(i, j, k, kp);
+    double g[5];
+    pull_heat<D2Q5>(Gpost, N, i, j, k, c, border, kp, g);
+    const double T1 = g[0] + g[1] + g[2] + g[3] + g[4];
+    double f[9];
+    pull_flow<D2Q9>(Fcur, N, i, j, k, c, border, kp, f);
+    const double rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    const double u = (f[1] - f[3] + f[5] - f[6] - f[7] + f[8] + 0.5 * 0.0) / rho;
+    const double v = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8] + 0.5 * fy_cur[c]) / rho;
+    fy_next[c] = collide_d2q9(f, f, rho, u, v, T1, kp);
+#pragma unroll
+    for (int a = 0; a < 9; ++a) Fnext[a * N + c] = f[a];
+}
+
+__global__ void __launch_bounds__(256) k2_state(KParams kp, const double *__restrict__ Fprev,
+                                                const double *__restrict__ Gpost,
+                                                const double *__restrict__ fy_prev,
+                                                double *__restrict__ mac, double *__restrict__ fo,
+                                                double *__restrict__ go, int k0, int nk)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border(i, j, k, kp);
+    double g[5], f[9];
+    pull_heat<D2Q5>(Gpost, N, i, j, k, c, border, kp, g);
+    pull_flow<D2Q9>(Fprev, N, i, j, k, c, border, kp, f);
+    const double rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    if (mac) {
+        mac[c] = rho;
+        mac[N + c] = (f[1] - f[3] + f[5] - f[6] - f[7] + f[8] + 0.5 * 0.0) / rho;
+        mac[2 * N + c] = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8] + 0.5 * fy_prev[c]) / rho;
+        mac[3 * N + c] = 0.0;
+        mac[4 * N + c] = g[0] + g[1] + g[2] + g[3] + g[4];
+    }
+    if (fo && k >= k0 && k < k0 + nk) {
+        const uint64_t r = c - (uint64_t)k0 * kp.nx * kp.ny;
+#pragma unroll
+        for (int a = 0; a < 9; ++a) fo[r * 9 + a] = f[a];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) go[r * 5 + a] = g[a];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Initial state, restart and plain-state transposes.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Exact branch of `initial` (2DRB/3DRB.F90:517-600): T from the reference profile (or from
+// mac[4N..] when use_T0), optional seeded noise, then f = w, g = wT T, rho = 1, u = 0,
+// B_prev = 0.  T is written back to mac, the plain f/g to Fout/Gout.
+__global__ void __launch_bounds__(256) k_init(KParams kp, InitParams ip, double *__restrict__ Fout,
+                                              double *__restrict__ Gout, double *__restrict__ Bp,
+                                              double *__restrict__ mac, int qf, int qt)
+{
+#pragma clang fp contract(off)
+    int i, j, k;
+    uint64_t c;
+    if (!node_of(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    double T;
+    if (ip.use_T0) {
+        T = mac[4 * N + c];
+    } else {
+        T = 0.0;
+        // coordinates x_i = (i - 1/2)/L with x_0 = 0 and x_{n+1} = n/L (2DRB/3DRB.F90:451-470)
+        const double L = ip.L;
+        const double xi = ((double)(i + 1) - 0.5) / L, xn = (double)kp.nx / L;
+        const double yj = ((double)(j + 1) - 0.5) / L, yn = (double)kp.ny / L;
+        if (kp.temp_x == T_CONST) T = kp.thot + (xi - 0.0) / (xn - 0.0) * (kp.tcold - kp.thot);
+        if (kp.temp_y == T_CONST) {
+            T = kp.thot + (yj - 0.0) / (yn - 0.0) * (kp.tcold - kp.thot);
+            if (ip.rb_perturb == 3)
+                T = T + ip.perturb_amp * cos(2.0 * ip.pi * xi / xn) * sin(ip.pi * yj / yn);
+            else if (ip.rb_perturb == 2)
+                T = T + ip.perturb_amp * sin(2.0 * ip.pi * xi / xn) * sin(ip.pi * yj / yn);
+        }
+    }
+    if (ip.noise_scale != 0.0) {
+        const uint64_t key = ip.seed + 0x9E3779B97F4A7C15ULL * (c + 1ULL);
+        const double r = (double)(mix64(key) >> 11) * 0x1.0p-53;
+        const double noise = __dmul_rn(ip.noise_scale, __dsub_rn(__dmul_rn(2.0, r), 1.0));
+        T = __dadd_rn(T, noise);
+    }
+    for (int a = 0; a < qf; ++a)
+        Fout[a * N + c] = qf == 19 ? w19(a) : w9(a);
+    for (int a = 0; a < qt; ++a)
+        Gout[a * N + c] = __dmul_rn(qt == 7 ? w7(a) : w5(a), T);
+    Bp[c] = 0.0;
+    Bp[N + c] = 0.0;
+    Bp[2 * N + c] = 0.0;
+    mac[c] = 1.0;
+    mac[N + c] = 0.0;
+    mac[2 * N + c] = 0.0;
+    mac[3 * N + c] = 0.0;
+    mac[4 * N + c] = T;
+}
+
+// AoS chunk (reference order) <-> plain SoA planes.  Nodes [c0, c0 + n).
+__global__ void k_aos_to_soa(const double *__restrict__ in, double *__restrict__ out, int q,
+                             uint64_t N, uint64_t c0, uint64_t n)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int a = 0; a < q; ++a) out[a * N + c0 + r] = in[r * q + a];
+}
+__global__ void k_soa_to_aos(const double *__restrict__ in, double *__restrict__ out, int q,
+                             uint64_t N, uint64_t c0, uint64_t n)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int a = 0; a < q; ++a) out[r * q + a] = in[a * N + c0 + r];
+}
+
+// reconstruct_macro_from_fg (2DRB/3DRB.F90:1392-1463): macroT, rho, momentum, three fixed-point
+// passes of the half-force velocity, B_prev = u T.  *bad is set if any rho <= 0.
+__global__ void k_reconstruct(KParams kp, const double *__restrict__ F, const double *__restrict__ G,
+                              double *__restrict__ Bp, double *__restrict__ mac, int qf, int qt,
+                              int *bad)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t N = kp.N;
+    if (c >= N) return;
+    double T = 0.0;
+    for (int a = 0; a < qt; ++a) T += G[a * N + c];
+    double f[19];
+    for (int a = 0; a < qf; ++a) f[a] = F[a * N + c];
+    double rho, mx, my, mz = 0.0;
+    if (qf == 19) {
+        rho = sum19(f);
+        mx = jx19(f);
+        my = jy19(f);
+        mz = jz19(f);
+    } else {
+        rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+        mx = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+        my = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+    }
+    double u = 0.0, v = 0.0, w = 0.0;
+    if (rho > 0.0) {
+        for (int it = 0; it < 3; ++it) {
+            const double Fy = rho * kp.gbeta * (T - kp.tref);
+            u = (mx + 0.5 * 0.0) / rho;
+            v = (my + 0.5 * Fy) / rho;
+            w = (mz + 0.5 * 0.0) / rho;
+        }
+    } else {
+        atomicOr(bad, 1);
+    }
+    mac[c] = rho;
+    mac[N + c] = u;
+    mac[2 * N + c] = v;
+    mac[3 * N + c] = w;
+    mac[4 * N + c] = T;
+    if (kp.use_g) {
+        Bp[c] = u * T;
+        Bp[N + c] = v * T;
+        Bp[2 * N + c] = w * T;
+    } else {
+        Bp[c] = Bp[N + c] = Bp[2 * N + c] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Reductions: Neumaier-compensated FP64 per thread, pairwise two-sum across the wavefront
+// and the block, one (sum, carry) pair per block and quantity.  The host adds the block pairs
+// in a fixed order, so results do not depend on scheduling.
+// ---------------------------------------------------------------------------------------
+struct KS {
+    double s, c;
+};
+__device__ __forceinline__ void ks_add(KS &a, double x)
+{
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+    else a.c += (x - t) + a.s;
+    a.s = t;
+}
+__device__ __forceinline__ KS ks_merge(KS a, KS b)
+{
+    // two-sum of the leading parts, carries added
+    const double s = a.s + b.s;
+    const double bb = s - a.s;
+    const double e = (a.s - (s - bb)) + (b.s - bb);
+    return KS{s, a.c + b.c + e};
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_reduce_store(KS (&v)[NQ], double *out)
+{
+    __shared__ double sh_s[NQ][4], sh_c[NQ][4];  // up to 4 wavefronts of 64 lanes
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        KS a = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            KS b{__shfl_down(a.s, off, 64), __shfl_down(a.c, off, 64)};
+            a = ks_merge(a, b);
+        }
+        if (lane == 0) {
+            sh_s[q][wid] = a.s;
+            sh_c[q][wid] = a.c;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            KS a{sh_s[q][0], sh_c[q][0]};
+            for (int w = 1; w < nw; ++w) a = ks_merge(a, KS{sh_s[q][w], sh_c[q][w]});
+            out[(blockIdx.x * NQ + q) * 2] = a.s;
+            out[(blockIdx.x * NQ + q) * 2 + 1] = a.c;
+        }
+    }
+}
+
+// Volume sums over the macro planes: calNuRe, RBcalc_Nu_global and the conservation sums.
+//   0: sum v T   1: sum |u|^2   2: sum qy (global Nu integrand)   3: sum rho   4: sum T
+//   5: sum 0.5 rho |u|^2
+__global__ void __launch_bounds__(256) k_reduce_volume(KParams kp, const double *__restrict__ mac,
+                                                       double L_over_kappa, double dy,
+                                                       double *__restrict__ out)
+{
+    const uint64_t N = kp.N;
+    const int nx = kp.nx, ny = kp.ny;
+    KS acc[6] = {};
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < N;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        const double rho = mac[c], u = mac[N + c], v = mac[2 * N + c], w = mac[3 * N + c];
+        const double T = mac[4 * N + c];
+        const int j = (int)((c / (uint64_t)nx) % (uint64_t)ny);
+        ks_add(acc[0], v * T);
+        const double q2 = u * u + v * v + w * w;
+        ks_add(acc[1], q2);
+        double dTdy;
+        if (j == 0) dTdy = (3.0 * T + mac[4 * N + c + nx] - 4.0 * kp.thot) / (3.0 * dy);
+        else if (j == ny - 1) dTdy = (4.0 * kp.tcold - 3.0 * T - mac[4 * N + c - nx]) / (3.0 * dy);
+        else dTdy = (mac[4 * N + c + nx] - mac[4 * N + c - nx]) / (2.0 * dy);
+        ks_add(acc[2], L_over_kappa * v * (T - kp.tref) - dTdy);
+        ks_add(acc[3], rho);
+        ks_add(acc[4], T);
+        ks_add(acc[5], 0.5 * rho * q2);
+    }
+    block_reduce_store<6>(acc, out);
+}
+
+// Wall sums of RBcalc_Nu_wall_avg: one block per x index i, reducing over z.
+//   per-i outputs: T(i,1,:) average and hot-wall Nu average (plain arrays, length nx)
+//   per-block pairs: 0 sum cold-wall Nu, 1 sum mid-plane Nu
+__global__ void __launch_bounds__(256) k_reduce_walls(KParams kp, const double *__restrict__ mac,
+                                                      double L_over_kappa, double dy,
+                                                      double *__restrict__ tbot,
+                                                      double *__restrict__ nubot,
+                                                      double *__restrict__ out)
+{
+    const int i = blockIdx.x;
+    const uint64_t N = kp.N;
+    const int nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const double *T = mac + 4 * N, *v = mac + 2 * N;
+    const double dT = kp.thot - kp.tcold;
+    KS acc[4] = {};
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+        const uint64_t b = (uint64_t)i + (uint64_t)nx * (uint64_t)ny * k;  // (i, 0, k)
+        ks_add(acc[0], T[b]);
+        ks_add(acc[1], ((8.0 * kp.thot - 9.0 * T[b] + T[b + nx]) / (3.0 * dy)) / dT);
+        const uint64_t t = b + (uint64_t)nx * (ny - 1);
+        ks_add(acc[2], ((-8.0 * kp.tcold + 9.0 * T[t] - T[t - nx]) / (3.0 * dy)) / dT);
+        double mid;
+        if (ny % 2 == 1) {
+            const uint64_t m = b + (uint64_t)nx * ((ny + 1) / 2 - 1);
+            mid = (L_over_kappa * v[m] * (T[m] - kp.tref) - (T[m + nx] - T[m - nx]) / (2.0 * dy)) / dT;
+        } else {
+            const uint64_t mb = b + (uint64_t)nx * (ny / 2 - 1), mt = mb + nx;
+            mid = (L_over_kappa * 0.5 * (v[mb] * (T[mb] - kp.tref) + v[mt] * (T[mt] - kp.tref)) +
+                   (T[mb] - T[mt]) / dy) / dT;
+        }
+        ks_add(acc[3], mid);
+    }
+    block_reduce_store<4>(acc, out);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double s0 = out[(blockIdx.x * 4 + 0) * 2] + out[(blockIdx.x * 4 + 0) * 2 + 1];
+        const double s1 = out[(blockIdx.x * 4 + 1) * 2] + out[(blockIdx.x * 4 + 1) * 2 + 1];
+        tbot[i] = s0 / (double)nz;
+        nubot[i] = s1 / (double)nz;
+    }
+}
+
+// `check` sums: 0 sum |u - u_p|^2, 1 sum |u|^2, 2 sum |T - T_p|, 3 sum |T|; then u_p = u.
+__global__ void __launch_bounds__(256) k_reduce_check(KParams kp, const double *__restrict__ mac,
+                                                      double *__restrict__ prev,
+                                                      double *__restrict__ out)
+{
+    const uint64_t N = kp.N;
+    KS acc[4] = {};
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < N;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        const double u = mac[N + c], v = mac[2 * N + c], w = mac[3 * N + c], T = mac[4 * N + c];
+        const double du = u - prev[c], dv = v - prev[N + c], dw = w - prev[2 * N + c];
+        ks_add(acc[0], du * du + dv * dv + dw * dw);
+        ks_add(acc[1], u * u + v * v + w * w);
+        ks_add(acc[2], fabs(T - prev[3 * N + c]));
+        ks_add(acc[3], fabs(T));
+        prev[c] = u;
+        prev[N + c] = v;
+        prev[2 * N + c] = w;
+        prev[3 * N + c] = T;
+    }
+    block_reduce_store<4>(acc, out);
+}
+
+// ---------------------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------------------
+static dim3 step_grid(const KParams &kp) { return dim3(kp.nblocks); }
+
+hipError_t launch_prologue(const KParams &kp, int bx, hipStream_t s, const double *Fin,
+                           double *Fout, const double *mac, double *fy_out, bool three_d)
+{
+    if (three_d) k3_prologue<<<step_grid(kp), bx, 0, s>>>(kp, Fin, Fout, mac);
+    else k2_prologue<<<step_grid(kp), bx, 0, s>>>(kp, Fin, Fout, mac, fy_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_step3(const KParams &kp, int bx, hipStream_t s, bool first, const double *Fcur,
+                        double *Fnext, const double *Gin, double *Gout, double *Bp, double *aux,
+                        const double *mac)
+{
+    if (first) k3_thermal<true><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
+    else k3_thermal<false><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
+    k3_flow<<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Fnext, Gout, aux);
+    return hipGetLastError();
+}
+
+hipError_t launch_step2(const KParams &kp, int bx, hipStream_t s, bool first, const double *Fcur,
+                        double *Fnext, const double *Gin, double *Gout, double *Bp,
+                        const double *fy_cur, double *fy_next, const double *mac)
+{
+    if (first) k2_thermal<true><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, fy_cur, mac);
+    else k2_thermal<false><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, fy_cur, mac);
+    k2_flow<<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Fnext, Gout, fy_cur, fy_next);
+    return hipGetLastError();
+}
+
+hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, const double *Fprev,
+                        const double *Gpost, const double *aux_or_fy, double *mac, double *fo,
+                        double *go, int k0, int nk)
+{
+    if (three_d) k3_state<<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, mac, fo, go, k0, nk);
+    else k2_state<<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, mac, fo, go, k0, nk);
+    return hipGetLastError();
+}
+
+hipError_t launch_init(const KParams &kp, int bx, hipStream_t s, const InitParams &ip, double *F,
+                       double *G, double *Bp, double *mac, int qf, int qt)
+{
+    k_init<<<step_grid(kp), bx, 0, s>>>(kp, ip, F, G, Bp, mac, qf, qt);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose(bool to_soa, hipStream_t s, const double *in, double *out, int q,
+                            uint64_t N, uint64_t c0, uint64_t n)
+{
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks == 0) return hipSuccess;
+    if (to_soa) k_aos_to_soa<<<blocks, 256, 0, s>>>(in, out, q, N, c0, n);
+    else k_soa_to_aos<<<blocks, 256, 0, s>>>(in, out, q, N, c0, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reconstruct(const KParams &kp, hipStream_t s, const double *F, const double *G,
+                              double *Bp, double *mac, int qf, int qt, int *bad)
+{
+    const unsigned blocks = (unsigned)((kp.N + 255) / 256);
+    k_reconstruct<<<blocks, 256, 0, s>>>(kp, F, G, Bp, mac, qf, qt, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_volume(const KParams &kp, hipStream_t s, const double *mac,
+                                double L_over_kappa, double dy, double *out, unsigned blocks)
+{
+    k_reduce_volume<<<blocks, 256, 0, s>>>(kp, mac, L_over_kappa, dy, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_walls(const KParams &kp, hipStream_t s, const double *mac,
+                               double L_over_kappa, double dy, double *tbot, double *nubot,
+                               double *out)
+{
+    k_reduce_walls<<<kp.nx, 256, 0, s>>>(kp, mac, L_over_kappa, dy, tbot, nubot, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *mac, double *prev,
+                               double *out, unsigned blocks)
+{
+    k_reduce_check<<<blocks, 256, 0, s>>>(kp, mac, prev, out);
+    return hipGetLastError();
+}
+
+} // namespace rblbm

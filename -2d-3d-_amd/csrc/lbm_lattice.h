@@ -1,0 +1,80 @@
+// lbm_lattice.h -- lattice tables and shared device helpers for the gfx950 kernels.
+//
+// Direction order, opposite table and weights follow the reference so that the C-ABI gather
+// returns populations in its order: D3Q19 / D3Q7 in 2DRB/3DRB.F90:656-678, D2Q9 / D2Q5 in
+// 2DRB/2DRB.F90:188-190 and :506-524.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rblbm {
+
+struct D3Q19 {
+    static constexpr int Q = 19;
+    static constexpr int ex[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+    static constexpr int ey[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+    static constexpr int ez[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+    static constexpr int opp[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+};
+struct D3Q7 {
+    static constexpr int Q = 7;
+    static constexpr int ex[7] = {0, 1, -1, 0, 0, 0, 0};
+    static constexpr int ey[7] = {0, 0, 0, 1, -1, 0, 0};
+    static constexpr int ez[7] = {0, 0, 0, 0, 0, 1, -1};
+    static constexpr int opp[7] = {0, 2, 1, 4, 3, 6, 5};
+};
+struct D2Q9 {
+    static constexpr int Q = 9;
+    static constexpr int ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    static constexpr int ey[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    static constexpr int ez[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    static constexpr int opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+};
+struct D2Q5 {
+    static constexpr int Q = 5;
+    static constexpr int ex[5] = {0, 1, 0, -1, 0};
+    static constexpr int ey[5] = {0, 0, 1, 0, -1};
+    static constexpr int ez[5] = {0, 0, 0, 0, 0};
+    static constexpr int opp[5] = {0, 3, 4, 1, 2};
+};
+
+// Weights of the flow lattices (rest, axis, diagonal) and of the thermal lattices (rest, axis).
+__host__ __device__ constexpr double w19(int a) { return a == 0 ? 1.0 / 3.0 : (a <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+__host__ __device__ constexpr double w7(int a) { return a == 0 ? 0.25 : 0.125; }
+__host__ __device__ constexpr double w9(int a) { return a == 0 ? 4.0 / 9.0 : (a <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+__host__ __device__ constexpr double w5(int a) { return a == 0 ? 1.0 / 3.0 : 1.0 / 6.0; }
+
+// Wall codes (mirror LBM_VEL_* / LBM_T_* of lbm_rb.h).
+enum { VEL_NOSLIP = 0, VEL_PERIODIC = 1 };
+enum { T_CONST = 0, T_ADIABATIC = 1, T_PERIODIC = 2 };
+
+// Everything a step kernel needs, passed by value (lives in SGPRs / kernarg segment).
+struct KParams {
+    int nx, ny, nz;
+    int vel_x, vel_z;
+    int temp_x, temp_y, temp_z;
+    int use_g;
+    int nxb;          // x blocks per row
+    unsigned nblocks; // total blocks of the launch (for the XCD remap)
+    uint64_t N;       // nodes
+    double gbeta, tref, thot, tcold;
+    double sv, sq;    // 1/tau_f (s_e = s_eps = s_nu = s_pi) and s_q = s_m
+    double qk, qnu, sg;
+};
+
+// Blocks are dealt to the 8 XCDs round-robin by hardware.  Renumber them so that each XCD
+// owns one contiguous run of logical blocks (a z-slab of the domain): the neighbour rows a
+// pull reads then sit in the same XCD's L2 instead of in another XCD's.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks)
+{
+    constexpr unsigned XCD = 8;
+    unsigned per = nblocks / XCD, rem = nblocks % XCD;
+    unsigned x = b % XCD, r = b / XCD;
+    // XCD x holds per (+1 for the first rem) consecutive logical ids
+    unsigned base = x * per + (x < rem ? x : rem);
+    unsigned cnt = per + (x < rem ? 1u : 0u);
+    return r < cnt ? base + r : b;  // b beyond the even split keeps identity (not reached)
+}
+
+} // namespace rblbm

@@ -1,0 +1,243 @@
+"""Python host binding of the gfx950 solver (ctypes over include/lbm_rb.h).
+
+This is the Python-side mirror of the reference's driver interface: a `Solver` owns one
+domain and exposes the reference's phases by name (`initial`, `step`, `check`, `calNuRe` /
+`diagnostics`, `backupData` / `gather`, `reload` / `scatter`).  Every call goes to the HIP
+library librblbm.so; there is no CPU fallback.  If the library is missing the import of a
+Solver fails with an error naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, asdict
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "librblbm.so")
+
+VEL_NOSLIP, VEL_PERIODIC = 0, 1
+T_CONST, T_ADIABATIC, T_PERIODIC = 0, 1, 2
+FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
+
+EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
+           "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",
+           "lbm_diagnostics", "lbm_gather", "lbm_scatter", "lbm_bytes_per_update",
+           "lbm_last_error", "lbm_error_string")
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lbm error {code}: {msg}")
+        self.code = code
+
+
+class _Config(C.Structure):
+    _fields_ = [("dims", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("rayleigh", C.c_double), ("prandtl", C.c_double), ("mach", C.c_double),
+                ("thot", C.c_double), ("tcold", C.c_double), ("side_heated", C.c_int32),
+                ("vel_x", C.c_int32), ("vel_z", C.c_int32), ("temp_x", C.c_int32),
+                ("temp_y", C.c_int32), ("temp_z", C.c_int32), ("use_g", C.c_int32),
+                ("precision", C.c_int32), ("device", C.c_int32)]
+
+
+class _Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "L", "tref", "tauf", "viscosity", "diffusivity", "gbeta", "time_unit", "velocity_unit",
+        "s_e", "s_eps", "s_nu", "s_pi", "s_q", "s_m", "taug", "q_k", "q_nu", "sg")]
+
+
+class _Diag(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "nu_vol", "re_vol", "nu_global", "nu_hot", "nu_cold", "nu_middle", "nu_hot_max",
+        "nu_hot_max_pos", "nu_hot_min", "nu_hot_min_pos", "mass", "heat", "kinetic")]
+
+
+_lib = None
+
+
+def library(path: str = LIB_PATH):
+    """Load librblbm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -2d-3d-_amd/build.py` "
+            "(or __graft_entry__.build()); there is no fallback path")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    h = C.c_void_p
+    L.lbm_create.argtypes = [C.POINTER(_Config), C.POINTER(h)]
+    L.lbm_destroy.argtypes = [h]
+    L.lbm_init_exact.argtypes = [h, dp]
+    L.lbm_init_seeded.argtypes = [h, C.c_uint64, C.c_double]
+    L.lbm_step.argtypes = [h, C.c_int32]
+    L.lbm_step_timed.argtypes = [h, C.c_int32, dp]
+    L.lbm_sync.argtypes = [h]
+    L.lbm_itc.argtypes = [h, C.POINTER(C.c_int64)]
+    L.lbm_get_params.argtypes = [h, C.POINTER(_Params)]
+    L.lbm_macros.argtypes = [h, dp, dp, dp, dp, dp]
+    L.lbm_check.argtypes = [h, dp, dp]
+    L.lbm_diagnostics.argtypes = [h, C.POINTER(_Diag)]
+    L.lbm_gather.argtypes = [h, dp, dp, dp]
+    L.lbm_scatter.argtypes = [h, dp, dp]
+    L.lbm_bytes_per_update.argtypes = [h, dp]
+    L.lbm_last_error.argtypes = [h]
+    L.lbm_last_error.restype = C.c_char_p
+    L.lbm_error_string.argtypes = [C.c_int]
+    L.lbm_error_string.restype = C.c_char_p
+    for name in EXPORTS:
+        if name not in ("lbm_last_error", "lbm_error_string"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class Config:
+    """Run-time form of the reference's compile-time switches (2DRB/3DRB.F90:9-136)."""
+    dims: int = 3
+    nx: int = 128
+    ny: int = 128
+    nz: int = 128
+    rayleigh: float = 1.0e6
+    prandtl: float = 0.71
+    mach: float = 0.1
+    thot: float = 0.5
+    tcold: float = -0.5
+    side_heated: int = 0
+    vel_x: int = VEL_NOSLIP
+    vel_z: int = VEL_NOSLIP
+    temp_x: int = T_ADIABATIC
+    temp_y: int = T_CONST
+    temp_z: int = T_ADIABATIC
+    use_g: int = 1
+    precision: int = FP64
+    device: int = 0
+
+    @property
+    def shape(self):
+        return (self.nz if self.dims == 3 else 1, self.ny, self.nx)
+
+    @property
+    def qf(self):
+        return 19 if self.dims == 3 else 9
+
+    @property
+    def qt(self):
+        return 7 if self.dims == 3 else 5
+
+    def c(self) -> _Config:
+        d = asdict(self)
+        return _Config(**d)
+
+
+class Solver:
+    """One GPU domain.  Methods follow the reference driver's phases."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.lib = library()
+        self._h = C.c_void_p()
+        self._c = cfg.c()
+        rc = self.lib.lbm_create(C.byref(self._c), C.byref(self._h))
+        if rc != 0:
+            raise LbmError(rc, self.lib.lbm_error_string(rc).decode())
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self.lib.lbm_last_error(self._h).decode() or self.lib.lbm_error_string(rc).decode()
+            raise LbmError(rc, msg)
+
+    def close(self):
+        if self._h:
+            self.lib.lbm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- initial (2DRB/3DRB.F90:342-643) -----------------------------------------------------
+    def initial(self, T0=None):
+        if T0 is not None:
+            T0 = np.ascontiguousarray(T0, dtype=np.float64).reshape(self.cfg.shape)
+        self._chk(self.lib.lbm_init_exact(self._h, _p(T0)))
+
+    def initial_seeded(self, seed: int, amp: float = 1.0e-3):
+        self._chk(self.lib.lbm_init_seeded(self._h, seed, amp))
+
+    # -- time loop body ---------------------------------------------------------------------
+    def step(self, n: int = 1):
+        self._chk(self.lib.lbm_step(self._h, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        ms = np.zeros(1)
+        self._chk(self.lib.lbm_step_timed(self._h, int(n), _p(ms)))
+        return float(ms[0])
+
+    def sync(self):
+        self._chk(self.lib.lbm_sync(self._h))
+
+    @property
+    def itc(self) -> int:
+        v = C.c_int64()
+        self._chk(self.lib.lbm_itc(self._h, C.byref(v)))
+        return v.value
+
+    def params(self) -> dict:
+        p = _Params()
+        self._chk(self.lib.lbm_get_params(self._h, C.byref(p)))
+        return {n: getattr(p, n) for n, _ in _Params._fields_}
+
+    def bytes_per_update(self) -> float:
+        b = np.zeros(1)
+        self._chk(self.lib.lbm_bytes_per_update(self._h, _p(b)))
+        return float(b[0])
+
+    # -- observation ------------------------------------------------------------------------
+    def macros(self) -> dict:
+        out = {k: np.empty(self.cfg.shape) for k in ("rho", "u", "v", "w", "T")}
+        self._chk(self.lib.lbm_macros(self._h, *[_p(out[k]) for k in ("rho", "u", "v", "w", "T")]))
+        return out
+
+    def check(self):
+        eu, et = np.zeros(1), np.zeros(1)
+        self._chk(self.lib.lbm_check(self._h, _p(eu), _p(et)))
+        return float(eu[0]), float(et[0])
+
+    def diagnostics(self) -> dict:
+        d = _Diag()
+        self._chk(self.lib.lbm_diagnostics(self._h, C.byref(d)))
+        return {n: getattr(d, n) for n, _ in _Diag._fields_}
+
+    def gather(self, with_bprev: bool = True) -> dict:
+        sh = self.cfg.shape
+        f = np.empty(sh + (self.cfg.qf,))
+        g = np.empty(sh + (self.cfg.qt,))
+        b = np.empty((3,) + sh) if with_bprev else None
+        self._chk(self.lib.lbm_gather(self._h, _p(f), _p(g), _p(b)))
+        out = {"f": f, "g": g}
+        if b is not None:
+            out["bx"], out["by"], out["bz"] = b[0], b[1], b[2]
+        return out
+
+    def scatter(self, f, g):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        self._chk(self.lib.lbm_scatter(self._h, _p(f), _p(g)))

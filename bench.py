@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""Throughput of the coupled D3Q19+D3Q7 MRT-LBM Rayleigh-Benard step on one MI355X.
+
+Workload: 3D cubic cavity, 512^3 nodes, Ra=1e8, Pr=0.71, Ma=0.1, hot bottom / cold top
+isothermal walls, adiabatic no-slip side walls, seeded initial T (linear profile + uniform
+noise +-1e-3 dT), rho=1, u=0.  A "step" is one full time step of both lattices over the whole
+domain; inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line:
+  value      coupled GLUPS = nodes * steps / seconds / 1e9 (device time of the timed steps)
+  roofline   algorithmic bytes per node update (464 B in FP64: both lattices read and written
+             once plus the M1G history) * nodes / step time, against the 8 TB/s HBM3E peak;
+             traffic = measured HBM bytes per step from rocprofv3 FETCH_SIZE/WRITE_SIZE
+             (profiles/traffic_fp64_512.json), or null when that file is absent
+  cpu_baseline  the FP64 OpenMP oracle (a restatement of the reference's CPU path) timed on
+             this host over a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "-2d-3d-_amd"))
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20261203
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--rayleigh", type=float, default=1e8)
+    ap.add_argument("--precision", choices=["fp64"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-nz", type=int, default=64)
+    return ap.parse_args()
+
+
+def cpu_baseline(size, nz, ra, steps):
+    """Time the oracle on a 'size x size x nz' slab of the same cavity (same parameters)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    os.environ.setdefault("OMP_PROC_BIND", "true")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    import orc  # the checker / baseline only; never the measured GPU path
+
+    cfg = orc.Config(dims=3, nx=size, ny=size, nz=nz, rayleigh=ra)
+    o = orc.Oracle(cfg)
+    o.init_seeded(SEED, 1e-3)
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    n = size * size * nz
+    o.close()
+    return {"value": n * steps / dt / 1e9, "unit": "GLUPS", "cores": cores, "kind": "port",
+            "sample": f"{size}x{size}x{nz} slab of the {size}^3 Ra={ra:g} cavity, FP64 OpenMP "
+                      f"oracle, {steps} timed steps after 1 warm-up step ({dt:.1f} s)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); the solver "
+              "runs a single domain per device and has no multi-GPU path.", file=sys.stderr)
+        sys.exit(2)
+
+    import rblbm
+
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:
+        torch, have_torch = None, False
+
+    n = a.size
+    cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=a.rayleigh, prandtl=0.71, mach=0.1,
+                       precision=rblbm.FP64)
+    sol = rblbm.Solver(cfg)
+    sol.initial_seeded(SEED, 1e-3)
+    mass0 = sol.diagnostics()["mass"]
+    sol.step(a.warmup)
+    sol.sync()
+    if have_torch:
+        torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    ms = sol.step_timed(a.steps)          # HIP events on the solver's own stream
+    if have_torch:
+        torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    nodes = n * n * n
+    glups = nodes * a.steps / (ms / 1e3) / 1e9
+    diag = sol.diagnostics()
+    bpu = sol.bytes_per_update()
+    ms_step = ms / a.steps
+    achieved = bpu * nodes / (ms_step / 1e3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_fp64_512.json")
+    if os.path.exists(tpath) and n == 512:
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_step")
+
+    cpu = None
+    if not a.no_cpu:
+        sol.close()
+        cpu = cpu_baseline(n, min(a.cpu_nz, n), a.rayleigh, a.cpu_steps)
+
+    out = {
+        "metric": "coupled D3Q19+D3Q7 GLUPS & %HBM roofline, FP32-shift vs FP64 at 512^3, Nu err",
+        "value": glups, "unit": "GLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D RB cavity {n}^3 Ra={a.rayleigh:g} Pr=0.71 Ma=0.1, FP64 "
+                               "populations, seeded T noise 1e-3",
+                   "nodes": nodes, "precision": "fp64", "wall_seconds": t_wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_update": bpu, "kernel": "k3_thermal + k3_flow (one step)"},
+        "cpu_baseline": cpu,
+        "checks": {"nu_vol": diag["nu_vol"], "re_vol": diag["re_vol"],
+                   "mass_rel_change": (diag["mass"] - mass0) / mass0},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,130 @@
+/*
+ * lbm_rb.h -- C ABI of the MI355X (gfx950) thermal MRT lattice-Boltzmann Rayleigh-Benard solver.
+ *
+ * The reference (19134984/-2D-3D-, Fortran 90) has no API: its hot path is the body of the time
+ * loop in 2DRB/3DRB.F90:225-264 (eight argument-less subroutines over module globals) plus the
+ * end-of-run diagnostics at 2DRB/3DRB.F90:289-297, and likewise 2DRB/2DRB.F90:236-304 in 2D.
+ * Each entry point below names the reference code it replaces.  A Fortran driver binds these
+ * through iso_c_binding (-2d-3d-_amd/fortran/lbm_rb.f90); Python binds them through ctypes.
+ *
+ * Conventions
+ *   - Every function returns LBM_OK (0) or a negative LBM_ERR_* code; nothing throws across
+ *     the ABI.  lbm_last_error() gives a message for the last failure on a handle.
+ *   - Host arrays use the reference's Fortran order: macros (nx,ny,nz) with x fastest;
+ *     populations f(0:q-1,nx,ny,nz) with the population index fastest, in the reference's
+ *     direction order (2DRB/3DRB.F90:656-658, 2DRB/2DRB.F90:188-190).
+ *   - One handle owns one GPU domain and one HIP stream; lbm_step is asynchronous on it.
+ *     A handle is not re-entrant; distinct handles may be used from distinct threads.
+ */
+#ifndef LBM_RB_H
+#define LBM_RB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_ARG (-1)          /* invalid argument or configuration */
+#define LBM_ERR_HIP (-2)          /* a HIP runtime call failed */
+#define LBM_ERR_NOMEM (-3)        /* device or host allocation failed */
+#define LBM_ERR_STATE (-4)        /* call out of order (e.g. step before init) */
+#define LBM_ERR_UNSUPPORTED (-5)  /* configuration not built (e.g. legacy thermal branch) */
+#define LBM_ERR_RHO (-6)          /* non-positive density during restart reconstruction */
+
+/* Wall conditions: the reference's cpp switches (2DRB/3DRB.F90:13-37, 2DRB/2DRB.F90:14-30). */
+#define LBM_VEL_NOSLIP 0          /* *WallsNoslip: half-way bounce-back */
+#define LBM_VEL_PERIODIC 1        /* *WallsPeriodicalU: same-row periodic copy */
+#define LBM_T_CONST 0             /* *WallsConstT: anti-bounce-back, hot at the low index */
+#define LBM_T_ADIABATIC 1         /* *WallsAdiabatic: bounce-back */
+#define LBM_T_PERIODIC 2          /* *WallsPeriodicalT */
+
+/* Population storage forms. */
+#define LBM_FP64 0                /* plain FP64 populations and arithmetic */
+#define LBM_FP32_SHIFT 1          /* FP32 deviations f - w rho0, g - wT T0; FP32 arithmetic */
+#define LBM_FP16_SHIFT 2          /* FP16 scaled deviations; FP32 arithmetic */
+
+typedef struct lbm_config {
+    int32_t dims;                 /* 2 (D2Q9 + D2Q5) or 3 (D3Q19 + D3Q7) */
+    int32_t nx, ny, nz;           /* interior nodes (nz ignored in 2D) */
+    double rayleigh, prandtl, mach;
+    double thot, tcold;
+    int32_t side_heated;          /* 0: RayleighBenardCell (L = ny); 1: SideHeatedCell (L = nx) */
+    int32_t vel_x, vel_z;         /* LBM_VEL_* for x (vertical) and z (spanwise) walls; y is no-slip */
+    int32_t temp_x, temp_y, temp_z; /* LBM_T_* */
+    int32_t use_g;                /* EnableUseG: M1G heat-flux correction */
+    int32_t precision;            /* LBM_FP64 | LBM_FP32_SHIFT | LBM_FP16_SHIFT */
+    int32_t device;               /* HIP device ordinal */
+} lbm_config;
+
+/* Derived parameters: module commondata3d, 2DRB/3DRB.F90:73-123 (2D: 2DRB/2DRB.F90:61-115). */
+typedef struct lbm_params {
+    double L, tref, tauf, viscosity, diffusivity, gbeta, time_unit, velocity_unit;
+    double s_e, s_eps, s_nu, s_pi, s_q, s_m;
+    double taug, q_k, q_nu, sg;
+} lbm_params;
+
+/* Diagnostics, FP64 with compensated sums. */
+typedef struct lbm_diag {
+    double nu_vol, re_vol;        /* calNuRe, 2DRB/3DRB.F90:1712-1778 */
+    double nu_global;             /* RBcalc_Nu_global, 2DRB/3DRB.F90:1839-1878 */
+    double nu_hot, nu_cold, nu_middle;       /* RBcalc_Nu_wall_avg, 2DRB/3DRB.F90:2049-2201 */
+    double nu_hot_max, nu_hot_max_pos, nu_hot_min, nu_hot_min_pos;
+    double mass, heat, kinetic;   /* sum rho, sum T, 0.5 sum rho|u|^2 (conservation checks) */
+} lbm_diag;
+
+typedef struct lbm_handle lbm_handle;
+
+/* Allocate the domain on cfg->device.  Replaces the allocate block of `initial`
+ * (2DRB/3DRB.F90:467-473) and the parameter module. */
+int lbm_create(const lbm_config *cfg, lbm_handle **out);
+int lbm_destroy(lbm_handle *h);
+
+/* Exact-initialisation branch of `initial` (2DRB/3DRB.F90:517-600; 2D 2DRB/2DRB.F90:540-608):
+ * rho = 1, u = 0, f = w, g = wT T, B_prev = u T.  T0 (Fortran order, nx*ny*nz) replaces the
+ * reference temperature profile when non-NULL. */
+int lbm_init_exact(lbm_handle *h, const double *T0);
+/* Exact initialisation plus seeded uniform noise amp*(thot-tcold)*[-1,1) on every node
+ * (counter-based SplitMix64 of seed and linear node index; no reference counterpart). */
+int lbm_init_seeded(lbm_handle *h, uint64_t seed, double amp);
+
+/* Advance nsteps time steps: each is one pass of collision, streaming, bounceback, macro,
+ * collisionT, streamingT, bouncebackT, macroT (2DRB/3DRB.F90:227-243). Asynchronous. */
+int lbm_step(lbm_handle *h, int32_t nsteps);
+/* Same, bracketed by HIP events on the handle's stream; *ms is the elapsed device time. */
+int lbm_step_timed(lbm_handle *h, int32_t nsteps, double *ms);
+int lbm_sync(lbm_handle *h);
+int lbm_itc(lbm_handle *h, int64_t *itc);
+int lbm_get_params(lbm_handle *h, lbm_params *p);
+
+/* rho, u, v, w, T after the last step (module arrays of 2DRB/3DRB.F90:155-157); any pointer
+ * may be NULL.  Synchronous.  In 2D, w is filled with zeros. */
+int lbm_macros(lbm_handle *h, double *rho, double *u, double *v, double *w, double *T);
+
+/* Steady-state convergence check, `check` (2DRB/3DRB.F90:1475-1528; 2D 2DRB/2DRB.F90:1213-1260):
+ * relative L2 change of u and relative L1 change of T since the previous call. */
+int lbm_check(lbm_handle *h, double *error_u, double *error_t);
+
+/* Nusselt / Reynolds / conservation diagnostics of the current state. */
+int lbm_diagnostics(lbm_handle *h, lbm_diag *d);
+
+/* f, g (reference AoS order, absolute FP64) and, if non-NULL, B_prev (3 planes of nx*ny*nz:
+ * Bx_prev, By_prev, Bz_prev, 2DRB/3DRB.F90:163).  Restart data of backupData
+ * (2DRB/3DRB.F90:1638-1665). */
+int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev);
+/* Load f, g and rebuild the macros like reconstruct_macro_from_fg (2DRB/3DRB.F90:1392-1463,
+ * 2DRB/2DRB.F90:1144-1200).  Returns LBM_ERR_RHO if any rho <= 0. */
+int lbm_scatter(lbm_handle *h, const double *f, const double *g);
+
+/* Bytes per cell update that the step moves at algorithmic minimum (both lattices read and
+ * written once, plus the M1G history): 464 (FP64 3D), 232 (FP32), 128 (FP16), 256 (FP64 2D). */
+int lbm_bytes_per_update(lbm_handle *h, double *bytes);
+
+const char *lbm_last_error(lbm_handle *h);
+const char *lbm_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
