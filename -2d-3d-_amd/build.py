@@ -13,7 +13,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "librblbm.so")
-SOURCES = ["lbm_kernels.hip", "lbm_api.hip"]
+# FP64 kernels and host code: no contraction, so FP64 results round exactly like the
+# reference's CPU build.
+SOURCES = {"lbm_kernels.hip": ["-ffp-contract=off"], "lbm_api.hip": ["-ffp-contract=off"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RBLBM_ARCH", "gfx950")
 
@@ -26,9 +28,9 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
-    for src in SOURCES:
+    for src, extra in SOURCES.items():
         obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

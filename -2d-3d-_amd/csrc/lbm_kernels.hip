@@ -11,6 +11,10 @@
 // f(a, x) = f_post(a, x - e_a) and applies the walls inline (the reference's streaming +
 // bounceback sweeps, 2DRB/3DRB.F90:957-1058 and :1209-1351).
 //
+// Arithmetic: this translation unit is compiled with -ffp-contract=off and the collisions follow
+// the reference's expression order (lbm_collide_ref.h), so the FP64 form is bitwise identical
+// to the reference's CPU path on the same inputs.
+//
 // One reference time step n (collision .. macroT, 2DRB/3DRB.F90:227-243) is two launches:
 //   A(n): pull f^{n+1} from F (f_post^n) -> rho, u^{n+1} with the half force of T^n;
 //         pull g^n from G (g_post^{n-1}) -> T^n; thermal collision -> g_post^n; B_prev.
@@ -20,7 +24,9 @@
 // collision at its six neighbours, which needs u^{n+1} there: two neighbour hops per step,
 // hence two passes.  A "prologue" launch performs the flow collision of step 1 from the plain
 // initial state, and the first A launch reads g^1 and T^1 unpulled from that state.
-#include "lbm_collide.h"
+#include <utility>
+
+#include "lbm_collide_ref.h"
 #include "lbm_internal.h"
 
 namespace rblbm {
@@ -72,40 +78,62 @@ __device__ __forceinline__ void pull_flow(const double *__restrict__ P, uint64_t
 // Thermal lattice (one non-zero component per direction, so only one wall applies):
 //   periodic -> g_post(a, wrap); constant T -> -g_post(opp a, x) + 2 wT_a T_wall (hot wall at
 //   the low index); adiabatic -> g_post(opp a, x).
+// The direction is a template argument so that its axis and sign are compile-time constants.
+template <class L, int A>
+__device__ __forceinline__ double heat_in(const double *__restrict__ P, uint64_t N, int i, int j,
+                                          int k, uint64_t c, const KParams &kp)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+    int kind = -1;
+    int64_t wrap = 0;
+    if (ex != 0) {
+        if ((unsigned)(i - ex) >= (unsigned)kp.nx) {
+            kind = kp.temp_x;
+            wrap = ex == 1 ? kp.nx - 1 - i : -i;
+        }
+    } else if (ey != 0) {
+        if ((unsigned)(j - ey) >= (unsigned)kp.ny) {
+            kind = kp.temp_y;
+            wrap = (int64_t)(ey == 1 ? kp.ny - 1 - j : -j) * sy;
+        }
+    } else if (ez != 0) {
+        if ((unsigned)(k - ez) >= (unsigned)kp.nz) {
+            kind = kp.temp_z;
+            wrap = (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz;
+        }
+    }
+    if (kind < 0) return P[A * N + c - (int64_t)(ex + ey * sy + ez * sz)];
+    if (kind == T_PERIODIC) return P[A * N + c + wrap];
+    const double back = P[L::opp[A] * N + c];
+    if (kind == T_ADIABATIC) return back;
+    constexpr double wa = (L::Q == 7) ? w7(A) : w5(A);
+    constexpr bool low_wall = (ex + ey + ez) > 0;   // entered through the low-index wall
+    const double tw = low_wall ? kp.thot : kp.tcold;
+    return -back + 2.0 * wa * tw;
+}
+
+template <class L, int... A>
+__device__ __forceinline__ void heat_in_all(const double *__restrict__ P, uint64_t N, int i, int j,
+                                            int k, uint64_t c, const KParams &kp, double *g,
+                                            std::integer_sequence<int, A...>)
+{
+    ((g[A] = heat_in<L, A>(P, N, i, j, k, c, kp)), ...);
+}
+
 template <class L>
 __device__ __forceinline__ void pull_heat(const double *__restrict__ P, uint64_t N, int i, int j,
                                           int k, uint64_t c, bool border, const KParams &kp,
                                           double *g)
 {
-    const int sy = kp.nx, sz = kp.nx * kp.ny;
     if (!border) {
+        const int sy = kp.nx, sz = kp.nx * kp.ny;
 #pragma unroll
         for (int a = 0; a < L::Q; ++a)
             g[a] = P[a * N + c - (L::ex[a] + L::ey[a] * sy + L::ez[a] * sz)];
         return;
     }
-#pragma unroll
-    for (int a = 0; a < L::Q; ++a) {
-        const int ex = L::ex[a], ey = L::ey[a], ez = L::ez[a];
-        const int si = i - ex, sj = j - ey, sk = k - ez;
-        int kind = -1, s = 0;
-        int64_t wrap = 0;
-        if ((unsigned)si >= (unsigned)kp.nx) {
-            kind = kp.temp_x; s = ex;
-            wrap = ex == 1 ? kp.nx - 1 - i : -i;
-        } else if ((unsigned)sj >= (unsigned)kp.ny) {
-            kind = kp.temp_y; s = ey;
-            wrap = (int64_t)(ey == 1 ? kp.ny - 1 - j : -j) * sy;
-        } else if ((unsigned)sk >= (unsigned)kp.nz) {
-            kind = kp.temp_z; s = ez;
-            wrap = (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz;
-        }
-        const double wa = (L::Q == 7) ? w7(a) : w5(a);
-        if (kind < 0) g[a] = P[a * N + c - (int64_t)(ex + ey * sy + ez * sz)];
-        else if (kind == T_PERIODIC) g[a] = P[a * N + c + wrap];
-        else if (kind == T_CONST) g[a] = -P[L::opp[a] * N + c] + 2.0 * wa * (s > 0 ? kp.thot : kp.tcold);
-        else g[a] = P[L::opp[a] * N + c];
-    }
+    heat_in_all<L>(P, N, i, j, k, c, kp, g, std::make_integer_sequence<int, L::Q>{});
 }
 
 __device__ __forceinline__ bool on_border(int i, int j, int k, const KParams &kp)
@@ -160,7 +188,7 @@ __global__ void __launch_bounds__(256) k3_prologue(KParams kp, const double *__r
     double f[19];
 #pragma unroll
     for (int a = 0; a < 19; ++a) f[a] = Fin[a * N + c];
-    collide_d3q19(f, f, mac[c], mac[N + c], mac[2 * N + c], mac[3 * N + c], mac[4 * N + c], kp);
+    ref_collide_d3q19(f, f, mac[c], mac[N + c], mac[2 * N + c], mac[3 * N + c], mac[4 * N + c], kp);
 #pragma unroll
     for (int a = 0; a < 19; ++a) Fout[a * N + c] = f[a];
 }
@@ -197,7 +225,7 @@ __global__ void __launch_bounds__(256) k3_thermal(KParams kp, const double *__re
     const double w = jz19(f) / rho;
     double bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
     double p[7];
-    collide_d3q7(g, p, Tn, u, v, w, bp, kp);
+    ref_collide_d3q7(g, p, Tn, u, v, w, bp, kp);
 #pragma unroll
     for (int a = 0; a < 7; ++a) Gout[a * N + c] = p[a];
     if (kp.use_g) {
@@ -229,7 +257,7 @@ __global__ void __launch_bounds__(256) k3_flow(KParams kp, const double *__restr
     const double u = jx19(f) / rho;
     const double v = (jy19(f) + 0.5 * Fy) / rho;
     const double w = jz19(f) / rho;
-    collide_d3q19(f, f, rho, u, v, w, T1, kp);
+    ref_collide_d3q19(f, f, rho, u, v, w, T1, kp);
 #pragma unroll
     for (int a = 0; a < 19; ++a) Fnext[a * N + c] = f[a];
 }
@@ -286,7 +314,7 @@ __global__ void __launch_bounds__(256) k2_prologue(KParams kp, const double *__r
     double f[9];
 #pragma unroll
     for (int a = 0; a < 9; ++a) f[a] = Fin[a * N + c];
-    fy_out[c] = collide_d2q9(f, f, mac[c], mac[N + c], mac[2 * N + c], mac[4 * N + c], kp);
+    fy_out[c] = ref_collide_d2q9(f, f, mac[c], mac[N + c], mac[2 * N + c], mac[4 * N + c], kp);
 #pragma unroll
     for (int a = 0; a < 9; ++a) Fout[a * N + c] = f[a];
 }
@@ -320,7 +348,7 @@ __global__ void __launch_bounds__(256) k2_thermal(KParams kp, const double *__re
     const double v = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8] + 0.5 * fy_cur[c]) / rho;
     double bp[2] = {Bp[c], Bp[N + c]};
     double p[5];
-    collide_d2q5(g, p, Tn, u, v, bp, kp);
+    ref_collide_d2q5(g, p, Tn, u, v, bp, kp);
 #pragma unroll
     for (int a = 0; a < 5; ++a) Gout[a * N + c] = p[a];
     if (kp.use_g) {
@@ -348,7 +376,7 @@ __global__ void __launch_bounds__(256) k2_flow(KParams kp, const double *__restr
     const double rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
     const double u = (f[1] - f[3] + f[5] - f[6] - f[7] + f[8] + 0.5 * 0.0) / rho;
     const double v = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8] + 0.5 * fy_cur[c]) / rho;
-    fy_next[c] = collide_d2q9(f, f, rho, u, v, T1, kp);
+    fy_next[c] = ref_collide_d2q9(f, f, rho, u, v, T1, kp);
 #pragma unroll
     for (int a = 0; a < 9; ++a) Fnext[a * N + c] = f[a];
 }
@@ -401,7 +429,6 @@ __global__ void __launch_bounds__(256) k_init(KParams kp, InitParams ip, double 
                                               double *__restrict__ Gout, double *__restrict__ Bp,
                                               double *__restrict__ mac, int qf, int qt)
 {
-#pragma clang fp contract(off)
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
@@ -427,13 +454,13 @@ __global__ void __launch_bounds__(256) k_init(KParams kp, InitParams ip, double 
     if (ip.noise_scale != 0.0) {
         const uint64_t key = ip.seed + 0x9E3779B97F4A7C15ULL * (c + 1ULL);
         const double r = (double)(mix64(key) >> 11) * 0x1.0p-53;
-        const double noise = __dmul_rn(ip.noise_scale, __dsub_rn(__dmul_rn(2.0, r), 1.0));
-        T = __dadd_rn(T, noise);
+        const double noise = ip.noise_scale * (2.0 * r - 1.0);
+        T = T + noise;
     }
     for (int a = 0; a < qf; ++a)
         Fout[a * N + c] = qf == 19 ? w19(a) : w9(a);
     for (int a = 0; a < qt; ++a)
-        Gout[a * N + c] = __dmul_rn(qt == 7 ? w7(a) : w5(a), T);
+        Gout[a * N + c] = (qt == 7 ? w7(a) : w5(a)) * T;
     Bp[c] = 0.0;
     Bp[N + c] = 0.0;
     Bp[2 * N + c] = 0.0;

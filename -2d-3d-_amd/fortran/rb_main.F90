@@ -1,0 +1,224 @@
+!==================================================================================================
+! rb_main -- Rayleigh-Benard driver with the reference's program structure (2DRB/3DRB.F90:198-330,
+! 2DRB/2DRB.F90:202-358), stepping the gfx950 solver through the lbm_rb C ABI.
+!
+!   initial   -> lbm_create + lbm_init_exact / lbm_init_seeded   (or restart: read f, g + scatter)
+!   time loop -> lbm_step (the eight sweeps), check every checkInterval steps
+!   end       -> lbm_diagnostics (RBcalc_Nu_global, RBcalc_Nu_wall_avg, calNuRe), MLUPS log,
+!                optional output_binary / backupData files in the reference's formats
+!
+! Parameters keep the reference's names (module commondata3d) and defaults; a namelist file
+! given as the first command-line argument overrides them at run time:
+!   &rb dims=3, nx=128, ny=128, nz=128, Rayleigh=1.0d6, itc_max=4999 /
+! As in the reference the loop runs while (errorU>epsU or errorT>epsT) and itc<=itc_max, i.e.
+! itc_max+1 steps when the run does not converge.
+!==================================================================================================
+program rb_main
+  use iso_c_binding
+  use lbm_rb
+  implicit none
+
+  ! ---- commondata3d ------------------------------------------------------------------------
+  integer :: dims = 3
+  integer :: nx = 120, ny = 120, nz = 120
+  real(8) :: Rayleigh = 1.0d7, Prandtl = 0.71d0, Mach = 0.1d0
+  real(8) :: Thot = 0.5d0, Tcold = -0.5d0
+  integer :: itc_max = 20000000
+  integer :: checkInterval = 2000
+  real(8) :: epsU = 1.0d-7, epsT = 1.0d-7
+  integer :: sideHeated = 0
+  integer :: velX = LBM_VEL_NOSLIP, velZ = LBM_VEL_NOSLIP
+  integer :: tempX = LBM_T_ADIABATIC, tempY = LBM_T_CONST, tempZ = LBM_T_ADIABATIC
+  integer :: useG = 1
+  integer :: precision = LBM_FP64
+  integer :: device = 0
+  integer :: seeded = 1
+  integer(8) :: seed = 20261203_8
+  real(8) :: noiseAmp = 1.0d-3
+  integer :: loadInitField = 0
+  character(len=256) :: reloadFile = 'backupFile3D-0.bin'
+  integer :: outputBinFile = 0
+  integer :: backupAtEnd = 0
+  character(len=256) :: settingsFile = 'SimulationSettings3D.txt'
+  namelist /rb/ dims, nx, ny, nz, Rayleigh, Prandtl, Mach, Thot, Tcold, itc_max, checkInterval, &
+       epsU, epsT, sideHeated, velX, velZ, tempX, tempY, tempZ, useG, precision, device, &
+       seeded, seed, noiseAmp, loadInitField, reloadFile, outputBinFile, backupAtEnd, settingsFile
+
+  type(lbm_config) :: cfg
+  type(lbm_params) :: prm
+  type(lbm_diag) :: dg
+  type(c_ptr) :: h = c_null_ptr
+  integer :: itc, nchunk, qf, qt, ios
+  real(8) :: errorU, errorT
+  integer(8) :: t0, t1, rate
+  real(8) :: seconds
+  character(len=256) :: nmlFile
+
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, nmlFile)
+    open(unit=10, file=trim(nmlFile), status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(*,*) 'cannot open namelist file ', trim(nmlFile)
+      stop 1
+    endif
+    read(10, nml=rb)
+    close(10)
+  endif
+  if (dims == 2) nz = 1
+  qf = merge(19, 9, dims == 3)
+  qt = merge(7, 5, dims == 3)
+
+  open(unit=00, file=trim(settingsFile), status='unknown')
+  write(00,*) 'Start: gfx950 MRT-LBM Rayleigh-Benard driver'
+  close(00)
+
+  call initial()
+
+  errorU = 100.0d0
+  errorT = 100.0d0
+  itc = 0
+  call system_clock(t0, rate)
+  do while (((errorU > epsU) .or. (errorT > epsT)) .and. (itc <= itc_max))
+    ! advance to the next check point (or to the last step) in one call
+    nchunk = min(checkInterval - mod(itc, checkInterval), itc_max + 1 - itc)
+    call lbm_require(lbm_step(h, int(nchunk, c_int32_t)), 'lbm_step', h, 6)
+    itc = itc + nchunk
+    if (mod(itc, checkInterval) == 0) call check()
+  enddo
+  call lbm_require(lbm_sync(h), 'lbm_sync', h, 6)
+  call system_clock(t1)
+  seconds = dble(t1 - t0) / dble(rate)
+
+  call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
+  if (outputBinFile == 1) call output_binary()
+  if (backupAtEnd == 1) call backupData()
+
+  write(*,'(a,1x,es16.8)') 'Nu_global =', dg%nu_global
+  write(*,'(a,1x,es16.8)') 'Nu_hot(bottom) =', dg%nu_hot
+  write(*,'(a,1x,es16.8)') 'Nu_cold(top)   =', dg%nu_cold
+  write(*,'(a,1x,es16.8)') 'Nu_middle      =', dg%nu_middle
+  write(*,'(a,1x,es16.8,2x,a,1x,es16.8)') 'Nu_hot_max =', dg%nu_hot_max, 'x_max =', dg%nu_hot_max_pos
+  write(*,'(a,1x,es16.8,2x,a,1x,es16.8)') 'Nu_hot_min =', dg%nu_hot_min, 'x_min =', dg%nu_hot_min_pos
+  write(*,'(a,1x,es16.8)') 'NuVolAvg =', dg%nu_vol
+  write(*,'(a,1x,es16.8)') 'ReVolAvg =', dg%re_vol
+  write(*,'(a,1x,i0)') 'itc =', itc
+
+  open(unit=00, file=trim(settingsFile), status='unknown', position='append')
+  write(00,*) '======================================================================'
+  write(00,*) 'Time (device) = ', seconds, 's'
+  write(00,*) 'MLUPS = ', dble(nx) * dble(ny) * dble(nz) * dble(itc) / max(seconds, 1.0d-12) / 1.0d6
+  write(00,*) 'Nu_global =', dg%nu_global
+  write(00,*) 'Nu_hot    =', dg%nu_hot
+  write(00,*) 'Nu_cold   =', dg%nu_cold
+  write(00,*) 'Nu_middle =', dg%nu_middle
+  write(00,*) 'NuVolAvg  =', dg%nu_vol
+  write(00,*) 'ReVolAvg  =', dg%re_vol
+  write(00,*) 'Successfully: DNS completed!'
+  close(00)
+
+  call lbm_require(lbm_destroy(h), 'lbm_destroy', c_null_ptr, 6)
+
+contains
+
+  !------------------------------------------------------------------------------------------
+  ! initial (2DRB/3DRB.F90:342-643): parameters, allocation, exact or restart initialisation
+  !------------------------------------------------------------------------------------------
+  subroutine initial()
+    real(8), allocatable, target :: f(:,:,:,:), g(:,:,:,:)
+    integer :: i, j, k, alpha
+
+    cfg%dims = dims; cfg%nx = nx; cfg%ny = ny; cfg%nz = nz
+    cfg%rayleigh = Rayleigh; cfg%prandtl = Prandtl; cfg%mach = Mach
+    cfg%thot = Thot; cfg%tcold = Tcold
+    cfg%side_heated = sideHeated; cfg%vel_x = velX; cfg%vel_z = velZ
+    cfg%temp_x = tempX; cfg%temp_y = tempY; cfg%temp_z = tempZ
+    cfg%use_g = useG; cfg%precision = precision; cfg%device = device
+    call lbm_require(lbm_create(cfg, h), 'lbm_create', h, 6)
+    call lbm_require(lbm_get_params(h, prm), 'lbm_get_params', h, 6)
+
+    open(unit=00, file=trim(settingsFile), status='unknown', position='append')
+    write(00,*) '-------------------------------------------------------------------------------'
+    write(00,*) 'Mesh:', nx, ny, nz
+    write(00,*) 'Rayleigh=', Rayleigh, '; Prandtl =', Prandtl, '; Mach =', Mach
+    write(00,*) 'Length unit: L0 =', prm%L
+    write(00,*) 'Time unit: Sqrt(L0/(gBeta*DeltaT)) =', prm%time_unit
+    write(00,*) 'Velocity unit: Sqrt(gBeta*L0*DeltaT) =', prm%velocity_unit
+    write(00,*) 'tauf =', prm%tauf
+    write(00,*) 'taug =', prm%taug, '; Qk =', prm%q_k, '; Qnu =', prm%q_nu
+    write(00,*) 'viscosity =', prm%viscosity, '; diffusivity =', prm%diffusivity
+    write(00,*) 'itc_max =', itc_max
+    write(00,*) 'default epsU =', epsU, '; epsT =', epsT
+    write(00,*) 'useG =', useG == 1
+    write(00,*) 'precision form =', precision
+
+    if (loadInitField == 0) then
+      if (seeded == 1) then
+        write(00,*) 'Initial field: exact profile + seeded noise, seed =', seed, ' amp =', noiseAmp
+        call lbm_require(lbm_init_seeded(h, int(seed, c_int64_t), noiseAmp), 'lbm_init_seeded', h, 6)
+      else
+        write(00,*) 'Initial field is set exactly'
+        call lbm_require(lbm_init_exact(h, c_null_ptr), 'lbm_init_exact', h, 6)
+      endif
+    else
+      ! restart: f then g records, interior nodes, population index outermost (2DRB/3DRB.F90:611-616)
+      allocate(f(0:qf-1, nx, ny, nz), g(0:qt-1, nx, ny, nz))
+      open(unit=01, file=trim(reloadFile), form='unformatted', access='sequential', status='old')
+      read(01) ((((f(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qf-1)
+      read(01) ((((g(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qt-1)
+      close(01)
+      write(00,*) 'Raw data is loaded from the file: ', trim(reloadFile)
+      call lbm_require(lbm_scatter(h, c_loc(f), c_loc(g)), 'lbm_scatter', h, 6)
+      deallocate(f, g)
+    endif
+    write(00,*) '-------------------------------------------------------------------------------'
+    close(00)
+  end subroutine initial
+
+  !------------------------------------------------------------------------------------------
+  ! check (2DRB/3DRB.F90:1475-1528)
+  !------------------------------------------------------------------------------------------
+  subroutine check()
+    call lbm_require(lbm_check(h, errorU, errorT), 'lbm_check', h, 6)
+    write(*,'(I12,1X,ES24.16,1X,ES24.16)') itc, errorU, errorT
+  end subroutine check
+
+  !------------------------------------------------------------------------------------------
+  ! output_binary (2DRB/3DRB.F90:1599-1627): u, v, w scaled by L/kappa, then T, rho
+  !------------------------------------------------------------------------------------------
+  subroutine output_binary()
+    real(8), allocatable, target :: rho(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:), T(:,:,:)
+    character(len=100) :: filename
+    real(8) :: scale
+    allocate(rho(nx,ny,nz), u(nx,ny,nz), v(nx,ny,nz), w(nx,ny,nz), T(nx,ny,nz))
+    call lbm_require(lbm_macros(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(w), c_loc(T)), &
+         'lbm_macros', h, 6)
+    scale = prm%L / prm%diffusivity
+    write(filename,'(i12.12)') itc
+    open(unit=03, file='buoyancyCavity3DbinFile-'//trim(adjustl(filename))//'.bin', &
+         form='unformatted', access='sequential')
+    write(03) scale * u
+    write(03) scale * v
+    write(03) scale * w
+    write(03) T
+    write(03) rho
+    close(03)
+  end subroutine output_binary
+
+  !------------------------------------------------------------------------------------------
+  ! backupData (2DRB/3DRB.F90:1638-1665): f then g, population index outermost
+  !------------------------------------------------------------------------------------------
+  subroutine backupData()
+    real(8), allocatable, target :: f(:,:,:,:), g(:,:,:,:)
+    character(len=100) :: filename
+    integer :: i, j, k, alpha
+    allocate(f(0:qf-1, nx, ny, nz), g(0:qt-1, nx, ny, nz))
+    call lbm_require(lbm_gather(h, c_loc(f), c_loc(g), c_null_ptr), 'lbm_gather', h, 6)
+    write(filename,'(i0)') itc
+    open(unit=05, file='backupFile3D-'//trim(filename)//'.bin', form='unformatted', access='sequential')
+    write(05) ((((f(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qf-1)
+    write(05) ((((g(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qt-1)
+    close(05)
+    deallocate(f, g)
+  end subroutine backupData
+
+end program rb_main

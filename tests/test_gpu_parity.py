@@ -86,8 +86,10 @@ def test_seeded_initial_state_bitwise():
 def test_3d_small_matches_oracle(name):
     rng = np.random.default_rng(1)
     T0 = rng.uniform(-0.3, 0.3, size=(8, 10, 12))
-    _run_pair(3, 12, 10, 8, (1, 2, 10, 60), init="exact", T0=T0, rayleigh=1e5, mach=0.3,
-              **CASES_3D[name])
+    sol, ora = _run_pair(3, 12, 10, 8, (1, 2, 10, 60), init="exact", T0=T0, rayleigh=1e5,
+                         mach=0.3, **CASES_3D[name])
+    # The FP64 form keeps the reference's expression order without contraction: bitwise equal.
+    np.testing.assert_array_equal(sol.gather()["f"], ora.state()["f"])
 
 
 CASES_2D = {
@@ -102,8 +104,9 @@ CASES_2D = {
 def test_2d_small_matches_oracle(name):
     rng = np.random.default_rng(2)
     T0 = rng.uniform(0.2, 0.8, size=(1, 10, 17))
-    _run_pair(2, 17, 10, 1, (1, 2, 10, 60), init="exact", T0=T0, rayleigh=1e5, mach=0.3,
-              thot=1.0, tcold=0.0, **CASES_2D[name])
+    sol, ora = _run_pair(2, 17, 10, 1, (1, 2, 10, 60), init="exact", T0=T0, rayleigh=1e5,
+                         mach=0.3, thot=1.0, tcold=0.0, **CASES_2D[name])
+    np.testing.assert_array_equal(sol.gather()["f"], ora.state()["f"])
 
 
 def test_reference_init_profile_and_perturbation():
