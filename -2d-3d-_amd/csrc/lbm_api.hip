@@ -22,6 +22,10 @@ struct lbm_handle {
     uint64_t N;
     hipStream_t stream;
     hipEvent_t ev0, ev1;
+    int es;                 // bytes per stored population (8 / 4 / 2)
+    bool lp;                // reduced-precision deviation storage (3D only)
+    // F, G, Bp, aux are double in the FP64 form; F, G are float / _Float16 and Bp, aux float
+    // in the reduced-precision forms.
     double *F[2], *G[2], *Bp, *aux[2], *mac, *prev;
     double *red;            // reduction partials (device)
     double *wall;           // per-x wall profiles (device, 2 * nx)
@@ -117,7 +121,8 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
     if (c.temp_x < 0 || c.temp_x > 2 || c.temp_z < 0 || c.temp_z > 2) return LBM_ERR_ARG;
     if (c.temp_y < 0 || c.temp_y > 1) return LBM_ERR_ARG;  // no periodic option for y walls
     if (c.dims == 3 && c.temp_z == LBM_T_CONST) return LBM_ERR_UNSUPPORTED;
-    if (c.precision != LBM_FP64) return LBM_ERR_UNSUPPORTED;
+    if (c.precision < LBM_FP64 || c.precision > LBM_FP16_SHIFT) return LBM_ERR_ARG;
+    if (c.precision != LBM_FP64 && c.dims != 3) return LBM_ERR_UNSUPPORTED;
 
     lbm_handle *h = (lbm_handle *)calloc(1, sizeof(lbm_handle));
     if (!h) return LBM_ERR_NOMEM;
@@ -154,14 +159,18 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
         return LBM_ERR_HIP;
     }
     const size_t N = h->N;
-    struct { double **p; size_t n; } plan[] = {
-        {&h->F[0], N * h->qf}, {&h->F[1], N * h->qf}, {&h->G[0], N * h->qt},
-        {&h->G[1], N * h->qt}, {&h->Bp, N * 3},       {&h->aux[0], N},
-        {&h->aux[1], c.dims == 2 ? N : 1},            {&h->mac, N * 5},
-        {&h->wall, (size_t)2 * c.nx},
+    h->lp = c.precision != LBM_FP64;
+    h->es = c.precision == LBM_FP64 ? 8 : (c.precision == LBM_FP32_SHIFT ? 4 : 2);
+    const size_t ea = h->lp ? sizeof(float) : sizeof(double);   // B_prev / aux element
+    struct { double **p; size_t bytes; } plan[] = {
+        {&h->F[0], N * h->qf * h->es}, {&h->F[1], N * h->qf * h->es},
+        {&h->G[0], N * h->qt * h->es}, {&h->G[1], N * h->qt * h->es},
+        {&h->Bp, N * 3 * ea},          {&h->aux[0], N * ea},
+        {&h->aux[1], (c.dims == 2 ? N : 1) * ea},
+        {&h->mac, N * 5 * sizeof(double)}, {&h->wall, (size_t)2 * c.nx * sizeof(double)},
     };
     for (auto &b : plan) {
-        if (hipMalloc((void **)b.p, b.n * sizeof(double)) != hipSuccess) { rc = LBM_ERR_NOMEM; break; }
+        if (hipMalloc((void **)b.p, b.bytes) != hipSuccess) { rc = LBM_ERR_NOMEM; break; }
     }
     h->red_blocks = 1024;
     if (rc == LBM_OK) {
@@ -207,7 +216,18 @@ static int do_init(lbm_handle *h, const double *T0, uint64_t seed, double amp, b
     ip.noise_scale = seeded ? amp * (c.thot - c.tcold) : 0.0;
     if (T0) HIPCHK(h, hipMemcpyAsync(h->mac + 4 * h->N, T0, h->N * sizeof(double),
                                      hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, launch_init(h->kp, h->bx, h->stream, ip, h->F[0], h->G[0], h->Bp, h->mac, h->qf, h->qt));
+    if (!h->lp) {
+        HIPCHK(h, launch_init(h->kp, h->bx, h->stream, ip, h->F[0], h->G[0], h->Bp, h->mac, h->qf, h->qt));
+    } else {
+        // macros (T profile, noise) in FP64 exactly as the FP64 form, then the deviation state
+        HIPCHK(h, launch_init(h->kp, h->bx, h->stream, ip, nullptr, nullptr, nullptr, h->mac, h->qf, h->qt));
+        LPArgs a;
+        a.Fn = h->F[0];
+        a.Go = h->G[0];
+        a.Bp = (float *)h->Bp;
+        a.mac = h->mac;
+        HIPCHK(h, launch_lp(h->cfg.precision, LP_INIT, h->kp, h->bx, h->stream, a));
+    }
     if (h->prev) HIPCHK(h, hipMemsetAsync(h->prev, 0, 4 * h->N * sizeof(double), h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->phase = 0;
@@ -232,7 +252,15 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
     const bool d3 = h->cfg.dims == 3;
     if (h->phase == 0) {
         // flow collision of step 1 from the plain state (f^1 in F[0], macros in mac)
-        HIPCHK(h, launch_prologue(h->kp, h->bx, h->stream, h->F[0], h->F[1], h->mac, h->aux[1], d3));
+        if (h->lp) {
+            LPArgs a;
+            a.Fc = h->F[0];
+            a.Fn = h->F[1];
+            a.mac = h->mac;
+            HIPCHK(h, launch_lp(h->cfg.precision, LP_PROLOGUE, h->kp, h->bx, h->stream, a));
+        } else {
+            HIPCHK(h, launch_prologue(h->kp, h->bx, h->stream, h->F[0], h->F[1], h->mac, h->aux[1], d3));
+        }
         h->fcur = 1;
         h->gcur = 0;
         h->fy = 1;
@@ -244,7 +272,18 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
         double *Fn = h->F[h->fcur ^ 1];
         const double *Gi = h->G[h->gcur];
         double *Go = h->G[h->gcur ^ 1];
-        if (d3) {
+        if (h->lp) {
+            LPArgs a;
+            a.Fc = Fc;
+            a.Fn = Fn;
+            a.Gi = Gi;
+            a.Go = Go;
+            a.Bp = (float *)h->Bp;
+            a.aux = (float *)h->aux[0];
+            a.mac = h->mac;
+            a.first = h->first_thermal;
+            HIPCHK(h, launch_lp(h->cfg.precision, LP_STEP, h->kp, h->bx, h->stream, a));
+        } else if (d3) {
             HIPCHK(h, launch_step3(h->kp, h->bx, h->stream, h->first_thermal, Fc, Fn, Gi, Go, h->Bp,
                                    h->aux[0], h->mac));
         } else {
@@ -312,7 +351,7 @@ extern "C" int lbm_get_params(lbm_handle *h, lbm_params *p)
 extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
 {
     if (!h || !bytes) return LBM_ERR_ARG;
-    const double s_pop = 8.0, s_aux = 8.0;
+    const double s_pop = h->es, s_aux = h->lp ? 4.0 : 8.0;
     const int q = h->qf + h->qt, naux = h->cfg.dims == 3 ? 3 : 2;
     *bytes = 2.0 * q * s_pop + 2.0 * naux * s_aux;
     return LBM_OK;
@@ -322,7 +361,14 @@ extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
 static int refresh_macros(lbm_handle *h)
 {
     if (h->phase < 0) return fail(h, LBM_ERR_STATE, "no state: call lbm_init_* first");
-    if (h->phase == 1) {
+    if (h->phase == 1 && h->lp) {
+        LPArgs a;
+        a.Fc = h->F[h->fcur ^ 1];
+        a.Gi = h->G[h->gcur];
+        a.aux = (float *)h->aux[0];
+        a.mac = h->mac;
+        HIPCHK(h, launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a));
+    } else if (h->phase == 1) {
         const bool d3 = h->cfg.dims == 3;
         const double *aux = d3 ? h->aux[0] : h->aux[h->fy ^ 1];
         HIPCHK(h, launch_state(h->kp, h->bx, h->stream, d3, h->F[h->fcur ^ 1], h->G[h->gcur], aux,
@@ -552,7 +598,25 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
         const int n = (k0 + nk <= h->cfg.nz) ? nk : h->cfg.nz - k0;
         const uint64_t c0 = (uint64_t)k0 * plane, cnt = (uint64_t)n * plane;
         hipError_t e;
-        if (h->phase == 0) {
+        if (h->lp) {
+            LPArgs a;
+            a.fo = fo;
+            a.go = go;
+            if (h->phase == 0) {
+                a.Fc = h->F[0];
+                a.Gi = h->G[0];
+                a.c0 = c0;
+                a.n = cnt;
+                e = launch_lp(h->cfg.precision, LP_TO_ABS, h->kp, h->bx, h->stream, a);
+            } else {
+                a.Fc = h->F[h->fcur ^ 1];
+                a.Gi = h->G[h->gcur];
+                a.aux = (float *)h->aux[0];
+                a.k0 = k0;
+                a.nk = n;
+                e = launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a);
+            }
+        } else if (h->phase == 0) {
             e = launch_transpose(false, h->stream, h->F[0], fo, h->qf, h->N, c0, cnt);
             if (e == hipSuccess) e = launch_transpose(false, h->stream, h->G[0], go, h->qt, h->N, c0, cnt);
         } else {
@@ -569,9 +633,23 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
     }
     if (stage) (void)hipFree(stage);
     if (rc) return rc;
-    if (bprev) {
+    if (bprev && !h->lp) {
         HIPCHK(h, hipMemcpyAsync(bprev, h->Bp, 3 * h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
+    } else if (bprev) {
+        double *tmp = nullptr;
+        if (hipMalloc((void **)&tmp, 3 * h->N * sizeof(double)) != hipSuccess)
+            return fail(h, LBM_ERR_NOMEM, "gather B_prev staging");
+        LPArgs a;
+        a.Bp = (float *)h->Bp;
+        a.fo = tmp;
+        a.n = 3 * h->N;
+        hipError_t e = launch_lp(h->cfg.precision, LP_BPREV, h->kp, h->bx, h->stream, a);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(bprev, tmp, 3 * h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(h, LBM_ERR_HIP, "gather B_prev: %s", hipGetErrorString(e));
     }
     return LBM_OK;
 }
@@ -592,15 +670,37 @@ extern "C" int lbm_scatter(lbm_handle *h, const double *f, const double *g)
         const uint64_t c0 = (uint64_t)k0 * plane, cnt = (uint64_t)n * plane;
         hipError_t e = hipMemcpyAsync(fo, f + c0 * h->qf, cnt * h->qf * sizeof(double), hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(go, g + c0 * h->qt, cnt * h->qt * sizeof(double), hipMemcpyHostToDevice, h->stream);
-        if (e == hipSuccess) e = launch_transpose(true, h->stream, fo, h->F[0], h->qf, h->N, c0, cnt);
-        if (e == hipSuccess) e = launch_transpose(true, h->stream, go, h->G[0], h->qt, h->N, c0, cnt);
+        if (h->lp) {
+            if (e == hipSuccess && k0 == 0) e = hipMemsetAsync(h->flag, 0, sizeof(int), h->stream);
+            LPArgs a;
+            a.fo = fo;
+            a.go = go;
+            a.Fn = h->F[0];
+            a.Go = h->G[0];
+            a.mac = h->mac;
+            a.c0 = c0;
+            a.n = cnt;
+            a.flag = h->flag;
+            if (e == hipSuccess) e = launch_lp(h->cfg.precision, LP_FROM_ABS, h->kp, h->bx, h->stream, a);
+            if (e == hipSuccess) e = launch_lp(h->cfg.precision, LP_RECON_AOS, h->kp, h->bx, h->stream, a);
+        } else {
+            if (e == hipSuccess) e = launch_transpose(true, h->stream, fo, h->F[0], h->qf, h->N, c0, cnt);
+            if (e == hipSuccess) e = launch_transpose(true, h->stream, go, h->G[0], h->qt, h->N, c0, cnt);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) rc = fail(h, LBM_ERR_HIP, "scatter: %s", hipGetErrorString(e));
     }
     (void)hipFree(stage);
     if (rc) return rc;
-    HIPCHK(h, hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
-    HIPCHK(h, launch_reconstruct(h->kp, h->stream, h->F[0], h->G[0], h->Bp, h->mac, h->qf, h->qt, h->flag));
+    if (h->lp) {
+        LPArgs a;
+        a.mac = h->mac;
+        a.Bp = (float *)h->Bp;
+        HIPCHK(h, launch_lp(h->cfg.precision, LP_BP_FROM_MAC, h->kp, h->bx, h->stream, a));
+    } else {
+        HIPCHK(h, hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
+        HIPCHK(h, launch_reconstruct(h->kp, h->stream, h->F[0], h->G[0], h->Bp, h->mac, h->qf, h->qt, h->flag));
+    }
     int bad = 0;
     HIPCHK(h, hipMemcpyAsync(&bad, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

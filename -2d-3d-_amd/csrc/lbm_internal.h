@@ -41,4 +41,29 @@ hipError_t launch_reduce_walls(const KParams &kp, hipStream_t s, const double *m
 hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *mac, double *prev,
                                double *out, unsigned blocks);
 
+// Reduced-precision (FP32 / FP16 deviation storage) 3D kernels, lbm_kernels_lp.hip.
+enum {
+    LP_PROLOGUE,      // flow collision of step 1: Fc (plain) -> Fn, macros from mac
+    LP_STEP,          // passes A and B: Fc, Gi -> Go, Fn; Bp, aux updated
+    LP_STATE,         // reference state from Fc (= f_post^n), Gi (= g_post^n), aux -> mac, fo/go
+    LP_INIT,          // plain state from mac: Fn = 0, Go = wT (T - T0), Bp = u T
+    LP_FROM_ABS,      // plain state from an absolute AoS chunk fo/go [c0, c0 + n) -> Fn, Go
+    LP_TO_ABS,        // plain state Fc, Gi [c0, c0 + n) -> absolute AoS chunk fo/go
+    LP_BPREV,         // Bp (3 float planes) -> fo (FP64), n values
+    LP_RECON_AOS,     // reconstruct_macro_from_fg on the AoS chunk fo/go -> mac, *flag on rho <= 0
+    LP_BP_FROM_MAC,   // Bp = u T from mac (zero without use_g)
+};
+struct LPArgs {
+    const void *Fc = nullptr;
+    void *Fn = nullptr;
+    const void *Gi = nullptr;
+    void *Go = nullptr;
+    float *Bp = nullptr, *aux = nullptr;
+    double *mac = nullptr, *fo = nullptr, *go = nullptr;
+    int first = 0, k0 = 0, nk = 0;
+    uint64_t c0 = 0, n = 0;
+    int *flag = nullptr;
+};
+hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a);
+
 } // namespace rblbm

@@ -457,13 +457,15 @@ __global__ void __launch_bounds__(256) k_init(KParams kp, InitParams ip, double 
         const double noise = ip.noise_scale * (2.0 * r - 1.0);
         T = T + noise;
     }
-    for (int a = 0; a < qf; ++a)
-        Fout[a * N + c] = qf == 19 ? w19(a) : w9(a);
-    for (int a = 0; a < qt; ++a)
-        Gout[a * N + c] = (qt == 7 ? w7(a) : w5(a)) * T;
-    Bp[c] = 0.0;
-    Bp[N + c] = 0.0;
-    Bp[2 * N + c] = 0.0;
+    if (Fout) {   // FP64 form; the reduced-precision forms build their state from mac
+        for (int a = 0; a < qf; ++a)
+            Fout[a * N + c] = qf == 19 ? w19(a) : w9(a);
+        for (int a = 0; a < qt; ++a)
+            Gout[a * N + c] = (qt == 7 ? w7(a) : w5(a)) * T;
+        Bp[c] = 0.0;
+        Bp[N + c] = 0.0;
+        Bp[2 * N + c] = 0.0;
+    }
     mac[c] = 1.0;
     mac[N + c] = 0.0;
     mac[2 * N + c] = 0.0;

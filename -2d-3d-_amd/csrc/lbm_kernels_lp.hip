@@ -1,0 +1,604 @@
+// lbm_kernels_lp.hip -- reduced-precision storage forms of the 3D step (FP32 arithmetic).
+//
+// Populations are stored as deviations from the rest equilibrium, which keeps their magnitude
+// near the signal and away from the constant weight:
+//   h_a = f_a - w_a rho0 (rho0 = 1)            k_a = g_a - wT_a T0 (T0 = Tref)
+// LBM_FP32_SHIFT stores h, k as float; LBM_FP16_SHIFT stores h_a / w_a and k_a / wT_a as
+// _Float16 (per-direction scaling puts every direction in the same numeric range).  All
+// collision arithmetic is FP32 (FMA contraction on); the history B_prev and the stored previous
+// temperature are FP32.  Global sums stay FP64 (lbm_kernels.hip reductions run on FP64 macros
+// produced here).
+//
+// The moment-space collision is the reference's (`collision`, 2DRB/3DRB.F90:689-949;
+// `collisionT`, :1106-1198) rewritten for deviations: with M w = (1, -11, 3, 0, ...) and
+// N wT = (1, 0, 0, 0, -3/4, 0, 0), m' = M h, and the equilibria of the constant part cancel,
+// leaving m'_eq(0) = drho, m'_eq(1) = -11 drho + 19 rho u^2, m'_eq(2) = 3 drho - 11/2 rho u^2,
+// n'_eq(4) = -3/4 dT; every other equilibrium moment is unchanged.  Walls act on deviations
+// the same way (bounce-back and periodic copies are linear and w is symmetric); anti-bounce-back
+// becomes k_in = -k_out(opp) + 2 wT (T_wall - T0).
+#include <utility>
+
+#include "lbm_internal.h"
+
+namespace rblbm {
+
+// Storage policy: load/store one population of direction A, converting to/from FP32 deviations.
+template <typename S> struct Pop;
+template <> struct Pop<float> {
+    template <int Q, int A> static __device__ __forceinline__ float ld(const float *p) { return *p; }
+    template <int Q, int A> static __device__ __forceinline__ void st(float *p, float v) { *p = v; }
+};
+template <int Q> __host__ __device__ constexpr double wq(int a)
+{
+    return Q == 19 ? w19(a) : (Q == 7 ? w7(a) : (Q == 9 ? w9(a) : w5(a)));
+}
+template <> struct Pop<_Float16> {
+    // stored value = deviation / w_a
+    template <int Q, int A> static __device__ __forceinline__ float ld(const _Float16 *p)
+    {
+        return (float)(*p) * (float)wq<Q>(A);
+    }
+    template <int Q, int A> static __device__ __forceinline__ void st(_Float16 *p, float v)
+    {
+        *p = (_Float16)(v * (float)(1.0 / wq<Q>(A)));
+    }
+};
+
+struct LPParams {
+    float gbeta, sv, sq, qk, qnu, sg;
+    float dthot, dtcold;        // wall temperatures minus T0
+    double tref;                // T0
+};
+
+// -------------------------------------------------------------------------------------------
+// pulls (walls as in lbm_kernels.hip: last outside axis decides for the flow lattice)
+// -------------------------------------------------------------------------------------------
+template <class L, typename S, int A>
+__device__ __forceinline__ float flow_in(const S *__restrict__ P, uint64_t N, int i, int j, int k,
+                                         uint64_t c, bool border, const KParams &kp)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+    if (!border) return Pop<S>::template ld<L::Q, A>(P + A * N + c - (ex + ey * sy + ez * sz));
+    const bool ox = (unsigned)(i - ex) >= (unsigned)kp.nx;
+    const bool oy = (unsigned)(j - ey) >= (unsigned)kp.ny;
+    const bool oz = (unsigned)(k - ez) >= (unsigned)kp.nz;
+    if (oz) {
+        if (kp.vel_z == VEL_PERIODIC)
+            return Pop<S>::template ld<L::Q, A>(P + A * N + c + (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz);
+        return Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
+    }
+    if (oy) return Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
+    if (ox) {
+        if (kp.vel_x == VEL_PERIODIC)
+            return Pop<S>::template ld<L::Q, A>(P + A * N + c + (int64_t)(ex == 1 ? kp.nx - 1 - i : -i));
+        return Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
+    }
+    return Pop<S>::template ld<L::Q, A>(P + A * N + c - (int64_t)(ex + ey * sy + ez * sz));
+}
+
+template <class L, typename S, int A>
+__device__ __forceinline__ float heat_in(const S *__restrict__ P, uint64_t N, int i, int j, int k,
+                                         uint64_t c, bool border, const KParams &kp,
+                                         const LPParams &lp)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+    if (!border) return Pop<S>::template ld<L::Q, A>(P + A * N + c - (ex + ey * sy + ez * sz));
+    int kind = -1;
+    int64_t wrap = 0;
+    if (ex != 0) {
+        if ((unsigned)(i - ex) >= (unsigned)kp.nx) { kind = kp.temp_x; wrap = ex == 1 ? kp.nx - 1 - i : -i; }
+    } else if (ey != 0) {
+        if ((unsigned)(j - ey) >= (unsigned)kp.ny) { kind = kp.temp_y; wrap = (int64_t)(ey == 1 ? kp.ny - 1 - j : -j) * sy; }
+    } else if (ez != 0) {
+        if ((unsigned)(k - ez) >= (unsigned)kp.nz) { kind = kp.temp_z; wrap = (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz; }
+    }
+    if (kind < 0) return Pop<S>::template ld<L::Q, A>(P + A * N + c - (int64_t)(ex + ey * sy + ez * sz));
+    if (kind == T_PERIODIC) return Pop<S>::template ld<L::Q, A>(P + A * N + c + wrap);
+    const float back = Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
+    if (kind == T_ADIABATIC) return back;
+    constexpr bool low_wall = (ex + ey + ez) > 0;
+    const float tw = low_wall ? lp.dthot : lp.dtcold;
+    return -back + 2.0f * (float)wq<L::Q>(A) * tw;
+}
+
+template <class L, typename S, int... A>
+__device__ __forceinline__ void pull_flow_lp(const S *__restrict__ P, uint64_t N, int i, int j, int k,
+                                             uint64_t c, bool border, const KParams &kp, float *f,
+                                             std::integer_sequence<int, A...>)
+{
+    ((f[A] = flow_in<L, S, A>(P, N, i, j, k, c, border, kp)), ...);
+}
+template <class L, typename S, int... A>
+__device__ __forceinline__ void pull_heat_lp(const S *__restrict__ P, uint64_t N, int i, int j, int k,
+                                             uint64_t c, bool border, const KParams &kp,
+                                             const LPParams &lp, float *g,
+                                             std::integer_sequence<int, A...>)
+{
+    ((g[A] = heat_in<L, S, A>(P, N, i, j, k, c, border, kp, lp)), ...);
+}
+template <class L, typename S, int... A>
+__device__ __forceinline__ void store_all(S *__restrict__ P, uint64_t N, uint64_t c, const float *v,
+                                          std::integer_sequence<int, A...>)
+{
+    (Pop<S>::template st<L::Q, A>(P + A * N + c, v[A]), ...);
+}
+template <class L, typename S, int... A>
+__device__ __forceinline__ void load_all(const S *__restrict__ P, uint64_t N, uint64_t c, float *v,
+                                         std::integer_sequence<int, A...>)
+{
+    ((v[A] = Pop<S>::template ld<L::Q, A>(P + A * N + c)), ...);
+}
+#define SEQ(L) std::make_integer_sequence<int, L::Q>{}
+
+// -------------------------------------------------------------------------------------------
+// deviation-form collisions (FP32)
+// -------------------------------------------------------------------------------------------
+__device__ __forceinline__ void collide19_dev(const float *f, float *p, float drho, float u, float v,
+                                              float w, float dT, const LPParams &k)
+{
+    const float rho = 1.0f + drho;
+    const float f0 = f[0];
+    const float sxx = f[1] + f[2], syy = f[3] + f[4], szz = f[5] + f[6];
+    const float dxy = (f[7] + f[8]) + (f[9] + f[10]);
+    const float dxz = (f[11] + f[12]) + (f[13] + f[14]);
+    const float dyz = (f[15] + f[16]) + (f[17] + f[18]);
+    const float s1 = sxx + syy + szz, s2 = dxy + dxz + dyz;
+    const float xax = f[1] - f[2], yax = f[3] - f[4], zax = f[5] - f[6];
+    const float a78 = f[7] - f[8], a910 = f[9] - f[10], b78 = f[7] + f[8], b910 = f[9] + f[10];
+    const float a1112 = f[11] - f[12], a1314 = f[13] - f[14], b1112 = f[11] + f[12], b1314 = f[13] + f[14];
+    const float a1516 = f[15] - f[16], a1718 = f[17] - f[18], b1516 = f[15] + f[16], b1718 = f[17] + f[18];
+    const float xd = (a78 + a910) + (a1112 + a1314);
+    const float yd = (b78 - b910) + (a1516 + a1718);
+    const float zd = (b1112 - b1314) + (b1516 - b1718);
+    float m[19];
+    m[1] = -30.0f * f0 - 11.0f * s1 + 8.0f * s2;
+    m[2] = 12.0f * f0 - 4.0f * s1 + s2;
+    m[3] = xax + xd;
+    m[4] = -4.0f * xax + xd;
+    m[5] = yax + yd;
+    m[6] = -4.0f * yax + yd;
+    m[7] = zax + zd;
+    m[8] = -4.0f * zax + zd;
+    m[9] = 2.0f * sxx - syy - szz + dxy + dxz - 2.0f * dyz;
+    m[10] = -4.0f * sxx + 2.0f * syy + 2.0f * szz + dxy + dxz - 2.0f * dyz;
+    m[11] = syy - szz + dxy - dxz;
+    m[12] = -2.0f * syy + 2.0f * szz + dxy - dxz;
+    m[13] = a78 - a910;
+    m[14] = a1516 - a1718;
+    m[15] = a1112 - a1314;
+    m[16] = (a78 + a910) - (a1112 + a1314);
+    m[17] = -(b78 - b910) + (a1516 + a1718);
+    m[18] = (b1112 - b1314) - (b1516 - b1718);
+
+    const float u2 = u * u + v * v + w * w;
+    const float sv = k.sv, sq = k.sq;
+    const float cv = 1.0f - 0.5f * sv, cq = 1.0f - 0.5f * sq;
+    const float Fy = rho * k.gbeta * dT;
+    const float vF = v * Fy;
+    float h[19];
+    h[0] = drho * (1.0f / 19.0f);
+    h[1] = (m[1] - sv * (m[1] - (-11.0f * drho + 19.0f * rho * u2)) + cv * 38.0f * vF) * (1.0f / 2394.0f);
+    h[2] = (m[2] - sv * (m[2] - (3.0f * drho - 5.5f * rho * u2)) + cv * (-11.0f) * vF) * (1.0f / 252.0f);
+    h[3] = m[3] * 0.1f;
+    h[4] = (m[4] - sq * (m[4] + (2.0f / 3.0f) * rho * u)) * (1.0f / 40.0f);
+    h[5] = (m[5] + Fy) * 0.1f;
+    h[6] = (m[6] - sq * (m[6] + (2.0f / 3.0f) * rho * v) + cq * (-2.0f / 3.0f) * Fy) * (1.0f / 40.0f);
+    h[7] = m[7] * 0.1f;
+    h[8] = (m[8] - sq * (m[8] + (2.0f / 3.0f) * rho * w)) * (1.0f / 40.0f);
+    const float pxx = rho * (2.0f * u * u - v * v - w * w);
+    const float pww = rho * (v * v - w * w);
+    h[9] = (m[9] - sv * (m[9] - pxx) + cv * (-2.0f * vF)) * (1.0f / 36.0f);
+    h[10] = (m[10] - sv * (m[10] + 0.5f * pxx) + cv * vF) * (1.0f / 72.0f);
+    h[11] = (m[11] - sv * (m[11] - pww) + cv * (2.0f * vF)) * (1.0f / 12.0f);
+    h[12] = (m[12] - sv * (m[12] + 0.5f * pww) + cv * (-vF)) * (1.0f / 24.0f);
+    h[13] = (m[13] - sv * (m[13] - rho * u * v) + cv * (u * Fy)) * 0.25f;
+    h[14] = (m[14] - sv * (m[14] - rho * v * w) + cv * (w * Fy)) * 0.25f;
+    h[15] = (m[15] - sv * (m[15] - rho * u * w)) * 0.25f;
+    h[16] = (1.0f - sq) * m[16] * 0.125f;
+    h[17] = (1.0f - sq) * m[17] * 0.125f;
+    h[18] = (1.0f - sq) * m[18] * 0.125f;
+
+    p[0] = h[0] - 30.0f * h[1] + 12.0f * h[2];
+    const float ba = h[0] - 11.0f * h[1] - 4.0f * h[2];
+    const float qx = h[3] - 4.0f * h[4], qy = h[5] - 4.0f * h[6], qz = h[7] - 4.0f * h[8];
+    const float pxa = 2.0f * h[9] - 4.0f * h[10];
+    const float pya = -h[9] + 2.0f * h[10];
+    const float pwa = h[11] - 2.0f * h[12];
+    p[1] = ba + qx + pxa;
+    p[2] = ba - qx + pxa;
+    p[3] = ba + qy + pya + pwa;
+    p[4] = ba - qy + pya + pwa;
+    p[5] = ba + qz + pya - pwa;
+    p[6] = ba - qz + pya - pwa;
+    const float bd = h[0] + 8.0f * h[1] + h[2];
+    const float X = h[3] + h[4], Y = h[5] + h[6], Z = h[7] + h[8];
+    const float pxy = bd + (h[9] + h[10]) + (h[11] + h[12]);
+    const float pxz = bd + (h[9] + h[10]) - (h[11] + h[12]);
+    const float pyz = bd - 2.0f * (h[9] + h[10]);
+    p[7] = pxy + X + Y + h[13] + (h[16] - h[17]);
+    p[8] = pxy - X + Y - h[13] - (h[16] + h[17]);
+    p[9] = pxy + X - Y - h[13] + (h[16] + h[17]);
+    p[10] = pxy - X - Y + h[13] - (h[16] - h[17]);
+    p[11] = pxz + X + Z + h[15] - (h[16] - h[18]);
+    p[12] = pxz - X + Z - h[15] + (h[16] + h[18]);
+    p[13] = pxz + X - Z - h[15] - (h[16] + h[18]);
+    p[14] = pxz - X - Z + h[15] + (h[16] - h[18]);
+    p[15] = pyz + Y + Z + h[14] + (h[17] - h[18]);
+    p[16] = pyz - Y + Z - h[14] - (h[17] + h[18]);
+    p[17] = pyz + Y - Z - h[14] + (h[17] + h[18]);
+    p[18] = pyz - Y - Z + h[14] - (h[17] - h[18]);
+}
+
+// g deviations k[7] -> k_post[7]; dT the temperature deviation used by collisionT (T - T0);
+// Tfull = T0 + dT in FP32; bp the M1G history.
+__device__ __forceinline__ void collide7_dev(const float *g, float *p, float dT, float Tfull, float u,
+                                             float v, float w, float *bp, int use_g, const LPParams &k)
+{
+    const float Bx = u * Tfull, By = v * Tfull, Bz = w * Tfull;
+    float dBx = 0.f, dBy = 0.f, dBz = 0.f;
+    if (use_g) {
+        dBx = Bx - bp[0]; dBy = By - bp[1]; dBz = Bz - bp[2];
+        bp[0] = Bx; bp[1] = By; bp[2] = Bz;
+    }
+    const float n0 = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
+    const float n1 = g[1] - g[2], n2 = g[3] - g[4], n3 = g[5] - g[6];
+    const float s16 = (g[1] + g[2]) + (g[3] + g[4]) + (g[5] + g[6]);
+    const float n4 = -6.0f * g[0] + s16;
+    const float n5 = 2.0f * (g[1] + g[2]) - ((g[3] + g[4]) + (g[5] + g[6]));
+    const float n6 = (g[3] + g[4]) - (g[5] + g[6]);
+    const float p1 = n1 - k.qk * (n1 - Bx) + k.sg * dBx;
+    const float p2 = n2 - k.qk * (n2 - By) + k.sg * dBy;
+    const float p3 = n3 - k.qk * (n3 - Bz) + k.sg * dBz;
+    const float p4 = n4 - k.qnu * (n4 + 0.75f * dT);
+    const float p5 = (1.0f - k.qnu) * n5;
+    const float p6 = (1.0f - k.qnu) * n6;
+    const float c0 = n0 * (1.0f / 7.0f);
+    const float c4 = p4 * (1.0f / 42.0f);
+    p[0] = c0 - p4 * (1.0f / 7.0f);
+    const float ax = c0 + c4 + p5 * (1.0f / 6.0f);
+    const float ay = c0 + c4 - p5 * (1.0f / 12.0f);
+    p[1] = ax + 0.5f * p1;
+    p[2] = ax - 0.5f * p1;
+    p[3] = ay + 0.5f * p2 + 0.25f * p6;
+    p[4] = ay - 0.5f * p2 + 0.25f * p6;
+    p[5] = ay + 0.5f * p3 - 0.25f * p6;
+    p[6] = ay - 0.5f * p3 - 0.25f * p6;
+}
+
+__device__ __forceinline__ bool on_border_lp(int i, int j, int k, const KParams &kp)
+{
+    return i == 0 || j == 0 || k == 0 || i == kp.nx - 1 || j == kp.ny - 1 || k == kp.nz - 1;
+}
+__device__ __forceinline__ bool node_of_lp(const KParams &kp, int &i, int &j, int &k, uint64_t &c)
+{
+    const unsigned lb = xcd_remap(blockIdx.x, kp.nblocks);
+    const unsigned xb = lb % (unsigned)kp.nxb;
+    const unsigned row = lb / (unsigned)kp.nxb;
+    j = (int)(row % (unsigned)kp.ny);
+    k = (int)(row / (unsigned)kp.ny);
+    i = (int)(xb * blockDim.x + threadIdx.x);
+    c = (uint64_t)i + (uint64_t)kp.nx * ((uint64_t)j + (uint64_t)kp.ny * (uint64_t)k);
+    return i < kp.nx;
+}
+
+__device__ __forceinline__ void moments19(const float *f, float &drho, float &jx, float &jy, float &jz)
+{
+    drho = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) +
+           ((f[8] + f[9]) + (f[10] + f[11])) + ((f[12] + f[13]) + (f[14] + f[15])) +
+           ((f[16] + f[17]) + f[18]);
+    jx = ((f[1] - f[2]) + (f[7] - f[8])) + ((f[9] - f[10]) + (f[11] - f[12])) + (f[13] - f[14]);
+    jy = ((f[3] - f[4]) + (f[7] + f[8])) - ((f[9] + f[10]) - (f[15] - f[16])) + (f[17] - f[18]);
+    jz = ((f[5] - f[6]) + (f[11] + f[12])) - ((f[13] + f[14]) - (f[15] + f[16])) - (f[17] + f[18]);
+}
+
+// -------------------------------------------------------------------------------------------
+// kernels
+// -------------------------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(256) lp3_prologue(KParams kp, LPParams lp, const S *__restrict__ Fin,
+                                                    S *__restrict__ Fout, const double *__restrict__ mac)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of_lp(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    float f[19];
+    load_all<D3Q19, S>(Fin, N, c, f, SEQ(D3Q19));
+    const float drho = (float)(mac[c] - 1.0);
+    const float dT = (float)(mac[4 * N + c] - lp.tref);
+    collide19_dev(f, f, drho, (float)mac[N + c], (float)mac[2 * N + c], (float)mac[3 * N + c], dT, lp);
+    store_all<D3Q19, S>(Fout, N, c, f, SEQ(D3Q19));
+}
+
+// Pass A of step n (see lbm_kernels.hip).  aux holds dT^n as float.
+template <typename S, bool FIRST>
+__global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, const S *__restrict__ Fcur,
+                                                   const S *__restrict__ Gin, S *__restrict__ Gout,
+                                                   float *__restrict__ Bp, float *__restrict__ aux,
+                                                   const double *__restrict__ mac)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of_lp(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border_lp(i, j, k, kp);
+    float g[7], dT;
+    if (FIRST) {
+        load_all<D3Q7, S>(Gin, N, c, g, SEQ(D3Q7));
+        dT = (float)(mac[4 * N + c] - lp.tref);
+    } else {
+        pull_heat_lp<D3Q7, S>(Gin, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+        dT = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
+    }
+    float f[19];
+    pull_flow_lp<D3Q19, S>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    float drho, jx, jy, jz;
+    moments19(f, drho, jx, jy, jz);
+    const float rho = 1.0f + drho;
+    const float ir = 1.0f / rho;
+    const float Fy = rho * lp.gbeta * dT;
+    const float u = jx * ir, v = (jy + 0.5f * Fy) * ir, w = jz * ir;
+    float bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
+    float p[7];
+    collide7_dev(g, p, dT, (float)lp.tref + dT, u, v, w, bp, kp.use_g, lp);
+    store_all<D3Q7, S>(Gout, N, c, p, SEQ(D3Q7));
+    if (kp.use_g) {
+        Bp[c] = bp[0];
+        Bp[N + c] = bp[1];
+        Bp[2 * N + c] = bp[2];
+    }
+    aux[c] = dT;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S *__restrict__ Fcur,
+                                                S *__restrict__ Fnext, const S *__restrict__ Gpost,
+                                                const float *__restrict__ aux)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of_lp(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border_lp(i, j, k, kp);
+    float g[7];
+    pull_heat_lp<D3Q7, S>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+    const float dT1 = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
+    float f[19];
+    pull_flow_lp<D3Q19, S>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    float drho, jx, jy, jz;
+    moments19(f, drho, jx, jy, jz);
+    const float rho = 1.0f + drho;
+    const float ir = 1.0f / rho;
+    const float Fy = rho * lp.gbeta * aux[c];
+    const float u = jx * ir, v = (jy + 0.5f * Fy) * ir, w = jz * ir;
+    collide19_dev(f, f, drho, u, v, w, dT1, lp);
+    store_all<D3Q19, S>(Fnext, N, c, f, SEQ(D3Q19));
+}
+
+// Reference state (absolute FP64) from the pipelined buffers; see k3_state.
+template <typename S>
+__global__ void __launch_bounds__(256) lp3_state(KParams kp, LPParams lp, const S *__restrict__ Fprev,
+                                                 const S *__restrict__ Gpost, const float *__restrict__ aux,
+                                                 double *__restrict__ mac, double *__restrict__ fo,
+                                                 double *__restrict__ go, int k0, int nk)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of_lp(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    const bool border = on_border_lp(i, j, k, kp);
+    float g[7], f[19];
+    pull_heat_lp<D3Q7, S>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+    pull_flow_lp<D3Q19, S>(Fprev, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    double drho = 0.0, jx = 0.0, jy = 0.0, jz = 0.0, dT = 0.0;
+#pragma unroll
+    for (int a = 0; a < 19; ++a) {
+        drho += (double)f[a];
+        jx += (double)D3Q19::ex[a] * f[a];
+        jy += (double)D3Q19::ey[a] * f[a];
+        jz += (double)D3Q19::ez[a] * f[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 7; ++a) dT += (double)g[a];
+    const double rho = 1.0 + drho;
+    if (mac) {
+        const double Fy = rho * (double)lp.gbeta * (double)aux[c];
+        mac[c] = rho;
+        mac[N + c] = jx / rho;
+        mac[2 * N + c] = (jy + 0.5 * Fy) / rho;
+        mac[3 * N + c] = jz / rho;
+        mac[4 * N + c] = lp.tref + dT;
+    }
+    if (fo && k >= k0 && k < k0 + nk) {
+        const uint64_t r = c - (uint64_t)k0 * kp.nx * kp.ny;
+#pragma unroll
+        for (int a = 0; a < 19; ++a) fo[r * 19 + a] = w19(a) + (double)f[a];
+#pragma unroll
+        for (int a = 0; a < 7; ++a) go[r * 7 + a] = w7(a) * lp.tref + (double)g[a];
+    }
+}
+
+// Plain initial state: deviations of f = w (zero) and of g = wT T.
+template <typename S>
+__global__ void __launch_bounds__(256) lp_init(KParams kp, LPParams lp, S *__restrict__ F, S *__restrict__ G,
+                                               float *__restrict__ Bp, const double *__restrict__ mac)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of_lp(kp, i, j, k, c)) return;
+    const uint64_t N = kp.N;
+    float f[19] = {}, g[7];
+    const double T = mac[4 * N + c];
+#pragma unroll
+    for (int a = 0; a < 7; ++a) g[a] = (float)(w7(a) * T - w7(a) * lp.tref);
+    store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
+    store_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
+    Bp[c] = (float)(mac[N + c] * T);
+    Bp[N + c] = (float)(mac[2 * N + c] * T);
+    Bp[2 * N + c] = (float)(mac[3 * N + c] * T);
+}
+
+// Plain state from absolute FP64 SoA planes (restart): deviations rounded once.
+template <typename S>
+__global__ void lp_from_abs(KParams kp, LPParams lp, const double *__restrict__ Fd, const double *__restrict__ Gd,
+                            S *__restrict__ F, S *__restrict__ G, uint64_t c0, uint64_t n)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t c = c0 + r, N = kp.N;
+    float f[19], g[7];
+#pragma unroll
+    for (int a = 0; a < 19; ++a) f[a] = (float)(Fd[r * 19 + a] - w19(a));
+#pragma unroll
+    for (int a = 0; a < 7; ++a) g[a] = (float)(Gd[r * 7 + a] - w7(a) * lp.tref);
+    store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
+    store_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
+}
+
+// Plain state back to absolute FP64 AoS (gather before the first step).
+template <typename S>
+__global__ void lp_to_abs(KParams kp, LPParams lp, const S *__restrict__ F, const S *__restrict__ G,
+                          double *__restrict__ Fd, double *__restrict__ Gd, uint64_t c0, uint64_t n)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t c = c0 + r, N = kp.N;
+    float f[19], g[7];
+    load_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
+    load_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
+#pragma unroll
+    for (int a = 0; a < 19; ++a) Fd[r * 19 + a] = w19(a) + (double)f[a];
+#pragma unroll
+    for (int a = 0; a < 7; ++a) Gd[r * 7 + a] = w7(a) * lp.tref + (double)g[a];
+}
+
+// reconstruct_macro_from_fg (2DRB/3DRB.F90:1392-1463) on an absolute FP64 AoS chunk.
+__global__ void lp_recon_aos(KParams kp, const double *__restrict__ fo, const double *__restrict__ go,
+                             double *__restrict__ mac, uint64_t c0, uint64_t n, int *bad)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t c = c0 + r, N = kp.N;
+    const double *F = fo + r * 19, *G = go + r * 7;
+    const double T = G[0] + G[1] + G[2] + G[3] + G[4] + G[5] + G[6];
+    const double rho = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8] + F[9] + F[10] +
+                       F[11] + F[12] + F[13] + F[14] + F[15] + F[16] + F[17] + F[18];
+    const double mx = F[1] - F[2] + F[7] - F[8] + F[9] - F[10] + F[11] - F[12] + F[13] - F[14];
+    const double my = F[3] - F[4] + F[7] + F[8] - F[9] - F[10] + F[15] - F[16] + F[17] - F[18];
+    const double mz = F[5] - F[6] + F[11] + F[12] - F[13] - F[14] + F[15] + F[16] - F[17] - F[18];
+    double u = 0.0, v = 0.0, w = 0.0;
+    if (rho > 0.0) {
+        const double Fy = rho * kp.gbeta * (T - kp.tref);
+        u = mx / rho;
+        v = (my + 0.5 * Fy) / rho;
+        w = mz / rho;
+    } else {
+        atomicOr(bad, 1);
+    }
+    mac[c] = rho;
+    mac[N + c] = u;
+    mac[2 * N + c] = v;
+    mac[3 * N + c] = w;
+    mac[4 * N + c] = T;
+}
+
+__global__ void lp_bp_from_mac(KParams kp, const double *__restrict__ mac, float *__restrict__ Bp)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t N = kp.N;
+    if (c >= N) return;
+    const double T = mac[4 * N + c];
+    Bp[c] = kp.use_g ? (float)(mac[N + c] * T) : 0.0f;
+    Bp[N + c] = kp.use_g ? (float)(mac[2 * N + c] * T) : 0.0f;
+    Bp[2 * N + c] = kp.use_g ? (float)(mac[3 * N + c] * T) : 0.0f;
+}
+
+// B_prev as FP64 planes (gather).
+__global__ void lp_bprev_to_f64(const float *__restrict__ Bp, double *__restrict__ out, uint64_t n)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = (double)Bp[r];
+}
+
+// -------------------------------------------------------------------------------------------
+// launchers
+// -------------------------------------------------------------------------------------------
+static LPParams lp_params(const KParams &kp)
+{
+    LPParams lp;
+    lp.gbeta = (float)kp.gbeta;
+    lp.sv = (float)kp.sv;
+    lp.sq = (float)kp.sq;
+    lp.qk = (float)kp.qk;
+    lp.qnu = (float)kp.qnu;
+    lp.sg = (float)kp.sg;
+    lp.tref = kp.tref;
+    lp.dthot = (float)(kp.thot - kp.tref);
+    lp.dtcold = (float)(kp.tcold - kp.tref);
+    return lp;
+}
+
+template <typename S>
+static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a)
+{
+    const LPParams lp = lp_params(kp);
+    const dim3 grid(kp.nblocks);
+    const S *Fc = (const S *)a.Fc;
+    switch (op) {
+    case LP_PROLOGUE:
+        lp3_prologue<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, a.mac);
+        break;
+    case LP_STEP:
+        if (a.first)
+            lp3_thermal<S, true><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
+        else
+            lp3_thermal<S, false><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
+        lp3_flow<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, (const S *)a.Go, a.aux);
+        break;
+    case LP_STATE:
+        lp3_state<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux, a.mac, a.fo, a.go, a.k0, a.nk);
+        break;
+    case LP_INIT:
+        lp_init<S><<<grid, bx, 0, s>>>(kp, lp, (S *)a.Fn, (S *)a.Go, a.Bp, a.mac);
+        break;
+    case LP_FROM_ABS: {
+        const unsigned b = (unsigned)((a.n + 255) / 256);
+        if (b) lp_from_abs<S><<<b, 256, 0, s>>>(kp, lp, a.fo, a.go, (S *)a.Fn, (S *)a.Go, a.c0, a.n);
+        break;
+    }
+    case LP_TO_ABS: {
+        const unsigned b = (unsigned)((a.n + 255) / 256);
+        if (b) lp_to_abs<S><<<b, 256, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.fo, a.go, a.c0, a.n);
+        break;
+    }
+    case LP_RECON_AOS: {
+        const unsigned b = (unsigned)((a.n + 255) / 256);
+        if (b) lp_recon_aos<<<b, 256, 0, s>>>(kp, a.fo, a.go, a.mac, a.c0, a.n, a.flag);
+        break;
+    }
+    case LP_BP_FROM_MAC: {
+        const unsigned b = (unsigned)((kp.N + 255) / 256);
+        lp_bp_from_mac<<<b, 256, 0, s>>>(kp, a.mac, a.Bp);
+        break;
+    }
+    case LP_BPREV: {
+        const unsigned b = (unsigned)((a.n + 255) / 256);
+        if (b) lp_bprev_to_f64<<<b, 256, 0, s>>>(a.Bp, a.fo, a.n);
+        break;
+    }
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a)
+{
+    if (precision == 1) return lp_dispatch<float>(op, kp, bx, s, a);
+    if (precision == 2) return lp_dispatch<_Float16>(op, kp, bx, s, a);
+    return hipErrorInvalidValue;
+}
+
+} // namespace rblbm
