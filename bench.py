@@ -37,11 +37,28 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--rayleigh", type=float, default=1e8)
-    ap.add_argument("--precision", choices=["fp64"], default="fp64")
+    ap.add_argument("--forms", default="fp64,fp32",
+                    help="storage forms to time, comma separated; the first one is `value`")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-nz", type=int, default=64)
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run a device-to-device copy of GIB GiB (counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(gib):
+    """A plain copy kernel of known size: reads and writes gib GiB once per launch."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                      "launches": 3}), flush=True)
 
 
 def cpu_baseline(size, nz, ra, steps):
@@ -70,8 +87,39 @@ def cpu_baseline(size, nz, ra, steps):
                       f"oracle, {steps} timed steps after 1 warm-up step ({dt:.1f} s)"}
 
 
+def measure(rblbm, torch, n, ra, prec, steps, warmup):
+    """Time `steps` steps of one storage form on the n^3 cavity (inputs resident in HBM)."""
+    cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=ra, prandtl=0.71, mach=0.1,
+                       precision=prec)
+    sol = rblbm.Solver(cfg)
+    sol.initial_seeded(SEED, 1e-3)
+    mass0 = sol.diagnostics()["mass"]
+    sol.step(warmup)
+    sol.sync()
+    if torch is not None:
+        torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    ms = sol.step_timed(steps)            # HIP events on the solver's own stream
+    if torch is not None:
+        torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    nodes = n ** 3
+    ms_step = ms / steps
+    bpu = sol.bytes_per_update()
+    diag = sol.diagnostics()
+    sol.close()
+    return {"glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms_step,
+            "bytes_per_update": bpu, "achieved_gbs": bpu * nodes / (ms_step / 1e3) / 1e9,
+            "roofline_frac": bpu * nodes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "wall_seconds": t_wall, "nu_vol": diag["nu_vol"], "diag": diag, "mass0": mass0,
+            "mass_rel_change": (diag["mass"] - mass0) / mass0}
+
+
 def main():
     a = parse()
+    if a.copy_probe > 0:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); the solver "
               "runs a single domain per device and has no multi-GPU path.", file=sys.stderr)
@@ -86,26 +134,24 @@ def main():
         torch, have_torch = None, False
 
     n = a.size
-    cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=a.rayleigh, prandtl=0.71, mach=0.1,
-                       precision=rblbm.FP64)
-    sol = rblbm.Solver(cfg)
-    sol.initial_seeded(SEED, 1e-3)
-    mass0 = sol.diagnostics()["mass"]
-    sol.step(a.warmup)
-    sol.sync()
-    if have_torch:
-        torch.cuda.synchronize()
-    t_wall = time.perf_counter()
-    ms = sol.step_timed(a.steps)          # HIP events on the solver's own stream
-    if have_torch:
-        torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
     nodes = n * n * n
-    glups = nodes * a.steps / (ms / 1e3) / 1e9
-    diag = sol.diagnostics()
-    bpu = sol.bytes_per_update()
-    ms_step = ms / a.steps
-    achieved = bpu * nodes / (ms_step / 1e3) / 1e9
+    forms = {}
+    for name in a.forms.split(","):
+        prec = {"fp64": rblbm.FP64, "fp32": rblbm.FP32_SHIFT, "fp16": rblbm.FP16_SHIFT}[name]
+        forms[name] = measure(rblbm, torch if have_torch else None, n, a.rayleigh, prec,
+                              a.steps, a.warmup)
+    main_form = forms[a.forms.split(",")[0]]
+    ms_step = main_form["ms_per_step"]
+    glups = main_form["glups"]
+    bpu = main_form["bytes_per_update"]
+    achieved = main_form["achieved_gbs"]
+    t_wall = main_form["wall_seconds"]
+    diag = main_form["diag"]
+    mass0 = main_form["mass0"]
+    if "fp64" in forms and "fp32" in forms:
+        forms["fp32"]["glups_vs_fp64"] = forms["fp32"]["glups"] / forms["fp64"]["glups"]
+    for f in forms.values():
+        f.pop("diag", None)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_fp64_512.json")
@@ -115,17 +161,18 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        sol.close()
         cpu = cpu_baseline(n, min(a.cpu_nz, n), a.rayleigh, a.cpu_steps)
 
     out = {
         "metric": "coupled D3Q19+D3Q7 GLUPS & %HBM roofline, FP32-shift vs FP64 at 512^3, Nu err",
         "value": glups, "unit": "GLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16/f32"}[a.forms.split(",")[0]],
+        "data": "synthetic",
         "config": {"workload": f"3D RB cavity {n}^3 Ra={a.rayleigh:g} Pr=0.71 Ma=0.1, FP64 "
                                "populations, seeded T noise 1e-3",
                    "nodes": nodes, "precision": "fp64", "wall_seconds": t_wall},
+        "forms": forms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_update": bpu, "kernel": "k3_thermal + k3_flow (one step)"},

@@ -1,0 +1,89 @@
+"""Reduced-precision storage forms (FP32 / FP16 deviations, FP32 arithmetic) on the GPU.
+
+Bars, stated per form: after the given steps from the same seeded state, the velocity error is
+measured relative to max|u| of the FP64 oracle and the temperature error relative to dT; the
+Nusselt / Reynolds numbers relative to their oracle values.
+    FP32-shift: u <= 2e-4, T <= 2e-6, Nu_vol, Re <= 1e-4
+    FP16-shift: u <= 5e-2, T <= 2e-3, Nu_vol <= 2e-2
+"""
+import numpy as np
+import pytest
+
+import orc
+import rblbm
+
+pytestmark = pytest.mark.gpu
+
+BARS = {rblbm.FP32_SHIFT: dict(u=2e-4, T=2e-6, nu=1e-4),
+        rblbm.FP16_SHIFT: dict(u=5e-2, T=2e-3, nu=2e-2)}
+
+
+def _pair(prec, nx=24, ny=20, nz=16, **kw):
+    oc = orc.Config(dims=3, nx=nx, ny=ny, nz=nz, **kw)
+    gc = rblbm.Config(dims=3, nx=nx, ny=ny, nz=nz, precision=prec, **kw)
+    return orc.Oracle(oc), rblbm.Solver(gc)
+
+
+def _errors(sol, ora):
+    s, m = ora.state(), sol.macros()
+    umax = max(np.abs(s[k]).max() for k in ("u", "v", "w"))
+    du = max(np.abs(m[k] - s[k]).max() for k in ("u", "v", "w")) / umax
+    dT = np.abs(m["T"] - s["T"]).max() / (sol.cfg.thot - sol.cfg.tcold)
+    return du, dT
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+@pytest.mark.parametrize("case", ["rb", "periodic_xz"])
+def test_lowprec_tracks_fp64_oracle(prec, case):
+    kw = dict(rayleigh=1e6, mach=0.1)
+    if case == "periodic_xz":
+        kw.update(vel_x=1, vel_z=1, temp_x=2, temp_z=2)
+    ora, sol = _pair(prec, **kw)
+    ora.init_seeded(5, 0.05)
+    sol.initial_seeded(5, 0.05)
+    for n in (1, 50, 300):
+        ora.step(n)
+        sol.step(n)
+        du, dT = _errors(sol, ora)
+        print(prec, case, n, f"u {du:.2e} T {dT:.2e}", flush=True)
+        assert du <= BARS[prec]["u"] and dT <= BARS[prec]["T"], (n, du, dT)
+    d, e = sol.diagnostics(), ora.diagnostics()
+    for k in ("nu_vol", "re_vol", "nu_global"):
+        assert abs(d[k] - e[k]) <= BARS[prec]["nu"] * abs(e[k]), (k, d[k], e[k])
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+def test_lowprec_scatter_gather_roundtrip(prec):
+    ora, sol = _pair(prec, rayleigh=1e5, mach=0.3)
+    ora.init_seeded(9, 0.1)
+    ora.step(20)
+    s = ora.state()
+    sol.scatter(s["f"], s["g"])
+    st = sol.gather()
+    w19 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    dev = np.abs(s["f"] - w19).max()
+    tol = 1e-6 if prec == rblbm.FP32_SHIFT else 2e-3
+    assert np.abs(st["f"] - s["f"]).max() <= tol * dev
+    m = sol.macros()
+    assert np.abs(m["T"] - s["T"]).max() <= (1e-6 if prec == rblbm.FP32_SHIFT else 1e-3)
+
+
+def test_fp32_conserves_mass_and_runs_rest_state():
+    ora, sol = _pair(rblbm.FP32_SHIFT, rayleigh=1e6)
+    sol.initial(np.zeros((16, 20, 24)))        # T = Tref: no buoyancy, rest is a fixed point
+    sol.step(100)
+    st = sol.gather()
+    w19 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    assert np.abs(st["f"] - w19).max() == 0.0
+    sol.initial_seeded(1, 0.05)
+    m0 = sol.diagnostics()["mass"]
+    sol.step(200)
+    assert sol.diagnostics()["mass"] == pytest.approx(m0, rel=1e-9)
+
+
+def test_lowprec_bytes_per_update():
+    for prec, b in ((rblbm.FP64, 464.0), (rblbm.FP32_SHIFT, 232.0), (rblbm.FP16_SHIFT, 128.0)):
+        sol = rblbm.Solver(rblbm.Config(dims=3, nx=8, ny=8, nz=8, precision=prec))
+        assert sol.bytes_per_update() == b
+    with pytest.raises(rblbm.LbmError):
+        rblbm.Solver(rblbm.Config(dims=2, nx=8, ny=8, nz=1, precision=rblbm.FP32_SHIFT))
