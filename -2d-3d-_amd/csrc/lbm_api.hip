@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <utility>
 #include <vector>
 
 using namespace rblbm;
@@ -26,7 +27,9 @@ struct lbm_handle {
     bool lp;                // reduced-precision deviation storage (3D only)
     // F, G, Bp, aux are double in the FP64 form; F, G are float / _Float16 and Bp, aux float
     // in the reduced-precision forms.
-    double *F[2], *G[2], *Bp, *aux[2], *mac, *prev;
+    // Bp is the current M1G history; Bp_alt receives the one-pass form's update (ring nodes of a
+    // tile read B_prev of nodes that a neighbouring tile updates in the same launch).
+    double *F[2], *G[2], *Bp, *Bp_alt, *aux[2], *mac, *prev;
     double *red;            // reduction partials (device)
     double *wall;           // per-x wall profiles (device, 2 * nx)
     int *flag;              // device error flag
@@ -34,6 +37,10 @@ struct lbm_handle {
     int fcur, gcur, fy;     // buffer indices, see k*_ kernels
     int phase;              // -1: not initialised, 0: plain state, 1: pipelined
     int first_thermal;      // next thermal pass reads the plain g / T
+    int aux_valid;          // aux holds T^n (last step ran the two-pass form)
+    int variant;            // LBM_KERNEL_* requested
+    bool fused_ok;          // the configuration admits the one-pass form
+    FusedGeom fg;
     int64_t itc;
     char err[256];
 };
@@ -95,7 +102,7 @@ extern "C" const char *lbm_last_error(lbm_handle *h) { return h ? h->err : "null
 
 static void release(lbm_handle *h)
 {
-    double **bufs[] = {&h->F[0], &h->F[1], &h->G[0], &h->G[1], &h->Bp, &h->aux[0],
+    double **bufs[] = {&h->F[0], &h->F[1], &h->G[0], &h->G[1], &h->Bp, &h->Bp_alt, &h->aux[0],
                        &h->aux[1], &h->mac, &h->prev, &h->red, &h->wall};
     for (auto b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
@@ -148,6 +155,14 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
     const uint64_t nb = (uint64_t)k.nxb * c.ny * c.nz;
     if (nb >= (1ULL << 31)) { free(h); return LBM_ERR_ARG; }
     k.nblocks = (unsigned)nb;
+    // one-pass form: 3D without periodic thermal walls (see lbm_fused.h)
+    h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC;
+    h->variant = LBM_KERNEL_AUTO;
+    h->fg.kchunk = c.nz >= 256 ? 64 : (c.nz >= 64 ? 32 : c.nz);
+    h->fg.ntx = (unsigned)((c.nx + FUSED_TX - 3) / (FUSED_TX - 2));
+    h->fg.nty = (unsigned)((c.ny + FUSED_TY - 3) / (FUSED_TY - 2));
+    h->fg.nzc = (unsigned)((c.nz + h->fg.kchunk - 1) / h->fg.kchunk);
+    h->fg.nblocks = h->fg.ntx * h->fg.nty * h->fg.nzc;
 
     int rc = LBM_OK;
     hipError_t e = hipSetDevice(c.device);
@@ -166,6 +181,7 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
         {&h->F[0], N * h->qf * h->es}, {&h->F[1], N * h->qf * h->es},
         {&h->G[0], N * h->qt * h->es}, {&h->G[1], N * h->qt * h->es},
         {&h->Bp, N * 3 * ea},          {&h->aux[0], N * ea},
+        {&h->Bp_alt, (h->fused_ok ? N * 3 : 1) * ea},
         {&h->aux[1], (c.dims == 2 ? N : 1) * ea},
         {&h->mac, N * 5 * sizeof(double)}, {&h->wall, (size_t)2 * c.nx * sizeof(double)},
     };
@@ -272,6 +288,8 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
         double *Fn = h->F[h->fcur ^ 1];
         const double *Gi = h->G[h->gcur];
         double *Go = h->G[h->gcur ^ 1];
+        // the first step after (re)initialisation reads the plain state: two-pass form
+        const bool fused = h->fused_ok && !h->first_thermal && h->variant != LBM_KERNEL_TWO_PASS;
         if (h->lp) {
             LPArgs a;
             a.Fc = Fc;
@@ -282,10 +300,19 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
             a.aux = (float *)h->aux[0];
             a.mac = h->mac;
             a.first = h->first_thermal;
-            HIPCHK(h, launch_lp(h->cfg.precision, LP_STEP, h->kp, h->bx, h->stream, a));
+            a.fg = &h->fg;
+            a.Bp_out = (float *)h->Bp_alt;
+            HIPCHK(h, launch_lp(h->cfg.precision, fused ? LP_FUSED : LP_STEP, h->kp, h->bx, h->stream, a));
+            h->aux_valid = !fused;
+            if (fused && h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
+        } else if (d3 && fused) {
+            HIPCHK(h, launch_fused_f64(h->kp, h->fg, h->stream, Fc, Fn, Gi, Go, h->Bp, h->Bp_alt));
+            h->aux_valid = 0;
+            if (h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
         } else if (d3) {
             HIPCHK(h, launch_step3(h->kp, h->bx, h->stream, h->first_thermal, Fc, Fn, Gi, Go, h->Bp,
                                    h->aux[0], h->mac));
+            h->aux_valid = 1;
         } else {
             HIPCHK(h, launch_step2(h->kp, h->bx, h->stream, h->first_thermal, Fc, Fn, Gi, Go, h->Bp,
                                    h->aux[h->fy], h->aux[h->fy ^ 1], h->mac));
@@ -357,6 +384,21 @@ extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
     return LBM_OK;
 }
 
+extern "C" int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk)
+{
+    if (!h || variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < 0)
+        return LBM_ERR_ARG;
+    if (variant == LBM_KERNEL_ONE_PASS && !h->fused_ok)
+        return fail(h, LBM_ERR_UNSUPPORTED, "one-pass form needs 3D and non-periodic thermal walls");
+    h->variant = variant;
+    if (kchunk > 0) {
+        h->fg.kchunk = kchunk > h->cfg.nz ? h->cfg.nz : kchunk;
+        h->fg.nzc = (unsigned)((h->cfg.nz + h->fg.kchunk - 1) / h->fg.kchunk);
+        h->fg.nblocks = h->fg.ntx * h->fg.nty * h->fg.nzc;
+    }
+    return LBM_OK;
+}
+
 // Bring the macro planes up to date with the current state (phase 1) or leave the plain ones.
 static int refresh_macros(lbm_handle *h)
 {
@@ -365,14 +407,15 @@ static int refresh_macros(lbm_handle *h)
         LPArgs a;
         a.Fc = h->F[h->fcur ^ 1];
         a.Gi = h->G[h->gcur];
-        a.aux = (float *)h->aux[0];
+        a.aux = h->aux_valid ? (float *)h->aux[0] : nullptr;
+        a.Gprev = h->G[h->gcur ^ 1];
         a.mac = h->mac;
         HIPCHK(h, launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a));
     } else if (h->phase == 1) {
         const bool d3 = h->cfg.dims == 3;
-        const double *aux = d3 ? h->aux[0] : h->aux[h->fy ^ 1];
+        const double *aux = d3 ? (h->aux_valid ? h->aux[0] : nullptr) : h->aux[h->fy ^ 1];
         HIPCHK(h, launch_state(h->kp, h->bx, h->stream, d3, h->F[h->fcur ^ 1], h->G[h->gcur], aux,
-                               h->mac, nullptr, nullptr, 0, 0));
+                               h->G[h->gcur ^ 1], h->mac, nullptr, nullptr, 0, 0));
     }
     return LBM_OK;
 }
@@ -611,7 +654,8 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
             } else {
                 a.Fc = h->F[h->fcur ^ 1];
                 a.Gi = h->G[h->gcur];
-                a.aux = (float *)h->aux[0];
+                a.aux = h->aux_valid ? (float *)h->aux[0] : nullptr;
+                a.Gprev = h->G[h->gcur ^ 1];
                 a.k0 = k0;
                 a.nk = n;
                 e = launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a);
@@ -620,9 +664,9 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
             e = launch_transpose(false, h->stream, h->F[0], fo, h->qf, h->N, c0, cnt);
             if (e == hipSuccess) e = launch_transpose(false, h->stream, h->G[0], go, h->qt, h->N, c0, cnt);
         } else {
-            const double *aux = d3 ? h->aux[0] : h->aux[h->fy ^ 1];
+            const double *aux = d3 ? (h->aux_valid ? h->aux[0] : nullptr) : h->aux[h->fy ^ 1];
             e = launch_state(h->kp, h->bx, h->stream, d3, h->F[h->fcur ^ 1], h->G[h->gcur], aux,
-                             nullptr, fo, go, k0, n);
+                             h->G[h->gcur ^ 1], nullptr, fo, go, k0, n);
         }
         if (e == hipSuccess && f)
             e = hipMemcpyAsync(f + c0 * h->qf, fo, cnt * h->qf * sizeof(double), hipMemcpyDeviceToHost, h->stream);

@@ -1,0 +1,134 @@
+// lbm_fused.h -- one-pass 3D step: both lattices advance one time step in a single launch.
+//
+// One reference step n needs two neighbour hops: u^{n+1} at a node needs f_post^n of its
+// neighbours, and T^{n+1} at a node needs the thermal collision (which needs u^{n+1}) of its six
+// face neighbours.  The two-pass form splits the hops over two launches and reads f twice.
+// This kernel instead gives each block a 2D tile of nodes plus a one-node ring around it, and
+// marches the tile through a chunk of z-planes:
+//
+//   iteration kk:  thermal part at plane kk for tile + ring   (pull g^n, pull f^{n+1} -> u^{n+1},
+//                  thermal collision -> g_post^n; tile nodes store g_post^n and B_prev)
+//                  exchange the four in-plane components of g_post^n through LDS
+//                  flow part at plane p = kk - 1 for the tile  (T^{n+1} = pull-sum of g_post^n:
+//                  in-plane neighbours from LDS, z neighbours from the thread's own registers;
+//                  flow collision of step n+1 with f^{n+1}(p) kept from the previous iteration)
+//
+// B_prev is read from one buffer and written to another: a ring node reads the history of a
+// node that the neighbouring tile updates during the same launch.
+// Ring nodes repeat the thermal part of their owner tile (their loads mostly hit L2); every
+// population and history value of a tile node crosses HBM once per step: 29 loads and 29
+// stores per node, the algorithmic minimum.  The planes k0 - 1 and k1 of a z-chunk are
+// likewise recomputed by both neighbouring chunks.
+//
+// A periodic thermal wall would need g_post^n of the opposite side of the domain, which no
+// tile holds; configurations with periodic thermal walls use the two-pass form.
+//
+// The including translation unit provides `Form` with the storage type S, arithmetic type R,
+// history type B and the per-node operations (pulls, macro, collisions) of its precision form.
+#pragma once
+
+#include "lbm_lattice.h"
+
+namespace rblbm {
+
+template <class Form, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY)
+k3_fused(KParams kp, typename Form::P fp, FusedGeom fg, const typename Form::S *__restrict__ Fcur,
+         typename Form::S *__restrict__ Fnext, const typename Form::S *__restrict__ Gin,
+         typename Form::S *__restrict__ Gout, const typename Form::B *__restrict__ Bp,
+         typename Form::B *__restrict__ Bpout)
+{
+    using R = typename Form::R;
+    constexpr int NT = TX * TY;
+    __shared__ R lds[3][4][NT];
+
+    const unsigned lb = xcd_remap(blockIdx.x, fg.nblocks);
+    const unsigned tile_x = lb % fg.ntx;
+    const unsigned rest = lb / fg.ntx;
+    const unsigned tile_y = rest % fg.nty;
+    const unsigned chunk = rest / fg.nty;
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = tid / TX;
+    const int i = (int)tile_x * (TX - 2) + lx - 1;
+    const int j = (int)tile_y * (TY - 2) + ly - 1;
+    const int nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const bool in_dom = (unsigned)i < (unsigned)nx && (unsigned)j < (unsigned)ny;
+    const bool xring = lx == 0 || lx == TX - 1, yring = ly == 0 || ly == TY - 1;
+    const bool active = in_dom && !(xring && yring);
+    const bool tile = in_dom && !xring && !yring;
+    const int k0 = (int)chunk * fg.kchunk;
+    const int k1 = min(nz, k0 + fg.kchunk);
+    const uint64_t N = kp.N;
+    const uint64_t col = (uint64_t)i + (uint64_t)nx * (uint64_t)j;   // valid when in_dom
+    const uint64_t plane = (uint64_t)nx * (uint64_t)ny;
+
+    R fprev[19];
+    R rho_p = 0, u_p = 0, v_p = 0, w_p = 0;
+    R gprev[7] = {};
+    R g5pp = 0;
+    bool have_prev = false, have_pp = false;
+
+    for (int kk = k0 - 1; kk <= k1; ++kk) {
+        R gnew[7] = {};
+        R fnew[19];
+        R rho_n = 0, u_n = 0, v_n = 0, w_n = 0;
+        const bool have = active && kk >= 0 && kk < nz;
+        if (have) {
+            const uint64_t c = col + plane * (uint64_t)kk;
+            const bool border = i == 0 || j == 0 || kk == 0 || i == nx - 1 || j == ny - 1 || kk == nz - 1;
+            R g[7];
+            Form::pull_g(Gin, N, i, j, kk, c, border, kp, fp, g);
+            const R Tn = Form::tsum(g);
+            Form::pull_f(Fcur, N, i, j, kk, c, border, kp, fp, fnew);
+            Form::macro(fnew, Tn, kp, fp, rho_n, u_n, v_n, w_n);
+            typename Form::B bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
+            Form::thermal(g, gnew, Tn, u_n, v_n, w_n, bp, kp, fp);
+            if (tile && kk >= k0 && kk < k1) {
+                Form::store_g(Gout, N, c, gnew);
+                if (kp.use_g) {
+                    Bpout[c] = bp[0];
+                    Bpout[N + c] = bp[1];
+                    Bpout[2 * N + c] = bp[2];
+                }
+            }
+        }
+        const int wb = (kk - k0 + 1) % 3;
+        lds[wb][0][tid] = gnew[1];
+        lds[wb][1][tid] = gnew[2];
+        lds[wb][2][tid] = gnew[3];
+        lds[wb][3][tid] = gnew[4];
+        __syncthreads();
+
+        const int p = kk - 1;
+        if (tile && p >= k0 && p < k1 && have_prev) {
+            const int rb = (p - k0 + 1) % 3;
+            // T^{n+1}(i, j, p): sum of the pulled post-wall g_post^n in direction order 0..6
+            R gin[7];
+            gin[0] = gprev[0];
+            gin[1] = i > 0 ? lds[rb][0][tid - 1] : Form::wall_g(1, gprev, kp.temp_x, fp);
+            gin[2] = i < nx - 1 ? lds[rb][1][tid + 1] : Form::wall_g(2, gprev, kp.temp_x, fp);
+            gin[3] = j > 0 ? lds[rb][2][tid - TX] : Form::wall_g(3, gprev, kp.temp_y, fp);
+            gin[4] = j < ny - 1 ? lds[rb][3][tid + TX] : Form::wall_g(4, gprev, kp.temp_y, fp);
+            gin[5] = (p > 0 && have_pp) ? g5pp : Form::wall_g(5, gprev, kp.temp_z, fp);
+            gin[6] = p < nz - 1 ? gnew[6] : Form::wall_g(6, gprev, kp.temp_z, fp);
+            const R T1 = Form::tsum(gin);
+            R out[19];
+            Form::flow(fprev, out, rho_p, u_p, v_p, w_p, T1, kp, fp);
+            Form::store_f(Fnext, N, col + plane * (uint64_t)p, out);
+        }
+        // shift the plane pipeline
+        g5pp = gprev[5];
+        have_pp = have_prev;
+#pragma unroll
+        for (int a = 0; a < 7; ++a) gprev[a] = gnew[a];
+#pragma unroll
+        for (int a = 0; a < 19; ++a) fprev[a] = fnew[a];
+        rho_p = rho_n;
+        u_p = u_n;
+        v_p = v_n;
+        w_p = w_n;
+        have_prev = have;
+    }
+}
+
+} // namespace rblbm

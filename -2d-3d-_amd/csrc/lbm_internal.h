@@ -25,8 +25,14 @@ hipError_t launch_step2(const KParams &kp, int bx, hipStream_t s, bool first, co
                         double *Fnext, const double *Gin, double *Gout, double *Bp,
                         const double *fy_cur, double *fy_next, const double *mac);
 hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, const double *Fprev,
-                        const double *Gpost, const double *aux_or_fy, double *mac, double *fo,
-                        double *go, int k0, int nk);
+                        const double *Gpost, const double *aux_or_fy, const double *Gprev,
+                        double *mac, double *fo, double *go, int k0, int nk);
+
+// One-pass form: tile of (FUSED_TX - 2) x (FUSED_TY - 2) nodes plus a one-node ring per block.
+constexpr int FUSED_TX = 32, FUSED_TY = 16;
+hipError_t launch_fused_f64(const KParams &kp, const FusedGeom &fg, hipStream_t s, const double *Fcur,
+                            double *Fnext, const double *Gin, double *Gout, const double *Bp,
+                            double *Bp_out);
 hipError_t launch_init(const KParams &kp, int bx, hipStream_t s, const InitParams &ip, double *F,
                        double *G, double *Bp, double *mac, int qf, int qt);
 hipError_t launch_transpose(bool to_soa, hipStream_t s, const double *in, double *out, int q,
@@ -52,6 +58,7 @@ enum {
     LP_BPREV,         // Bp (3 float planes) -> fo (FP64), n values
     LP_RECON_AOS,     // reconstruct_macro_from_fg on the AoS chunk fo/go -> mac, *flag on rho <= 0
     LP_BP_FROM_MAC,   // Bp = u T from mac (zero without use_g)
+    LP_FUSED,         // one-pass step: Fc, Gi -> Fn, Go; Bp updated (aux not written)
 };
 struct LPArgs {
     const void *Fc = nullptr;
@@ -63,6 +70,9 @@ struct LPArgs {
     int first = 0, k0 = 0, nk = 0;
     uint64_t c0 = 0, n = 0;
     int *flag = nullptr;
+    const void *Gprev = nullptr;   // LP_STATE: g_post^{n-1} when aux is not valid
+    const FusedGeom *fg = nullptr; // LP_FUSED
+    float *Bp_out = nullptr;       // LP_FUSED: updated history
 };
 hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a);
 

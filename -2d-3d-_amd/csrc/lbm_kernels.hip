@@ -268,6 +268,7 @@ __global__ void __launch_bounds__(256) k3_flow(KParams kp, const double *__restr
 __global__ void __launch_bounds__(256) k3_state(KParams kp, const double *__restrict__ Fprev,
                                                 const double *__restrict__ Gpost,
                                                 const double *__restrict__ aux,
+                                                const double *__restrict__ Gprev,
                                                 double *__restrict__ mac, double *__restrict__ fo,
                                                 double *__restrict__ go, int k0, int nk)
 {
@@ -280,8 +281,17 @@ __global__ void __launch_bounds__(256) k3_state(KParams kp, const double *__rest
     pull_heat<D3Q7>(Gpost, N, i, j, k, c, border, kp, g);
     pull_flow<D3Q19>(Fprev, N, i, j, k, c, border, kp, f);
     const double T = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
+    // T^n of the half force: stored by the two-pass form, else the pull-sum of g_post^{n-1}
+    double Tn;
+    if (aux) {
+        Tn = aux[c];
+    } else {
+        double gq[7];
+        pull_heat<D3Q7>(Gprev, N, i, j, k, c, border, kp, gq);
+        Tn = gq[0] + gq[1] + gq[2] + gq[3] + gq[4] + gq[5] + gq[6];
+    }
     const double rho = sum19(f);
-    const double Fy = rho * kp.gbeta * (aux[c] - kp.tref);
+    const double Fy = rho * kp.gbeta * (Tn - kp.tref);
     if (mac) {
         mac[c] = rho;
         mac[N + c] = jx19(f) / rho;
@@ -689,6 +699,90 @@ __global__ void __launch_bounds__(256) k_reduce_check(KParams kp, const double *
 }
 
 // ---------------------------------------------------------------------------------------
+// One-pass form (lbm_fused.h), FP64 exact arithmetic.
+// ---------------------------------------------------------------------------------------
+struct FormF64 {
+    using R = double;
+    using S = double;
+    using B = double;
+    struct P {
+        double thot, tcold;
+    };
+    static __device__ __forceinline__ void pull_g(const double *G, uint64_t N, int i, int j, int k,
+                                                  uint64_t c, bool border, const KParams &kp,
+                                                  const P &, double *g)
+    {
+        pull_heat<D3Q7>(G, N, i, j, k, c, border, kp, g);
+    }
+    static __device__ __forceinline__ void pull_f(const double *F, uint64_t N, int i, int j, int k,
+                                                  uint64_t c, bool border, const KParams &kp,
+                                                  const P &, double *f)
+    {
+        pull_flow<D3Q19>(F, N, i, j, k, c, border, kp, f);
+    }
+    static __device__ __forceinline__ double tsum(const double *g)
+    {
+        return g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
+    }
+    static __device__ __forceinline__ void macro(const double *f, double Tn, const KParams &kp,
+                                                 const P &, double &rho, double &u, double &v,
+                                                 double &w)
+    {
+        rho = sum19(f);
+        const double Fy = rho * kp.gbeta * (Tn - kp.tref);
+        u = jx19(f) / rho;
+        v = (jy19(f) + 0.5 * Fy) / rho;
+        w = jz19(f) / rho;
+    }
+    static __device__ __forceinline__ void thermal(const double *g, double *p, double T, double u,
+                                                   double v, double w, double *bp,
+                                                   const KParams &kp, const P &)
+    {
+        ref_collide_d3q7(g, p, T, u, v, w, bp, kp);
+    }
+    static __device__ __forceinline__ void flow(const double *f, double *out, double rho, double u,
+                                                double v, double w, double T1, const KParams &kp,
+                                                const P &)
+    {
+        ref_collide_d3q19(f, out, rho, u, v, w, T1, kp);
+    }
+    static __device__ __forceinline__ void store_g(double *G, uint64_t N, uint64_t c, const double *g)
+    {
+#pragma unroll
+        for (int a = 0; a < 7; ++a) G[a * N + c] = g[a];
+    }
+    static __device__ __forceinline__ void store_f(double *F, uint64_t N, uint64_t c, const double *f)
+    {
+#pragma unroll
+        for (int a = 0; a < 19; ++a) F[a * N + c] = f[a];
+    }
+    // Wall value of thermal direction a at the node itself (adiabatic or constant T).
+    static __device__ __forceinline__ double wall_g(int a, const double *gp, int kind, const P &p)
+    {
+        const double back = gp[D3Q7::opp[a]];
+        if (kind == T_ADIABATIC) return back;
+        const double tw = (a & 1) ? p.thot : p.tcold;
+        return -back + 2.0 * w7(a) * tw;
+    }
+};
+
+} // namespace rblbm
+
+#include "lbm_fused.h"
+
+namespace rblbm {
+
+hipError_t launch_fused_f64(const KParams &kp, const FusedGeom &fg, hipStream_t s, const double *Fcur,
+                            double *Fnext, const double *Gin, double *Gout, const double *Bp,
+                            double *Bp_out)
+{
+    FormF64::P p{kp.thot, kp.tcold};
+    k3_fused<FormF64, FUSED_TX, FUSED_TY><<<fg.nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
+        kp, p, fg, Fcur, Fnext, Gin, Gout, Bp, Bp_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------------------
 static dim3 step_grid(const KParams &kp) { return dim3(kp.nblocks); }
@@ -722,10 +816,10 @@ hipError_t launch_step2(const KParams &kp, int bx, hipStream_t s, bool first, co
 }
 
 hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, const double *Fprev,
-                        const double *Gpost, const double *aux_or_fy, double *mac, double *fo,
-                        double *go, int k0, int nk)
+                        const double *Gpost, const double *aux_or_fy, const double *Gprev,
+                        double *mac, double *fo, double *go, int k0, int nk)
 {
-    if (three_d) k3_state<<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, mac, fo, go, k0, nk);
+    if (three_d) k3_state<<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, Gprev, mac, fo, go, k0, nk);
     else k2_state<<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, mac, fo, go, k0, nk);
     return hipGetLastError();
 }

@@ -381,6 +381,7 @@ __global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S
 template <typename S>
 __global__ void __launch_bounds__(256) lp3_state(KParams kp, LPParams lp, const S *__restrict__ Fprev,
                                                  const S *__restrict__ Gpost, const float *__restrict__ aux,
+                                                 const S *__restrict__ Gprev,
                                                  double *__restrict__ mac, double *__restrict__ fo,
                                                  double *__restrict__ go, int k0, int nk)
 {
@@ -404,7 +405,16 @@ __global__ void __launch_bounds__(256) lp3_state(KParams kp, LPParams lp, const 
     for (int a = 0; a < 7; ++a) dT += (double)g[a];
     const double rho = 1.0 + drho;
     if (mac) {
-        const double Fy = rho * (double)lp.gbeta * (double)aux[c];
+        // dT^n of the half force: stored by the two-pass form, else pull-sum of g_post^{n-1}
+        float dTn;
+        if (aux) {
+            dTn = aux[c];
+        } else {
+            float gq[7];
+            pull_heat_lp<D3Q7, S>(Gprev, N, i, j, k, c, border, kp, lp, gq, SEQ(D3Q7));
+            dTn = (gq[0] + gq[1]) + (gq[2] + gq[3]) + (gq[4] + gq[5]) + gq[6];
+        }
+        const double Fy = rho * (double)lp.gbeta * (double)dTn;
         mac[c] = rho;
         mac[N + c] = jx / rho;
         mac[2 * N + c] = (jy + 0.5 * Fy) / rho;
@@ -523,6 +533,80 @@ __global__ void lp_bprev_to_f64(const float *__restrict__ Bp, double *__restrict
 }
 
 // -------------------------------------------------------------------------------------------
+// One-pass form (lbm_fused.h) for the deviation storage.
+// -------------------------------------------------------------------------------------------
+template <typename SS> struct FormLP {
+    using R = float;
+    using S = SS;
+    using B = float;
+    using P = LPParams;
+    static __device__ __forceinline__ void pull_g(const S *G, uint64_t N, int i, int j, int k,
+                                                  uint64_t c, bool border, const KParams &kp,
+                                                  const P &lp, float *g)
+    {
+        pull_heat_lp<D3Q7, S>(G, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+    }
+    static __device__ __forceinline__ void pull_f(const S *F, uint64_t N, int i, int j, int k,
+                                                  uint64_t c, bool border, const KParams &kp,
+                                                  const P &, float *f)
+    {
+        pull_flow_lp<D3Q19, S>(F, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    }
+    // temperature deviation dT = T - T0
+    static __device__ __forceinline__ float tsum(const float *g)
+    {
+        return (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
+    }
+    // returns drho (density deviation) in the rho slot
+    static __device__ __forceinline__ void macro(const float *f, float dTn, const KParams &,
+                                                 const P &lp, float &drho, float &u, float &v,
+                                                 float &w)
+    {
+        float jx, jy, jz;
+        moments19(f, drho, jx, jy, jz);
+        const float rho = 1.0f + drho;
+        const float ir = 1.0f / rho;
+        const float Fy = rho * lp.gbeta * dTn;
+        u = jx * ir;
+        v = (jy + 0.5f * Fy) * ir;
+        w = jz * ir;
+    }
+    static __device__ __forceinline__ void thermal(const float *g, float *p, float dT, float u,
+                                                   float v, float w, float *bp, const KParams &kp,
+                                                   const P &lp)
+    {
+        collide7_dev(g, p, dT, (float)lp.tref + dT, u, v, w, bp, kp.use_g, lp);
+    }
+    static __device__ __forceinline__ void flow(const float *f, float *out, float drho, float u,
+                                                float v, float w, float dT1, const KParams &,
+                                                const P &lp)
+    {
+        collide19_dev(f, out, drho, u, v, w, dT1, lp);
+    }
+    static __device__ __forceinline__ void store_g(S *G, uint64_t N, uint64_t c, const float *g)
+    {
+        store_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
+    }
+    static __device__ __forceinline__ void store_f(S *F, uint64_t N, uint64_t c, const float *f)
+    {
+        store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
+    }
+    static __device__ __forceinline__ float wall_g(int a, const float *gp, int kind, const P &lp)
+    {
+        const float back = gp[D3Q7::opp[a]];
+        if (kind == T_ADIABATIC) return back;
+        const float tw = (a & 1) ? lp.dthot : lp.dtcold;
+        return -back + 2.0f * (float)w7(a) * tw;
+    }
+};
+
+} // namespace rblbm
+
+#include "lbm_fused.h"
+
+namespace rblbm {
+
+// -------------------------------------------------------------------------------------------
 // launchers
 // -------------------------------------------------------------------------------------------
 static LPParams lp_params(const KParams &kp)
@@ -558,7 +642,12 @@ static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, 
         lp3_flow<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, (const S *)a.Go, a.aux);
         break;
     case LP_STATE:
-        lp3_state<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux, a.mac, a.fo, a.go, a.k0, a.nk);
+        lp3_state<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux, (const S *)a.Gprev,
+                                         a.mac, a.fo, a.go, a.k0, a.nk);
+        break;
+    case LP_FUSED:
+        k3_fused<FormLP<S>, FUSED_TX, FUSED_TY><<<a.fg->nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
+            kp, lp, *a.fg, Fc, (S *)a.Fn, (const S *)a.Gi, (S *)a.Go, a.Bp, a.Bp_out);
         break;
     case LP_INIT:
         lp_init<S><<<grid, bx, 0, s>>>(kp, lp, (S *)a.Fn, (S *)a.Go, a.Bp, a.mac);

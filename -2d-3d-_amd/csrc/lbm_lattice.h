@@ -63,6 +63,13 @@ struct KParams {
     double qk, qnu, sg;
 };
 
+// Launch geometry of the one-pass kernel (lbm_fused.h): ntx x nty tiles of (TX-2) x (TY-2)
+// nodes, z split into chunks of kchunk planes; tile x fastest, then tile y, then chunk.
+struct FusedGeom {
+    unsigned ntx, nty, nzc, nblocks;
+    int kchunk;
+};
+
 // Blocks are dealt to the 8 XCDs round-robin by hardware.  Renumber them so that each XCD
 // owns one contiguous run of logical blocks (a z-slab of the domain): the neighbour rows a
 // pull reads then sit in the same XCD's L2 instead of in another XCD's.

@@ -21,11 +21,12 @@ LIB_PATH = os.path.join(PKG, "lib", "librblbm.so")
 VEL_NOSLIP, VEL_PERIODIC = 0, 1
 T_CONST, T_ADIABATIC, T_PERIODIC = 0, 1, 2
 FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
+KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_ONE_PASS = 0, 1, 2
 
 EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
            "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",
            "lbm_diagnostics", "lbm_gather", "lbm_scatter", "lbm_bytes_per_update",
-           "lbm_last_error", "lbm_error_string")
+           "lbm_set_kernel_variant", "lbm_last_error", "lbm_error_string")
 
 
 class LbmError(RuntimeError):
@@ -85,6 +86,7 @@ def library(path: str = LIB_PATH):
     L.lbm_gather.argtypes = [h, dp, dp, dp]
     L.lbm_scatter.argtypes = [h, dp, dp]
     L.lbm_bytes_per_update.argtypes = [h, dp]
+    L.lbm_set_kernel_variant.argtypes = [h, C.c_int32, C.c_int32]
     L.lbm_last_error.argtypes = [h]
     L.lbm_last_error.restype = C.c_char_p
     L.lbm_error_string.argtypes = [C.c_int]
@@ -204,6 +206,10 @@ class Solver:
         p = _Params()
         self._chk(self.lib.lbm_get_params(self._h, C.byref(p)))
         return {n: getattr(p, n) for n, _ in _Params._fields_}
+
+    def set_kernel_variant(self, variant: int, kchunk: int = 0):
+        """KERNEL_AUTO / KERNEL_TWO_PASS / KERNEL_ONE_PASS; kchunk: z-planes per block."""
+        self._chk(self.lib.lbm_set_kernel_variant(self._h, int(variant), int(kchunk)))
 
     def bytes_per_update(self) -> float:
         b = np.zeros(1)

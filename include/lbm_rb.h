@@ -121,6 +121,15 @@ int lbm_scatter(lbm_handle *h, const double *f, const double *g);
  * written once, plus the M1G history): 464 (FP64 3D), 232 (FP32), 128 (FP16), 256 (FP64 2D). */
 int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 
+/* Kernel form of lbm_step (no reference counterpart: the reference runs eight sweeps).
+ * AUTO picks the one-pass form wherever it applies (3D without periodic thermal walls) and the
+ * two-pass form elsewhere; both give bitwise identical results in the FP64 form.  kchunk > 0
+ * sets the number of z-planes per block of the one-pass form (0 keeps the current value). */
+#define LBM_KERNEL_AUTO 0
+#define LBM_KERNEL_TWO_PASS 1
+#define LBM_KERNEL_ONE_PASS 2
+int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
+
 const char *lbm_last_error(lbm_handle *h);
 const char *lbm_error_string(int code);
 

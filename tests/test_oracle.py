@@ -203,6 +203,40 @@ def test_conduction_state_gives_unit_nusselt(dims, periodic):
     assert d["re_vol"] < 2e-6
 
 
+def test_volume_diagnostics_closed_form():
+    # A uniform flow U through the conduction profile T = Thot - (y/L) dT: every finite
+    # difference of the Nusselt formulas is exact on it (dT/dy = -dT, wall rows included), so
+    # each diagnostic reduces to a closed form in U.  The reload (reconstruct_macro_from_fg) adds
+    # the half force 0.5 rho g beta (T - Tref) to the y momentum, which the closed forms include.
+    cfg = orc.Config(dims=3, nx=8, ny=10, nz=6, rayleigh=1e5, mach=0.1)
+    p = orc.params(cfg)
+    U = np.array([0.01, -0.02, 0.005])
+    y = (np.arange(cfg.ny) + 0.5) / cfg.ny
+    T = (cfg.thot - y * (cfg.thot - cfg.tcold))[None, :, None] * np.ones(cfg.shape)
+    m = Model(3, cfg.nx, cfg.ny, cfg.nz)
+    f = m.feq(np.ones(cfg.shape), np.broadcast_to(U, cfg.shape + (3,)).copy())
+    g = m.geq(T, np.zeros(cfg.shape + (3,)))
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    assert o.set_fg(f, g) == 0
+    d = o.diagnostics()
+    rho = f.sum(axis=-1)
+    v = (rho * U[1] + 0.5 * rho * p["gbeta"] * (T - p["Tref"])) / rho
+    u2 = U[0] ** 2 + v ** 2 + U[2] ** 2
+    L, kappa, nu = p["L"], p["diffusivity"], p["viscosity"]
+    assert d["nu_vol"] == pytest.approx(1 + L / kappa * np.mean(v * T), rel=1e-12)
+    assert d["re_vol"] == pytest.approx(np.sqrt(np.mean(u2)) * L / nu, rel=1e-12)
+    assert d["mass"] == pytest.approx(rho.sum(), rel=1e-14)
+    dT = cfg.thot - cfg.tcold
+    conv = L / kappa * v * (T - p["Tref"])
+    assert d["nu_global"] == pytest.approx((np.mean(conv) + dT) / dT, rel=1e-12)
+    assert d["nu_hot"] == pytest.approx(1.0, rel=1e-12)
+    assert d["nu_cold"] == pytest.approx(1.0, rel=1e-12)
+    jb, jt = cfg.ny // 2 - 1, cfg.ny // 2
+    mid = 0.5 * (conv[:, jb, :] + conv[:, jt, :]).mean() / dT + 1.0
+    assert d["nu_middle"] == pytest.approx(mid, rel=1e-12)
+
+
 def test_restart_reconstruction_roundtrip():
     cfg = orc.Config(dims=3, nx=7, ny=6, nz=5, rayleigh=1e5, mach=0.3)
     o = orc.Oracle(cfg)
