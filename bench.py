@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-nz", type=int, default=64)
+    ap.add_argument("--variant", choices=["auto", "two-pass", "one-pass"], default="auto",
+                    help="kernel form of the step (auto: one-pass where it applies)")
+    ap.add_argument("--kchunk", type=int, default=0, help="z-planes per one-pass block (0: default)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run a device-to-device copy of GIB GiB (counter calibration)")
     return ap.parse_args()
@@ -87,11 +90,12 @@ def cpu_baseline(size, nz, ra, steps):
                       f"oracle, {steps} timed steps after 1 warm-up step ({dt:.1f} s)"}
 
 
-def measure(rblbm, torch, n, ra, prec, steps, warmup):
+def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0):
     """Time `steps` steps of one storage form on the n^3 cavity (inputs resident in HBM)."""
     cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=ra, prandtl=0.71, mach=0.1,
                        precision=prec)
     sol = rblbm.Solver(cfg)
+    sol.set_kernel_variant(variant, kchunk)
     sol.initial_seeded(SEED, 1e-3)
     mass0 = sol.diagnostics()["mass"]
     sol.step(warmup)
@@ -138,8 +142,10 @@ def main():
     forms = {}
     for name in a.forms.split(","):
         prec = {"fp64": rblbm.FP64, "fp32": rblbm.FP32_SHIFT, "fp16": rblbm.FP16_SHIFT}[name]
+        variant = {"auto": rblbm.KERNEL_AUTO, "two-pass": rblbm.KERNEL_TWO_PASS,
+                   "one-pass": rblbm.KERNEL_ONE_PASS}[a.variant]
         forms[name] = measure(rblbm, torch if have_torch else None, n, a.rayleigh, prec,
-                              a.steps, a.warmup)
+                              a.steps, a.warmup, variant, a.kchunk)
     main_form = forms[a.forms.split(",")[0]]
     ms_step = main_form["ms_per_step"]
     glups = main_form["glups"]
@@ -175,7 +181,9 @@ def main():
         "forms": forms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_update": bpu, "kernel": "k3_thermal + k3_flow (one step)"},
+                     "bytes_per_update": bpu,
+                     "kernel": ("k3_thermal + k3_flow (two-pass step)" if a.variant == "two-pass"
+                                else "k3_fused (one-pass step)")},
         "cpu_baseline": cpu,
         "checks": {"nu_vol": diag["nu_vol"], "re_vol": diag["re_vol"],
                    "mass_rel_change": (diag["mass"] - mass0) / mass0},
