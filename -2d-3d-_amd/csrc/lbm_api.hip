@@ -42,6 +42,9 @@ struct lbm_handle {
     bool fused_ok;          // the configuration admits the one-pass form
     FusedGeom fg;
     int64_t itc;
+    bool owns_stream;       // false for the domains of a batch (they share the batch's stream)
+    bool in_batch;          // destroyed with its batch, not by lbm_destroy
+    uint64_t instance;      // index within a batch (seeded-noise key)
     char err[256];
 };
 
@@ -109,12 +112,19 @@ static void release(lbm_handle *h)
     if (h->flag) { (void)hipFree(h->flag); h->flag = nullptr; }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
     h->ev0 = h->ev1 = nullptr;
     h->stream = nullptr;
 }
 
+static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **out);
+
 extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
+{
+    return create_impl(cfg, nullptr, out);
+}
+
+static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **out)
 {
     if (!cfg || !out) return LBM_ERR_ARG;
     *out = nullptr;
@@ -167,7 +177,9 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
     int rc = LBM_OK;
     hipError_t e = hipSetDevice(c.device);
     if (e != hipSuccess) { free(h); return LBM_ERR_HIP; }
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+    h->owns_stream = shared == nullptr;
+    if (shared) h->stream = shared;
+    if ((!shared && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
         release(h);
         free(h);
@@ -207,6 +219,7 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
 extern "C" int lbm_destroy(lbm_handle *h)
 {
     if (!h) return LBM_ERR_ARG;
+    if (h->in_batch) return fail(h, LBM_ERR_STATE, "domain belongs to a batch: use lbm_batch_destroy");
     (void)hipSetDevice(h->cfg.device);
     (void)hipStreamSynchronize(h->stream);
     release(h);
@@ -229,6 +242,7 @@ static int do_init(lbm_handle *h, const double *T0, uint64_t seed, double amp, b
         if (c.dims == 2 && c.rayleigh < 1.0e4) ip.rb_perturb = 2;    // 2DRB/2DRB.F90:568
     }
     ip.seed = seed;
+    ip.instance = h->instance;
     ip.noise_scale = seeded ? amp * (c.thot - c.tcold) : 0.0;
     if (T0) HIPCHK(h, hipMemcpyAsync(h->mac + 4 * h->N, T0, h->N * sizeof(double),
                                      hipMemcpyHostToDevice, h->stream));
@@ -750,5 +764,282 @@ extern "C" int lbm_scatter(lbm_handle *h, const double *f, const double *g)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->phase = 0;
     if (bad) return fail(h, LBM_ERR_RHO, "non-positive rho found during restart reconstruction");
+    return LBM_OK;
+}
+
+// =========================================================================================
+// Batched ensembles
+// =========================================================================================
+struct lbm_batch {
+    int n;
+    int precision;
+    lbm_handle **dom;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    int mode;
+    // one-launch tables, one per buffer parity, built at the first batched step
+    void *table[2];
+    uint16_t *owner;
+    unsigned nblocks;
+    int table_parity;      // parity whose table matches the domains' current buffers
+    bool tables_ready;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    bool graph_ready;
+    int table_parity_graph;   // parity the graph's first step starts from
+};
+
+static void batch_release(lbm_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int p = 0; p < 2; ++p)
+        if (b->table[p]) (void)hipFree(b->table[p]);
+    if (b->owner) (void)hipFree(b->owner);
+    if (b->dom) {
+        for (int i = 0; i < b->n; ++i)
+            if (b->dom[i]) {
+                b->dom[i]->in_batch = false;
+                lbm_destroy(b->dom[i]);
+            }
+        free(b->dom);
+    }
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    free(b);
+}
+
+extern "C" int lbm_batch_create(const lbm_config *cfgs, int32_t n, lbm_batch **out)
+{
+    if (!cfgs || !out || n <= 0 || n > 65535) return LBM_ERR_ARG;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i)
+        if (cfgs[i].dims != 3 || cfgs[i].precision != cfgs[0].precision ||
+            cfgs[i].device != cfgs[0].device || cfgs[i].temp_x == LBM_T_PERIODIC ||
+            cfgs[i].temp_z == LBM_T_PERIODIC)
+            return LBM_ERR_UNSUPPORTED;   // one-pass batched form: 3D walls, one precision
+    lbm_batch *b = (lbm_batch *)calloc(1, sizeof(lbm_batch));
+    if (!b) return LBM_ERR_NOMEM;
+    b->n = n;
+    b->precision = cfgs[0].precision;
+    b->mode = LBM_BATCH_GRAPH;
+    if (hipSetDevice(cfgs[0].device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        batch_release(b);
+        return LBM_ERR_HIP;
+    }
+    b->dom = (lbm_handle **)calloc(n, sizeof(lbm_handle *));
+    if (!b->dom) { batch_release(b); return LBM_ERR_NOMEM; }
+    for (int i = 0; i < n; ++i) {
+        int rc = create_impl(&cfgs[i], b->stream, &b->dom[i]);
+        if (rc) { batch_release(b); return rc; }
+        b->dom[i]->in_batch = true;
+        b->dom[i]->instance = (uint64_t)i;
+    }
+    *out = b;
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_destroy(lbm_batch *b)
+{
+    if (!b) return LBM_ERR_ARG;
+    (void)hipStreamSynchronize(b->stream);
+    batch_release(b);
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_domain(lbm_batch *b, int32_t i, lbm_handle **h)
+{
+    if (!b || !h || i < 0 || i >= b->n) return LBM_ERR_ARG;
+    *h = b->dom[i];
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_nodes(lbm_batch *b, int64_t *nodes)
+{
+    if (!b || !nodes) return LBM_ERR_ARG;
+    int64_t s = 0;
+    for (int i = 0; i < b->n; ++i) s += (int64_t)b->dom[i]->N;
+    *nodes = s;
+    return LBM_OK;
+}
+
+static void batch_invalidate(lbm_batch *b)
+{
+    b->tables_ready = false;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    b->graph_ready = false;
+}
+
+extern "C" int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp)
+{
+    if (!b) return LBM_ERR_ARG;
+    for (int i = 0; i < b->n; ++i) {
+        int rc = lbm_init_seeded(b->dom[i], seed, amp);
+        if (rc) return rc;
+    }
+    batch_invalidate(b);
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_set_mode(lbm_batch *b, int32_t mode)
+{
+    if (!b || mode < LBM_BATCH_PER_DOMAIN || mode > LBM_BATCH_GRAPH) return LBM_ERR_ARG;
+    b->mode = mode;
+    return LBM_OK;
+}
+
+// Tables for both buffer parities from the domains' current pointers: parity p is the current
+// state, parity p^1 the state after one one-pass step (F, G, and B_prev when use_g, swapped).
+static int batch_tables(lbm_batch *b)
+{
+    lbm_handle *h0 = b->dom[0];
+    const bool lp = h0->lp;
+    const size_t isz = lp ? fused_instance_bytes_lp(b->precision) : fused_instance_bytes_f64();
+    unsigned total = 0;
+    for (int i = 0; i < b->n; ++i) total += b->dom[i]->fg.nblocks;
+    if (!b->owner) {
+        if (hipMalloc((void **)&b->owner, total * sizeof(uint16_t)) != hipSuccess ||
+            hipMalloc(&b->table[0], isz * b->n) != hipSuccess ||
+            hipMalloc(&b->table[1], isz * b->n) != hipSuccess)
+            return fail(h0, LBM_ERR_NOMEM, "batch tables");
+        std::vector<uint16_t> own(total);
+        unsigned at = 0;
+        for (int i = 0; i < b->n; ++i)
+            for (unsigned k = 0; k < b->dom[i]->fg.nblocks; ++k) own[at++] = (uint16_t)i;
+        HIPCHK(h0, hipMemcpy(b->owner, own.data(), total * sizeof(uint16_t), hipMemcpyHostToDevice));
+        b->nblocks = total;
+    }
+    std::vector<unsigned char> host(isz * b->n);
+    for (int par = 0; par < 2; ++par) {
+        unsigned first = 0;
+        for (int i = 0; i < b->n; ++i) {
+            lbm_handle *h = b->dom[i];
+            const int fc = h->fcur ^ par, gc = h->gcur ^ par;
+            const bool sw = par == 1 && h->cfg.use_g;
+            FusedPtrs p;
+            p.Fcur = h->F[fc];
+            p.Fnext = h->F[fc ^ 1];
+            p.Gin = h->G[gc];
+            p.Gout = h->G[gc ^ 1];
+            p.Bp = sw ? h->Bp_alt : h->Bp;
+            p.Bpout = sw ? h->Bp : h->Bp_alt;
+            if (lp) fused_instance_fill_lp(b->precision, host.data() + isz * i, h->kp, h->fg, p, first);
+            else fused_instance_fill_f64(host.data() + isz * i, h->kp, h->fg, p, first);
+            first += h->fg.nblocks;
+        }
+        HIPCHK(h0, hipMemcpy(b->table[par], host.data(), host.size(), hipMemcpyHostToDevice));
+    }
+    b->table_parity = 0;
+    b->tables_ready = true;
+    return LBM_OK;
+}
+
+static hipError_t batch_launch(lbm_batch *b, int par)
+{
+    if (b->dom[0]->lp) return launch_fused_batch_lp(b->precision, b->table[par], b->owner, b->nblocks, b->stream);
+    return launch_fused_batch_f64(b->table[par], b->owner, b->nblocks, b->stream);
+}
+
+// Host bookkeeping of one one-pass step on every domain (buffers swap like enqueue_steps).
+static void batch_advance(lbm_batch *b)
+{
+    for (int i = 0; i < b->n; ++i) {
+        lbm_handle *h = b->dom[i];
+        h->fcur ^= 1;
+        h->gcur ^= 1;
+        h->aux_valid = 0;
+        h->itc += 1;
+        if (h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
+    }
+    b->table_parity ^= 1;
+}
+
+static int batch_enqueue(lbm_batch *b, int32_t nsteps)
+{
+    // the first step after initialisation reads the plain state: per domain, two-pass
+    for (int i = 0; i < b->n; ++i) {
+        lbm_handle *h = b->dom[i];
+        if (h->phase < 0) return fail(h, LBM_ERR_STATE, "batch domain %d not initialised", i);
+    }
+    int done = 0;
+    if (nsteps > 0 && (b->dom[0]->phase == 0 || b->dom[0]->first_thermal)) {
+        for (int i = 0; i < b->n; ++i) {
+            int rc = enqueue_steps(b->dom[i], 1);
+            if (rc) return rc;
+        }
+        batch_invalidate(b);
+        done = 1;
+    }
+    if (b->mode == LBM_BATCH_PER_DOMAIN) {
+        for (int s = done; s < nsteps; ++s)
+            for (int i = 0; i < b->n; ++i) {
+                int rc = enqueue_steps(b->dom[i], 1);
+                if (rc) return rc;
+            }
+        batch_invalidate(b);
+        return LBM_OK;
+    }
+    if (done >= nsteps) return LBM_OK;
+    if (!b->tables_ready) {
+        int rc = batch_tables(b);
+        if (rc) return rc;
+    }
+    lbm_handle *h0 = b->dom[0];
+    if (b->mode == LBM_BATCH_GRAPH) {
+        if (!b->graph_ready) {
+            // two steps return every buffer to its place: capture parity p then p^1
+            HIPCHK(h0, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = batch_launch(b, b->table_parity);
+            if (e == hipSuccess) e = batch_launch(b, b->table_parity ^ 1);
+            hipError_t e2 = hipStreamEndCapture(b->stream, &b->graph);
+            if (e != hipSuccess || e2 != hipSuccess)
+                return fail(h0, LBM_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+            HIPCHK(h0, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            b->graph_ready = true;
+            b->table_parity_graph = b->table_parity;
+        }
+        if (b->table_parity != b->table_parity_graph && nsteps - done >= 3) {
+            HIPCHK(h0, batch_launch(b, b->table_parity));   // realign with the graph's parity
+            batch_advance(b);
+            ++done;
+        }
+        while (nsteps - done >= 2 && b->table_parity == b->table_parity_graph) {
+            HIPCHK(h0, hipGraphLaunch(b->gexec, b->stream));
+            batch_advance(b);
+            batch_advance(b);
+            done += 2;
+        }
+    }
+    for (; done < nsteps; ++done) {
+        HIPCHK(h0, batch_launch(b, b->table_parity));
+        batch_advance(b);
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_step(lbm_batch *b, int32_t nsteps)
+{
+    if (!b || nsteps < 0) return LBM_ERR_ARG;
+    HIPCHK(b->dom[0], hipSetDevice(b->dom[0]->cfg.device));
+    return batch_enqueue(b, nsteps);
+}
+
+extern "C" int lbm_batch_step_timed(lbm_batch *b, int32_t nsteps, double *ms)
+{
+    if (!b || nsteps < 0 || !ms) return LBM_ERR_ARG;
+    lbm_handle *h0 = b->dom[0];
+    HIPCHK(h0, hipSetDevice(h0->cfg.device));
+    HIPCHK(h0, hipEventRecord(b->ev0, b->stream));
+    int rc = batch_enqueue(b, nsteps);
+    if (rc) return rc;
+    HIPCHK(h0, hipEventRecord(b->ev1, b->stream));
+    HIPCHK(h0, hipEventSynchronize(b->ev1));
+    float t = 0.f;
+    HIPCHK(h0, hipEventElapsedTime(&t, b->ev0, b->ev1));
+    *ms = t;
     return LBM_OK;
 }

@@ -31,18 +31,18 @@
 
 namespace rblbm {
 
+// One block's work: tile number lb (tile x fastest, then tile y, then z-chunk) of one domain.
 template <class Form, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY)
-k3_fused(KParams kp, typename Form::P fp, FusedGeom fg, const typename Form::S *__restrict__ Fcur,
-         typename Form::S *__restrict__ Fnext, const typename Form::S *__restrict__ Gin,
-         typename Form::S *__restrict__ Gout, const typename Form::B *__restrict__ Bp,
-         typename Form::B *__restrict__ Bpout)
+__device__ __forceinline__ void
+fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, unsigned lb,
+           const typename Form::S *__restrict__ Fcur, typename Form::S *__restrict__ Fnext,
+           const typename Form::S *__restrict__ Gin, typename Form::S *__restrict__ Gout,
+           const typename Form::B *__restrict__ Bp, typename Form::B *__restrict__ Bpout)
 {
     using R = typename Form::R;
     constexpr int NT = TX * TY;
     __shared__ R lds[3][4][NT];
 
-    const unsigned lb = xcd_remap(blockIdx.x, fg.nblocks);
     const unsigned tile_x = lb % fg.ntx;
     const unsigned rest = lb / fg.ntx;
     const unsigned tile_y = rest % fg.nty;
@@ -129,6 +129,44 @@ k3_fused(KParams kp, typename Form::P fp, FusedGeom fg, const typename Form::S *
         w_p = w_n;
         have_prev = have;
     }
+}
+
+template <class Form, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY)
+k3_fused(KParams kp, typename Form::P fp, FusedGeom fg, const typename Form::S *__restrict__ Fcur,
+         typename Form::S *__restrict__ Fnext, const typename Form::S *__restrict__ Gin,
+         typename Form::S *__restrict__ Gout, const typename Form::B *__restrict__ Bp,
+         typename Form::B *__restrict__ Bpout)
+{
+    fused_tile<Form, TX, TY>(kp, fp, fg, xcd_remap(blockIdx.x, fg.nblocks), Fcur, Fnext, Gin, Gout,
+                             Bp, Bpout);
+}
+
+// Batched form: many independent domains advance in one launch.  Block b (after the XCD remap)
+// looks up its domain in `owner` and its domain's parameters and buffers in `inst`; the first
+// block of domain d is inst[d].first_block.
+template <class Form> struct FusedInstance {
+    KParams kp;
+    typename Form::P fp;
+    FusedGeom fg;
+    const typename Form::S *Fcur;
+    typename Form::S *Fnext;
+    const typename Form::S *Gin;
+    typename Form::S *Gout;
+    const typename Form::B *Bp;
+    typename Form::B *Bpout;
+    unsigned first_block;
+};
+
+template <class Form, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY)
+k3_fused_batch(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restrict__ owner,
+               unsigned nblocks)
+{
+    const unsigned b = xcd_remap(blockIdx.x, nblocks);
+    const FusedInstance<Form> &d = inst[owner[b]];
+    fused_tile<Form, TX, TY>(d.kp, d.fp, d.fg, b - d.first_block, d.Fcur, d.Fnext, d.Gin, d.Gout,
+                             d.Bp, d.Bpout);
 }
 
 } // namespace rblbm

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "lbm_lattice.h"
 
@@ -13,6 +14,7 @@ struct InitParams {
     int rb_perturb;       // 0 none, 3 = 3D cos*sin (Ra <= 1e4), 2 = 2D sin*sin (Ra < 1e4)
     double L, pi, perturb_amp;
     uint64_t seed;
+    uint64_t instance;    // domain index within a batch (part of the noise key)
     double noise_scale;   // amp * (thot - tcold); 0 disables the noise
 };
 
@@ -75,5 +77,27 @@ struct LPArgs {
     float *Bp_out = nullptr;       // LP_FUSED: updated history
 };
 hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a);
+
+// Batched one-pass launch (many domains per launch).  The per-domain table entry is
+// form-specific; these helpers size and fill it on the host, then launch over `nblocks` blocks
+// whose owner domain is `owner[block]`.
+struct FusedPtrs {
+    const void *Fcur;
+    void *Fnext;
+    const void *Gin;
+    void *Gout;
+    const void *Bp;
+    void *Bpout;
+};
+size_t fused_instance_bytes_f64();
+void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, const FusedPtrs &p,
+                             unsigned first_block);
+hipError_t launch_fused_batch_f64(const void *inst, const uint16_t *owner, unsigned nblocks,
+                                  hipStream_t s);
+size_t fused_instance_bytes_lp(int precision);
+void fused_instance_fill_lp(int precision, void *dst, const KParams &kp, const FusedGeom &fg,
+                            const FusedPtrs &p, unsigned first_block);
+hipError_t launch_fused_batch_lp(int precision, const void *inst, const uint16_t *owner,
+                                 unsigned nblocks, hipStream_t s);
 
 } // namespace rblbm

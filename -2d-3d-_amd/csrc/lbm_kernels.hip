@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(256) k_init(KParams kp, InitParams ip, double 
         }
     }
     if (ip.noise_scale != 0.0) {
-        const uint64_t key = ip.seed + 0x9E3779B97F4A7C15ULL * (c + 1ULL);
+        const uint64_t key = ip.seed + 0x9E3779B97F4A7C15ULL * (c + 1ULL + (ip.instance << 40));
         const double r = (double)(mix64(key) >> 11) * 0x1.0p-53;
         const double noise = ip.noise_scale * (2.0 * r - 1.0);
         T = T + noise;
@@ -779,6 +779,33 @@ hipError_t launch_fused_f64(const KParams &kp, const FusedGeom &fg, hipStream_t 
     FormF64::P p{kp.thot, kp.tcold};
     k3_fused<FormF64, FUSED_TX, FUSED_TY><<<fg.nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
         kp, p, fg, Fcur, Fnext, Gin, Gout, Bp, Bp_out);
+    return hipGetLastError();
+}
+
+size_t fused_instance_bytes_f64() { return sizeof(FusedInstance<FormF64>); }
+
+void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, const FusedPtrs &p,
+                             unsigned first_block)
+{
+    FusedInstance<FormF64> d;
+    d.kp = kp;
+    d.fp = FormF64::P{kp.thot, kp.tcold};
+    d.fg = fg;
+    d.Fcur = (const double *)p.Fcur;
+    d.Fnext = (double *)p.Fnext;
+    d.Gin = (const double *)p.Gin;
+    d.Gout = (double *)p.Gout;
+    d.Bp = (const double *)p.Bp;
+    d.Bpout = (double *)p.Bpout;
+    d.first_block = first_block;
+    memcpy(dst, &d, sizeof(d));
+}
+
+hipError_t launch_fused_batch_f64(const void *inst, const uint16_t *owner, unsigned nblocks,
+                                  hipStream_t s)
+{
+    k3_fused_batch<FormF64, FUSED_TX, FUSED_TY><<<nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
+        (const FusedInstance<FormF64> *)inst, owner, nblocks);
     return hipGetLastError();
 }
 

@@ -683,6 +683,49 @@ static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, 
     return hipGetLastError();
 }
 
+template <typename S>
+static void fill_lp(void *dst, const KParams &kp, const FusedGeom &fg, const FusedPtrs &p,
+                    unsigned first_block)
+{
+    FusedInstance<FormLP<S>> d;
+    d.kp = kp;
+    d.fp = lp_params(kp);
+    d.fg = fg;
+    d.Fcur = (const S *)p.Fcur;
+    d.Fnext = (S *)p.Fnext;
+    d.Gin = (const S *)p.Gin;
+    d.Gout = (S *)p.Gout;
+    d.Bp = (const float *)p.Bp;
+    d.Bpout = (float *)p.Bpout;
+    d.first_block = first_block;
+    memcpy(dst, &d, sizeof(d));
+}
+
+size_t fused_instance_bytes_lp(int precision)
+{
+    return precision == 1 ? sizeof(FusedInstance<FormLP<float>>)
+                          : sizeof(FusedInstance<FormLP<_Float16>>);
+}
+
+void fused_instance_fill_lp(int precision, void *dst, const KParams &kp, const FusedGeom &fg,
+                            const FusedPtrs &p, unsigned first_block)
+{
+    if (precision == 1) fill_lp<float>(dst, kp, fg, p, first_block);
+    else fill_lp<_Float16>(dst, kp, fg, p, first_block);
+}
+
+hipError_t launch_fused_batch_lp(int precision, const void *inst, const uint16_t *owner,
+                                 unsigned nblocks, hipStream_t s)
+{
+    if (precision == 1)
+        k3_fused_batch<FormLP<float>, FUSED_TX, FUSED_TY><<<nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
+            (const FusedInstance<FormLP<float>> *)inst, owner, nblocks);
+    else
+        k3_fused_batch<FormLP<_Float16>, FUSED_TX, FUSED_TY><<<nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
+            (const FusedInstance<FormLP<_Float16>> *)inst, owner, nblocks);
+    return hipGetLastError();
+}
+
 hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a)
 {
     if (precision == 1) return lp_dispatch<float>(op, kp, bx, s, a);

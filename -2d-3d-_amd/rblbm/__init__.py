@@ -247,3 +247,83 @@ class Solver:
         f = np.ascontiguousarray(f, dtype=np.float64)
         g = np.ascontiguousarray(g, dtype=np.float64)
         self._chk(self.lib.lbm_scatter(self._h, _p(f), _p(g)))
+
+
+BATCH_PER_DOMAIN, BATCH_ONE_LAUNCH, BATCH_GRAPH = 0, 1, 2
+
+
+class _Domain(Solver):
+    """A domain owned by a Batch: the ordinary per-domain calls, no own lifetime."""
+
+    def __init__(self, cfg: Config, handle: C.c_void_p, lib):
+        self.cfg, self.lib, self._h = cfg, lib, handle
+
+    def close(self):
+        self._h = C.c_void_p()
+
+
+class Batch:
+    """Independent cavities advancing together on one device (lbm_batch_* entry points)."""
+
+    def __init__(self, cfgs):
+        self.lib = library()
+        L = self.lib
+        if not hasattr(L, "_batch_bound"):
+            h = C.c_void_p
+            L.lbm_batch_create.argtypes = [C.POINTER(_Config), C.c_int32, C.POINTER(h)]
+            L.lbm_batch_destroy.argtypes = [h]
+            L.lbm_batch_domain.argtypes = [h, C.c_int32, C.POINTER(h)]
+            L.lbm_batch_init_seeded.argtypes = [h, C.c_uint64, C.c_double]
+            L.lbm_batch_set_mode.argtypes = [h, C.c_int32]
+            L.lbm_batch_step.argtypes = [h, C.c_int32]
+            L.lbm_batch_step_timed.argtypes = [h, C.c_int32, C.POINTER(C.c_double)]
+            L.lbm_batch_nodes.argtypes = [h, C.POINTER(C.c_int64)]
+            L._batch_bound = True
+        self.cfgs = list(cfgs)
+        arr = (_Config * len(self.cfgs))(*[c.c() for c in self.cfgs])
+        self._b = C.c_void_p()
+        rc = L.lbm_batch_create(arr, len(self.cfgs), C.byref(self._b))
+        if rc != 0:
+            raise LbmError(rc, L.lbm_error_string(rc).decode())
+        self.domains = []
+        for i, c in enumerate(self.cfgs):
+            hd = C.c_void_p()
+            L.lbm_batch_domain(self._b, i, C.byref(hd))
+            self.domains.append(_Domain(c, hd, L))
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise LbmError(rc, self.lib.lbm_error_string(rc).decode())
+
+    def close(self):
+        if self._b:
+            for d in self.domains:
+                d.close()
+            self.lib.lbm_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initial_seeded(self, seed: int, amp: float = 1.0e-3):
+        self._chk(self.lib.lbm_batch_init_seeded(self._b, seed, amp))
+
+    def set_mode(self, mode: int):
+        self._chk(self.lib.lbm_batch_set_mode(self._b, int(mode)))
+
+    def step(self, n: int = 1):
+        self._chk(self.lib.lbm_batch_step(self._b, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        ms = C.c_double()
+        self._chk(self.lib.lbm_batch_step_timed(self._b, int(n), C.byref(ms)))
+        return ms.value
+
+    @property
+    def nodes(self) -> int:
+        v = C.c_int64()
+        self._chk(self.lib.lbm_batch_nodes(self._b, C.byref(v)))
+        return v.value

@@ -130,6 +130,23 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 #define LBM_KERNEL_ONE_PASS 2
 int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 
+/* ---- Batched ensembles (no reference counterpart: the reference runs one cavity per
+ * process).  A batch owns n independent domains of any shape and parameters; they share one
+ * HIP stream and advance together.  Domain i is an ordinary handle for every per-domain call
+ * (init, diagnostics, macros, gather); its seeded noise uses i as the instance key. */
+typedef struct lbm_batch lbm_batch;
+#define LBM_BATCH_PER_DOMAIN 0    /* one launch per domain per step */
+#define LBM_BATCH_ONE_LAUNCH 1    /* one launch per step for all domains (per-domain table) */
+#define LBM_BATCH_GRAPH 2         /* ONE_LAUNCH, two steps captured in a HIP graph and replayed */
+int lbm_batch_create(const lbm_config *cfgs, int32_t n, lbm_batch **out);
+int lbm_batch_destroy(lbm_batch *b);
+int lbm_batch_domain(lbm_batch *b, int32_t i, lbm_handle **h);
+int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp);
+int lbm_batch_set_mode(lbm_batch *b, int32_t mode);
+int lbm_batch_step(lbm_batch *b, int32_t nsteps);
+int lbm_batch_step_timed(lbm_batch *b, int32_t nsteps, double *ms);
+int lbm_batch_nodes(lbm_batch *b, int64_t *nodes);
+
 const char *lbm_last_error(lbm_handle *h);
 const char *lbm_error_string(int code);
 

@@ -1,0 +1,77 @@
+"""Batched ensembles: independent cavities of different shapes and parameters advancing in one
+launch per step (plain or HIP-graph replay) or one launch per domain, against the oracle."""
+import numpy as np
+import pytest
+
+import orc
+import rblbm
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(24, 20, 16, 1e5, 0.71), (32, 32, 32, 1e6, 7.0), (17, 29, 13, 3e4, 0.71),
+          (40, 24, 20, 1e7, 7.0), (16, 16, 48, 1e4, 0.71)]
+
+
+def _cfgs(prec):
+    return [rblbm.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=ra, prandtl=pr, precision=prec)
+            for a, b, c, ra, pr in SHAPES]
+
+
+def _oracles(seed, amp):
+    out = []
+    for i, (a, b, c, ra, pr) in enumerate(SHAPES):
+        o = orc.Oracle(orc.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=ra, prandtl=pr))
+        o.init_seeded(seed, amp, instance=i)
+        out.append(o)
+    return out
+
+
+@pytest.mark.parametrize("mode", [rblbm.BATCH_PER_DOMAIN, rblbm.BATCH_ONE_LAUNCH, rblbm.BATCH_GRAPH])
+def test_batch_fp64_bitwise_against_oracle(mode):
+    b = rblbm.Batch(_cfgs(rblbm.FP64))
+    b.set_mode(mode)
+    b.initial_seeded(31, 0.05)
+    ora = _oracles(31, 0.05)
+    done = 0
+    for n in (1, 2, 5, 12):
+        b.step(n - done)
+        for o in ora:
+            o.step(n - done)
+        done = n
+        for d, o in zip(b.domains, ora):
+            st, s = d.gather(), o.state()
+            np.testing.assert_array_equal(st["f"], s["f"])
+            np.testing.assert_array_equal(st["g"], s["g"])
+            np.testing.assert_array_equal(st["bx"], s["bx"])
+    for d, o in zip(b.domains, ora):
+        assert d.diagnostics()["nu_vol"] == pytest.approx(o.diagnostics()["nu_vol"], rel=1e-12)
+    assert b.nodes == sum(a * bb * c for a, bb, c, _, _ in SHAPES)
+    b.close()
+
+
+def test_batch_fp32_graph_tracks_oracle():
+    b = rblbm.Batch(_cfgs(rblbm.FP32_SHIFT))
+    b.set_mode(rblbm.BATCH_GRAPH)
+    b.initial_seeded(5, 0.05)
+    ora = _oracles(5, 0.05)
+    b.step(101)                     # first step two-pass, then 50 graph replays
+    for o in ora:
+        o.step(101)
+    for d, o in zip(b.domains, ora):
+        m, s = d.macros(), o.state()
+        umax = max(np.abs(s[k]).max() for k in ("u", "v", "w"))
+        du = max(np.abs(m[k] - s[k]).max() for k in ("u", "v", "w")) / umax
+        assert du <= 2e-4
+        assert np.abs(m["T"] - s["T"]).max() <= 2e-6
+    b.close()
+
+
+def test_batch_rejects_mixed_precision_and_owned_domain_destroy():
+    cfgs = _cfgs(rblbm.FP64)
+    cfgs[1].precision = rblbm.FP32_SHIFT
+    with pytest.raises(rblbm.LbmError):
+        rblbm.Batch(cfgs)
+    b = rblbm.Batch(_cfgs(rblbm.FP64)[:2])
+    lib = b.lib
+    assert lib.lbm_destroy(b.domains[0]._h) == -4
+    b.close()

@@ -148,6 +148,37 @@ def test_c2_128cubed_1000_steps():
     _compare_diag(sol, ora)
 
 
+def test_c2_128cubed_5000_steps():
+    """The full 128^3 run: checked at 1000 and 5000 steps, bitwise on f and g."""
+    sol, ora = _run_pair(3, 128, 128, 128, (1000, 5000), rayleigh=1e6)
+    np.testing.assert_array_equal(sol.gather()["g"], ora.state()["g"])
+    _compare_diag(sol, ora)
+
+
+def test_c1_2d_257x129_20000_steps():
+    """2D, 257 x 129, Ra=1e5, Pr=0.71, Thot=1 / Tcold=0, periodic sides, 20,000 steps."""
+    sol, ora = _run_pair(2, 257, 129, 1, (20000,), rayleigh=1e5, vel_x=1, temp_x=2,
+                         thot=1.0, tcold=0.0)
+    np.testing.assert_array_equal(sol.gather()["f"], ora.state()["f"])
+    _compare_diag(sol, ora)
+
+
+def test_2d_rb_ra1e4_nusselt_matches_published_value():
+    # Wang et al. (2013) Table 12 (2DRB/_paper_extract.txt:1123): 2D periodic RB, Ra=1e4,
+    # Pr=0.71, Ma=0.01: <Nu> = 2.6551312.  128 x 64 lattice at Ma=0.1, run to steady state.
+    cfg = rblbm.Config(dims=2, nx=128, ny=64, nz=1, rayleigh=1e4, vel_x=1, temp_x=2,
+                       thot=1.0, tcold=0.0)
+    sol = rblbm.Solver(cfg)
+    sol.initial_seeded(20261203, 1e-3)
+    steps = int(150 * sol.params()["time_unit"])
+    sol.step(steps)
+    d = sol.diagnostics()
+    print("Nu", d)
+    assert d["nu_hot"] == pytest.approx(2.6551312, rel=0.02)
+    assert d["nu_global"] == pytest.approx(2.6551312, rel=0.02)
+    assert d["nu_vol"] == pytest.approx(2.6551312, rel=0.02)
+
+
 def test_restart_scatter_matches_oracle_reload():
     oc, gc = _cfgs(3, 12, 10, 8, rayleigh=1e5, mach=0.3)
     ora = orc.Oracle(oc)

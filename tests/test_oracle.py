@@ -203,6 +203,19 @@ def test_conduction_state_gives_unit_nusselt(dims, periodic):
     assert d["re_vol"] < 2e-6
 
 
+def test_2d_rb_ra1e4_reproduces_published_nusselt():
+    # Wang et al. (2013) Table 12, quoted in 2DRB/_paper_extract.txt:1123: 2D periodic RB at
+    # Ra = 1e4, Pr = 0.71 has <Nu> = 2.6551312.  The scheme at Ma = 0.1 on 128 x 64 reaches a
+    # steady roll pattern within 100 free-fall times; its Nusselt numbers agree within 1%.
+    cfg = orc.Config(dims=2, nx=128, ny=64, rayleigh=1e4, vel_x=1, temp_x=2, thot=1.0, tcold=0.0)
+    o = orc.Oracle(cfg)
+    o.init_seeded(20261203, 1e-3)
+    o.step(int(100 * orc.params(cfg)["time_unit"]))
+    d = o.diagnostics()
+    for k in ("nu_hot", "nu_cold", "nu_global", "nu_vol"):
+        assert d[k] == pytest.approx(2.6551312, rel=0.01), (k, d[k])
+
+
 def test_volume_diagnostics_closed_form():
     # A uniform flow U through the conduction profile T = Thot - (y/L) dT: every finite
     # difference of the Nusselt formulas is exact on it (dT/dy = -dT, wall rows included), so
