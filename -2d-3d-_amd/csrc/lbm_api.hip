@@ -413,6 +413,82 @@ extern "C" int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kc
     return LBM_OK;
 }
 
+// ---- autotuner ---------------------------------------------------------------------------
+static void tune_key(const lbm_handle *h, char *key, size_t n)
+{
+    const lbm_config &c = h->cfg;
+    snprintf(key, n, "d%d_%dx%dx%d_p%d_w%d%d%d%d%d", c.dims, c.nx, c.ny, c.nz, c.precision,
+             c.vel_x, c.vel_z, c.temp_x, c.temp_y, c.temp_z);
+}
+
+static bool tune_lookup(const char *path, const char *key, int *variant, int *kchunk)
+{
+    if (!path) return false;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return false;
+    char k[128];
+    int v, kc;
+    double ms;
+    bool found = false;
+    while (fscanf(fp, "%127s %d %d %lf", k, &v, &kc, &ms) == 4)
+        if (strcmp(k, key) == 0) {   // the last entry for a key wins
+            *variant = v;
+            *kchunk = kc;
+            found = true;
+        }
+    fclose(fp);
+    return found;
+}
+
+extern "C" int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *ms_per_step)
+{
+    if (!h || steps <= 0) return LBM_ERR_ARG;
+    char key[128];
+    tune_key(h, key, sizeof(key));
+    int variant = 0, kchunk = 0;
+    if (ms_per_step) *ms_per_step = 0.0;
+    if (tune_lookup(cache_path, key, &variant, &kchunk))
+        return lbm_set_kernel_variant(h, variant, kchunk);
+
+    lbm_handle *t = nullptr;
+    int rc = create_impl(&h->cfg, nullptr, &t);
+    if (rc) return fail(h, rc, "autotune: scratch domain: %s", lbm_error_string(rc));
+    rc = lbm_init_seeded(t, 1, 1e-3);
+    if (!rc) rc = lbm_step(t, 1);   // leave the plain state: the candidates time steady steps
+    struct Cand { int variant, kchunk; };
+    std::vector<Cand> cands;
+    cands.push_back({LBM_KERNEL_TWO_PASS, 0});
+    if (t->fused_ok)
+        for (int kc : {4, 8, 16, 32, 64, 128})
+            if (kc <= h->cfg.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc});
+    double best = 1e300;
+    Cand win = cands[0];
+    for (const Cand &cd : cands) {
+        if (rc) break;
+        rc = lbm_set_kernel_variant(t, cd.variant, cd.kchunk);
+        double ms = 0.0;
+        if (!rc) rc = lbm_step(t, 2);                   // warm-up with this form
+        if (!rc) rc = lbm_step_timed(t, steps, &ms);
+        if (!rc && ms / steps < best) {
+            best = ms / steps;
+            win = cd;
+        }
+    }
+    lbm_destroy(t);
+    if (rc) return fail(h, rc, "autotune: timing failed");
+    rc = lbm_set_kernel_variant(h, win.variant, win.kchunk);
+    if (rc) return rc;
+    if (ms_per_step) *ms_per_step = best;
+    if (cache_path) {
+        FILE *fp = fopen(cache_path, "a");
+        if (fp) {
+            fprintf(fp, "%s %d %d %.6f\n", key, win.variant, win.kchunk, best);
+            fclose(fp);
+        }
+    }
+    return LBM_OK;
+}
+
 // Bring the macro planes up to date with the current state (phase 1) or leave the plain ones.
 static int refresh_macros(lbm_handle *h)
 {
@@ -869,6 +945,9 @@ extern "C" int lbm_batch_nodes(lbm_batch *b, int64_t *nodes)
 static void batch_invalidate(lbm_batch *b)
 {
     b->tables_ready = false;
+    for (int p = 0; p < 2; ++p)
+        if (b->table[p]) { (void)hipFree(b->table[p]); b->table[p] = nullptr; }
+    if (b->owner) { (void)hipFree(b->owner); b->owner = nullptr; }
     if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     b->graph_ready = false;
@@ -880,6 +959,27 @@ extern "C" int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp)
     for (int i = 0; i < b->n; ++i) {
         int rc = lbm_init_seeded(b->dom[i], seed, amp);
         if (rc) return rc;
+    }
+    batch_invalidate(b);
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t steps)
+{
+    if (!b || steps <= 0) return LBM_ERR_ARG;
+    HIPCHK(b->dom[0], hipStreamSynchronize(b->stream));
+    // tune each distinct shape once, then copy the choice to every domain of that shape
+    for (int i = 0; i < b->n; ++i) {
+        int j = 0;
+        for (; j < i; ++j)
+            if (b->dom[j]->cfg.nx == b->dom[i]->cfg.nx && b->dom[j]->cfg.ny == b->dom[i]->cfg.ny &&
+                b->dom[j]->cfg.nz == b->dom[i]->cfg.nz)
+                break;
+        int rc;
+        if (j < i) rc = lbm_set_kernel_variant(b->dom[i], b->dom[j]->variant, b->dom[j]->fg.kchunk);
+        else rc = lbm_autotune(b->dom[i], cache_path, steps, nullptr);
+        if (rc) return rc;
+        // the one-launch batch always runs the one-pass kernel; keep the tuned z-chunk
     }
     batch_invalidate(b);
     return LBM_OK;

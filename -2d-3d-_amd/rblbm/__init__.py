@@ -211,6 +211,15 @@ class Solver:
         """KERNEL_AUTO / KERNEL_TWO_PASS / KERNEL_ONE_PASS; kchunk: z-planes per block."""
         self._chk(self.lib.lbm_set_kernel_variant(self._h, int(variant), int(kchunk)))
 
+    def autotune(self, cache: str | None = None, steps: int = 10) -> float:
+        """Pick the kernel form and z-chunk for this shape (lbm_autotune); returns ms/step."""
+        L = self.lib
+        L.lbm_autotune.argtypes = [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_double)]
+        ms = C.c_double()
+        self._chk(L.lbm_autotune(self._h, cache.encode() if cache else None, int(steps),
+                                 C.byref(ms)))
+        return ms.value
+
     def bytes_per_update(self) -> float:
         b = np.zeros(1)
         self._chk(self.lib.lbm_bytes_per_update(self._h, _p(b)))
@@ -313,6 +322,11 @@ class Batch:
 
     def set_mode(self, mode: int):
         self._chk(self.lib.lbm_batch_set_mode(self._b, int(mode)))
+
+    def autotune(self, cache: str | None = None, steps: int = 10):
+        L = self.lib
+        L.lbm_batch_autotune.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        self._chk(L.lbm_batch_autotune(self._b, cache.encode() if cache else None, int(steps)))
 
     def step(self, n: int = 1):
         self._chk(self.lib.lbm_batch_step(self._b, int(n)))

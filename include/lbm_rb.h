@@ -130,6 +130,14 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 #define LBM_KERNEL_ONE_PASS 2
 int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 
+/* Autotuner (no reference counterpart).  Picks the kernel form and the one-pass z-chunk
+ * (nodes marched per thread) for this handle's shape by timing `steps` steps of each candidate
+ * on a scratch domain of the same configuration, applies the winner and appends it to the text
+ * cache `cache_path` (NULL: no cache).  A shape already in the cache is applied without timing.
+ * The handle's own state is not advanced.  *ms_per_step (may be NULL) receives the winner's
+ * time per step (0 when it came from the cache). */
+int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *ms_per_step);
+
 /* ---- Batched ensembles (no reference counterpart: the reference runs one cavity per
  * process).  A batch owns n independent domains of any shape and parameters; they share one
  * HIP stream and advance together.  Domain i is an ordinary handle for every per-domain call
@@ -146,6 +154,8 @@ int lbm_batch_set_mode(lbm_batch *b, int32_t mode);
 int lbm_batch_step(lbm_batch *b, int32_t nsteps);
 int lbm_batch_step_timed(lbm_batch *b, int32_t nsteps, double *ms);
 int lbm_batch_nodes(lbm_batch *b, int64_t *nodes);
+/* lbm_autotune for each distinct domain shape (the one-launch modes use the tuned z-chunks). */
+int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t steps);
 
 const char *lbm_last_error(lbm_handle *h);
 const char *lbm_error_string(int code);

@@ -65,6 +65,24 @@ def test_one_pass_switching_and_periodic_heat_rejected():
         per.set_kernel_variant(rblbm.KERNEL_ONE_PASS)
 
 
+def test_autotune_caches_choice_and_keeps_state(tmp_path):
+    dims = dict(dims=3, nx=45, ny=37, nz=29, rayleigh=1e5)
+    cache = str(tmp_path / "tune.txt")
+    sol = rblbm.Solver(rblbm.Config(**dims))
+    sol.initial_seeded(2, 0.01)
+    ms = sol.autotune(cache, steps=3)
+    assert ms > 0
+    lines = open(cache).read().split("\n")
+    assert lines[0].startswith("d3_45x37x29_p0")
+    assert sol.autotune(cache, steps=3) == 0.0           # from the cache, no timing
+    ora = orc.Oracle(orc.Config(**dims))
+    ora.init_seeded(2, 0.01)
+    sol.step(6)
+    ora.step(6)
+    assert sol.itc == 6
+    np.testing.assert_array_equal(sol.gather()["f"], ora.state()["f"])
+
+
 @pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
 def test_one_pass_lowprec_matches_two_pass(prec):
     dims = dict(dims=3, nx=45, ny=37, nz=29, rayleigh=1e6, precision=prec)
