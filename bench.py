@@ -71,11 +71,14 @@ def cpu_baseline(size, nz, ra, steps):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
+    # an OMP_NUM_THREADS set by the environment (the host's CPU share) takes precedence
+    cores = int(os.environ.get("OMP_NUM_THREADS", cores))
     os.environ["OMP_NUM_THREADS"] = str(cores)
     os.environ.setdefault("OMP_PROC_BIND", "true")
     os.environ.setdefault("OMP_PLACES", "cores")
     import orc  # the checker / baseline only; never the measured GPU path
 
+    cores = orc.set_threads(cores)
     cfg = orc.Config(dims=3, nx=size, ny=size, nz=nz, rayleigh=ra)
     o = orc.Oracle(cfg)
     o.init_seeded(SEED, 1e-3)

@@ -78,6 +78,8 @@ def lib():
         L.orc_set_fg.restype = C.c_int
         L.orc_check.argtypes = [C.c_void_p, dp, dp]
         L.orc_diagnostics.argtypes = [C.c_void_p, C.POINTER(_Diag)]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
         L.orc_fit_adiabatic_wall_T4.argtypes = [C.c_double, dp, dp, dp]
         L.orc_fit_parabola_ls5.argtypes = [dp, dp, C.c_int, dp, dp]
         _lib = L
@@ -132,6 +134,11 @@ def params(cfg: Config) -> dict:
     c = cfg.c()
     lib().orc_params_compute(C.byref(c), C.byref(p))
     return {n: getattr(p, n) for n, _ in _Params._fields_}
+
+
+def set_threads(n: int) -> int:
+    """OpenMP thread count of the oracle's sweeps; returns the count in effect."""
+    return lib().orc_set_threads(int(n))
 
 
 def noise(seed: int, instance: int, idx: int) -> float:

@@ -19,6 +19,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ----------------------------------------------------------------------------------------- */
 /* Lattice tables.  D3Q19 order, opp and weights: 2DRB/3DRB.F90:656-678.                     */
@@ -178,6 +181,18 @@ void orc_destroy(orc_state *s)
 }
 
 long orc_itc(const orc_state *s) { return s->itc; }
+
+/* Thread count of the OpenMP sweeps (the reference hard-codes 24, 2DRB/3DRB.F90:215). */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 /* ----------------------------------------------------------------------------------------- */
 /* Seeded noise: SplitMix64 finaliser of (seed, instance, global linear index).              */

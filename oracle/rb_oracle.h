@@ -66,6 +66,7 @@ double orc_noise(uint64_t seed, uint64_t instance, uint64_t idx);
 
 void orc_step(orc_state *s, int nsteps);
 long orc_itc(const orc_state *s);
+int orc_set_threads(int n);   /* returns the thread count now in effect */
 
 /* Copy-out of the reference state (Fortran order; f/g population index fastest). */
 void orc_get(const orc_state *s, double *f, double *g, double *rho, double *u, double *v,
