@@ -67,12 +67,31 @@ def main():
     ap.add_argument("--forms", default="fp32")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--out", default="")
+    ap.add_argument("--ref-n", type=int, default=0,
+                    help="run the FP64 reference at this (coarser) edge; fields are compared "
+                         "after averaging the fine run over blocks of (n/ref_n)^3 nodes")
     a = ap.parse_args()
 
-    ref, ref_fields, ref_glups = run("fp64", a.n, a.ra, a.steps, a.every, a.seed, True, a.mach)
+    ref_n = a.ref_n or a.n
+    ratio = a.n // ref_n
+    assert ref_n * ratio == a.n, "--n must be a multiple of --ref-n"
+    # the same physical time: the coarse run takes n/ref_n fewer steps per free-fall time
+    ref_steps, ref_every = a.steps // ratio, max(1, a.every // ratio)
+    ref, ref_fields, ref_glups = run("fp64", ref_n, a.ra, ref_steps, ref_every, a.seed, True,
+                                     a.mach)
+    for s in ref:
+        s["itc"] *= ratio
     result = {"workload": f"3D RB cavity {a.n}^3 Ra={a.ra:g} Pr=0.71 Ma={a.mach}",
+              "reference": f"FP64 at {ref_n}^3, {ref_steps} steps",
               "steps": a.steps, "sample_every": a.every, "avg_from": a.avg_from,
               "fp64": {"glups_incl_diag": ref_glups, "series": ref}}
+
+    def coarsen(x):
+        if ratio == 1:
+            return x
+        nz, ny, nx = x.shape
+        return x.reshape(nz // ratio, ratio, ny // ratio, ratio, nx // ratio, ratio).mean(
+            axis=(1, 3, 5))
 
     def tavg(series, key):
         v = [s[key] for s in series if s["itc"] >= a.avg_from]
@@ -82,6 +101,10 @@ def main():
         result["fp64"][f"tavg_{k}"] = tavg(ref, k)
     for form in a.forms.split(","):
         series, fields, glups = run(form, a.n, a.ra, a.steps, a.every, a.seed, True, a.mach)
+        fields = {k: coarsen(v) for k, v in fields.items()}
+        if ratio > 1:   # samples at the coarse run's times only
+            times = {s["itc"] for s in ref}
+            series = [s for s in series if s["itc"] in times]
         r = {"glups_incl_diag": glups, "series": series}
         for k in KEYS:
             r[f"tavg_{k}"] = tavg(series, k)

@@ -45,6 +45,7 @@ struct lbm_handle {
     bool owns_stream;       // false for the domains of a batch (they share the batch's stream)
     bool in_batch;          // destroyed with its batch, not by lbm_destroy
     uint64_t instance;      // index within a batch (seeded-noise key)
+    size_t dev_bytes;       // device memory held by the domain
     char err[256];
 };
 
@@ -119,6 +120,21 @@ static void release(lbm_handle *h)
 
 static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **out);
 
+// One-pass launch geometry for z-chunks of `kchunk` planes and thread tile shape `tile`.
+static void set_fused_geom(lbm_handle *h, int kchunk, int tile, int prefetch)
+{
+    int tx, ty;
+    fused_tile_dims(tile, &tx, &ty);
+    FusedGeom &g = h->fg;
+    g.kchunk = kchunk > h->cfg.nz ? h->cfg.nz : kchunk;
+    g.tile = tile;
+    g.pf = prefetch;
+    g.ntx = (unsigned)((h->cfg.nx + tx - 3) / (tx - 2));
+    g.nty = (unsigned)((h->cfg.ny + ty - 3) / (ty - 2));
+    g.nzc = (unsigned)((h->cfg.nz + g.kchunk - 1) / g.kchunk);
+    g.nblocks = g.ntx * g.nty * g.nzc;
+}
+
 extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
 {
     return create_impl(cfg, nullptr, out);
@@ -168,11 +184,7 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     // one-pass form: 3D without periodic thermal walls (see lbm_fused.h)
     h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC;
     h->variant = LBM_KERNEL_AUTO;
-    h->fg.kchunk = c.nz >= 256 ? 64 : (c.nz >= 64 ? 32 : c.nz);
-    h->fg.ntx = (unsigned)((c.nx + FUSED_TX - 3) / (FUSED_TX - 2));
-    h->fg.nty = (unsigned)((c.ny + FUSED_TY - 3) / (FUSED_TY - 2));
-    h->fg.nzc = (unsigned)((c.nz + h->fg.kchunk - 1) / h->fg.kchunk);
-    h->fg.nblocks = h->fg.ntx * h->fg.nty * h->fg.nzc;
+    set_fused_geom(h, c.nz >= 256 ? 64 : (c.nz >= 64 ? 32 : c.nz), 0, 0);
 
     int rc = LBM_OK;
     hipError_t e = hipSetDevice(c.device);
@@ -199,6 +211,7 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     };
     for (auto &b : plan) {
         if (hipMalloc((void **)b.p, b.bytes) != hipSuccess) { rc = LBM_ERR_NOMEM; break; }
+        h->dev_bytes += b.bytes;
     }
     h->red_blocks = 1024;
     if (rc == LBM_OK) {
@@ -398,18 +411,43 @@ extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
     return LBM_OK;
 }
 
-extern "C" int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk)
+// Kernel form of one domain (no batch check: the batch code retunes its own domains).
+static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefetch)
 {
-    if (!h || variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < 0)
+    if (variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < 0 ||
+        tile < LBM_TILE_32X16 || tile > LBM_TILE_32X8 || prefetch < 0 || prefetch > 1)
         return LBM_ERR_ARG;
     if (variant == LBM_KERNEL_ONE_PASS && !h->fused_ok)
         return fail(h, LBM_ERR_UNSUPPORTED, "one-pass form needs 3D and non-periodic thermal walls");
     h->variant = variant;
-    if (kchunk > 0) {
-        h->fg.kchunk = kchunk > h->cfg.nz ? h->cfg.nz : kchunk;
-        h->fg.nzc = (unsigned)((h->cfg.nz + h->fg.kchunk - 1) / h->fg.kchunk);
-        h->fg.nblocks = h->fg.ntx * h->fg.nty * h->fg.nzc;
-    }
+    set_fused_geom(h, kchunk > 0 ? kchunk : h->fg.kchunk, tile, prefetch);
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk)
+{
+    if (!h) return LBM_ERR_ARG;
+    if (h->in_batch)
+        return fail(h, LBM_ERR_STATE, "kernel form of a batch domain: use lbm_batch_autotune");
+    return set_form(h, variant, kchunk, h->fg.tile, h->fg.pf);
+}
+
+extern "C" int lbm_set_fused_tile(lbm_handle *h, int32_t tile, int32_t prefetch)
+{
+    if (!h) return LBM_ERR_ARG;
+    if (h->in_batch)
+        return fail(h, LBM_ERR_STATE, "kernel form of a batch domain: use lbm_batch_autotune");
+    return set_form(h, h->variant, 0, tile, prefetch);
+}
+
+extern "C" int lbm_get_kernel_form(lbm_handle *h, int32_t *variant, int32_t *kchunk, int32_t *tile,
+                                   int32_t *prefetch)
+{
+    if (!h) return LBM_ERR_ARG;
+    if (variant) *variant = h->variant;
+    if (kchunk) *kchunk = h->fg.kchunk;
+    if (tile) *tile = h->fg.tile;
+    if (prefetch) *prefetch = h->fg.pf;
     return LBM_OK;
 }
 
@@ -421,51 +459,68 @@ static void tune_key(const lbm_handle *h, char *key, size_t n)
              c.vel_x, c.vel_z, c.temp_x, c.temp_y, c.temp_z);
 }
 
-static bool tune_lookup(const char *path, const char *key, int *variant, int *kchunk)
+// Cache lines: "<key> <variant> <kchunk> <tile> <prefetch> <ms per step>"; the last line for a
+// key wins.
+struct TuneChoice { int variant, kchunk, tile, pf; };
+
+static bool tune_lookup(const char *path, const char *key, TuneChoice *out)
 {
     if (!path) return false;
     FILE *fp = fopen(path, "r");
     if (!fp) return false;
-    char k[128];
-    int v, kc;
-    double ms;
+    char line[256], k[128];
     bool found = false;
-    while (fscanf(fp, "%127s %d %d %lf", k, &v, &kc, &ms) == 4)
-        if (strcmp(k, key) == 0) {   // the last entry for a key wins
-            *variant = v;
-            *kchunk = kc;
+    while (fgets(line, sizeof(line), fp)) {
+        TuneChoice c;
+        double ms;
+        if (sscanf(line, "%127s %d %d %d %d %lf", k, &c.variant, &c.kchunk, &c.tile, &c.pf, &ms) == 6 &&
+            strcmp(k, key) == 0) {
+            *out = c;
             found = true;
         }
+    }
     fclose(fp);
     return found;
 }
 
-extern "C" int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *ms_per_step)
+// Time every candidate form on a scratch domain of the same configuration (z shortened when
+// the device cannot hold a second copy) and keep the fastest.
+static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, double *ms_per_step)
 {
-    if (!h || steps <= 0) return LBM_ERR_ARG;
     char key[128];
     tune_key(h, key, sizeof(key));
-    int variant = 0, kchunk = 0;
     if (ms_per_step) *ms_per_step = 0.0;
-    if (tune_lookup(cache_path, key, &variant, &kchunk))
-        return lbm_set_kernel_variant(h, variant, kchunk);
+    TuneChoice win{};
+    if (tune_lookup(cache_path, key, &win))
+        return set_form(h, win.variant, win.kchunk, win.tile, win.pf);
 
+    lbm_config sc = h->cfg;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    HIPCHK(h, hipMemGetInfo(&free_b, &total_b));
+    const double per_plane = (double)h->dev_bytes / (sc.dims == 3 ? sc.nz : 1);
+    const int fit = (int)(0.8 * (double)free_b / per_plane);
+    if (sc.dims == 3 && fit < sc.nz) {
+        if (fit < 16) return fail(h, LBM_ERR_NOMEM, "autotune: no room for a scratch domain");
+        sc.nz = fit;
+    }
     lbm_handle *t = nullptr;
-    int rc = create_impl(&h->cfg, nullptr, &t);
+    int rc = create_impl(&sc, nullptr, &t);
     if (rc) return fail(h, rc, "autotune: scratch domain: %s", lbm_error_string(rc));
     rc = lbm_init_seeded(t, 1, 1e-3);
     if (!rc) rc = lbm_step(t, 1);   // leave the plain state: the candidates time steady steps
-    struct Cand { int variant, kchunk; };
-    std::vector<Cand> cands;
-    cands.push_back({LBM_KERNEL_TWO_PASS, 0});
+    std::vector<TuneChoice> cands;
+    cands.push_back({LBM_KERNEL_TWO_PASS, h->fg.kchunk, h->fg.tile, h->fg.pf});
     if (t->fused_ok)
-        for (int kc : {4, 8, 16, 32, 64, 128})
-            if (kc <= h->cfg.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc});
+        for (int tile = 0; tile < FUSED_TILES; ++tile)
+            for (int pf = 0; pf < 2; ++pf)
+                for (int kc : {8, 16, 32, 64, 128})
+                    if (kc <= sc.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc, tile, pf});
     double best = 1e300;
-    Cand win = cands[0];
-    for (const Cand &cd : cands) {
+    win = cands[0];
+    for (const TuneChoice &cd : cands) {
         if (rc) break;
-        rc = lbm_set_kernel_variant(t, cd.variant, cd.kchunk);
+        rc = set_form(t, cd.variant, cd.kchunk, cd.tile, cd.pf);
         double ms = 0.0;
         if (!rc) rc = lbm_step(t, 2);                   // warm-up with this form
         if (!rc) rc = lbm_step_timed(t, steps, &ms);
@@ -476,17 +531,26 @@ extern "C" int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps
     }
     lbm_destroy(t);
     if (rc) return fail(h, rc, "autotune: timing failed");
-    rc = lbm_set_kernel_variant(h, win.variant, win.kchunk);
+    rc = set_form(h, win.variant, win.kchunk, win.tile, win.pf);
     if (rc) return rc;
-    if (ms_per_step) *ms_per_step = best;
+    if (ms_per_step) *ms_per_step = best * (double)h->cfg.nz / (double)sc.nz;
     if (cache_path) {
         FILE *fp = fopen(cache_path, "a");
         if (fp) {
-            fprintf(fp, "%s %d %d %.6f\n", key, win.variant, win.kchunk, best);
+            fprintf(fp, "%s %d %d %d %d %.6f\n", key, win.variant, win.kchunk, win.tile, win.pf,
+                    best * (double)h->cfg.nz / (double)sc.nz);
             fclose(fp);
         }
     }
     return LBM_OK;
+}
+
+extern "C" int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *ms_per_step)
+{
+    if (!h || steps <= 0) return LBM_ERR_ARG;
+    if (h->in_batch)
+        return fail(h, LBM_ERR_STATE, "kernel form of a batch domain: use lbm_batch_autotune");
+    return autotune_impl(h, cache_path, steps, ms_per_step);
 }
 
 // Bring the macro planes up to date with the current state (phase 1) or leave the plain ones.
@@ -976,10 +1040,17 @@ extern "C" int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t 
                 b->dom[j]->cfg.nz == b->dom[i]->cfg.nz)
                 break;
         int rc;
-        if (j < i) rc = lbm_set_kernel_variant(b->dom[i], b->dom[j]->variant, b->dom[j]->fg.kchunk);
-        else rc = lbm_autotune(b->dom[i], cache_path, steps, nullptr);
+        const lbm_handle *src = b->dom[j];
+        if (j < i) rc = set_form(b->dom[i], src->variant, src->fg.kchunk, src->fg.tile, src->fg.pf);
+        else rc = autotune_impl(b->dom[i], cache_path, steps, nullptr);
         if (rc) return rc;
-        // the one-launch batch always runs the one-pass kernel; keep the tuned z-chunk
+    }
+    // One launch has one block shape: every domain takes the tile shape and prefetch choice of
+    // domain 0 and keeps the z-chunk tuned for its own shape.
+    for (int i = 1; i < b->n; ++i) {
+        lbm_handle *h = b->dom[i];
+        int rc = set_form(h, h->variant, h->fg.kchunk, b->dom[0]->fg.tile, b->dom[0]->fg.pf);
+        if (rc) return rc;
     }
     batch_invalidate(b);
     return LBM_OK;
@@ -1040,8 +1111,10 @@ static int batch_tables(lbm_batch *b)
 
 static hipError_t batch_launch(lbm_batch *b, int par)
 {
-    if (b->dom[0]->lp) return launch_fused_batch_lp(b->precision, b->table[par], b->owner, b->nblocks, b->stream);
-    return launch_fused_batch_f64(b->table[par], b->owner, b->nblocks, b->stream);
+    const FusedGeom &fg0 = b->dom[0]->fg;
+    if (b->dom[0]->lp)
+        return launch_fused_batch_lp(b->precision, fg0, b->table[par], b->owner, b->nblocks, b->stream);
+    return launch_fused_batch_f64(fg0, b->table[par], b->owner, b->nblocks, b->stream);
 }
 
 // Host bookkeeping of one one-pass step on every domain (buffers swap like enqueue_steps).

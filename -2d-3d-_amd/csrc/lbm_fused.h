@@ -23,6 +23,11 @@
 // A periodic thermal wall would need g_post^n of the opposite side of the domain, which no
 // tile holds; configurations with periodic thermal walls use the two-pass form.
 //
+// Tile shapes and prefetching are compile-time variants (FusedGeom::tile / ::pf pick one at run
+// time): one 512-thread block per CU keeps every wave in step with the plane barrier, while
+// 256-thread tiles let two independent blocks share a CU, and prefetching the next plane's
+// loads overlaps them with the current flow collision.
+//
 // The including translation unit provides `Form` with the storage type S, arithmetic type R,
 // history type B and the per-node operations (pulls, macro, collisions) of its precision form.
 #pragma once
@@ -32,7 +37,7 @@
 namespace rblbm {
 
 // One block's work: tile number lb (tile x fastest, then tile y, then z-chunk) of one domain.
-template <class Form, int TX, int TY>
+template <class Form, int TX, int TY, bool PF>
 __device__ __forceinline__ void
 fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, unsigned lb,
            const typename Form::S *__restrict__ Fcur, typename Form::S *__restrict__ Fnext,
@@ -68,21 +73,47 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     R g5pp = 0;
     bool have_prev = false, have_pp = false;
 
+    // Loads of the thermal part of plane kq: pulled g^n, pulled f^{n+1}, B_prev.
+    using B = typename Form::B;
+    auto load = [&](int kq, R(&lg)[7], R(&lf)[19], B(&lb)[3]) -> bool {
+        if (!active || kq < 0 || kq >= nz) return false;
+        const uint64_t c = col + plane * (uint64_t)kq;
+        const bool border = i == 0 || j == 0 || kq == 0 || i == nx - 1 || j == ny - 1 || kq == nz - 1;
+        Form::pull_g(Gin, N, i, j, kq, c, border, kp, fp, lg);
+        Form::pull_f(Fcur, N, i, j, kq, c, border, kp, fp, lf);
+        lb[0] = Bp[c];
+        lb[1] = Bp[N + c];
+        lb[2] = Bp[2 * N + c];
+        return true;
+    };
+    // With PF the loads of plane kk + 1 are issued before the flow part of iteration kk, so
+    // that their latency hides behind that collision (they then live across it in registers).
+    R qg[7], qf[19];
+    B qb[3];
+    bool q_have = PF ? load(k0 - 1, qg, qf, qb) : false;
+
     for (int kk = k0 - 1; kk <= k1; ++kk) {
+        R lg[7], fnew[19];
+        B bp[3];
+        bool have;
+        if constexpr (PF) {
+            have = q_have;
+#pragma unroll
+            for (int a = 0; a < 7; ++a) lg[a] = qg[a];
+#pragma unroll
+            for (int a = 0; a < 19; ++a) fnew[a] = qf[a];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) bp[a] = qb[a];
+        } else {
+            have = load(kk, lg, fnew, bp);
+        }
         R gnew[7] = {};
-        R fnew[19];
         R rho_n = 0, u_n = 0, v_n = 0, w_n = 0;
-        const bool have = active && kk >= 0 && kk < nz;
         if (have) {
             const uint64_t c = col + plane * (uint64_t)kk;
-            const bool border = i == 0 || j == 0 || kk == 0 || i == nx - 1 || j == ny - 1 || kk == nz - 1;
-            R g[7];
-            Form::pull_g(Gin, N, i, j, kk, c, border, kp, fp, g);
-            const R Tn = Form::tsum(g);
-            Form::pull_f(Fcur, N, i, j, kk, c, border, kp, fp, fnew);
+            const R Tn = Form::tsum(lg);
             Form::macro(fnew, Tn, kp, fp, rho_n, u_n, v_n, w_n);
-            typename Form::B bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
-            Form::thermal(g, gnew, Tn, u_n, v_n, w_n, bp, kp, fp);
+            Form::thermal(lg, gnew, Tn, u_n, v_n, w_n, bp, kp, fp);
             if (tile && kk >= k0 && kk < k1) {
                 Form::store_g(Gout, N, c, gnew);
                 if (kp.use_g) {
@@ -98,6 +129,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         lds[wb][2][tid] = gnew[3];
         lds[wb][3][tid] = gnew[4];
         __syncthreads();
+        if constexpr (PF) q_have = kk < k1 && load(kk + 1, qg, qf, qb);
 
         const int p = kk - 1;
         if (tile && p >= k0 && p < k1 && have_prev) {
@@ -131,15 +163,17 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     }
 }
 
-template <class Form, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY)
+// At least two waves per SIMD: with 256 registers per lane at most, a 512-thread tile fits one
+// CU and 256-thread tiles two per CU.
+template <class Form, int TX, int TY, bool PF>
+__global__ void __launch_bounds__(TX *TY, 2)
 k3_fused(KParams kp, typename Form::P fp, FusedGeom fg, const typename Form::S *__restrict__ Fcur,
          typename Form::S *__restrict__ Fnext, const typename Form::S *__restrict__ Gin,
          typename Form::S *__restrict__ Gout, const typename Form::B *__restrict__ Bp,
          typename Form::B *__restrict__ Bpout)
 {
-    fused_tile<Form, TX, TY>(kp, fp, fg, xcd_remap(blockIdx.x, fg.nblocks), Fcur, Fnext, Gin, Gout,
-                             Bp, Bpout);
+    fused_tile<Form, TX, TY, PF>(kp, fp, fg, xcd_remap(blockIdx.x, fg.nblocks), Fcur, Fnext, Gin,
+                                 Gout, Bp, Bpout);
 }
 
 // Batched form: many independent domains advance in one launch.  Block b (after the XCD remap)
@@ -158,15 +192,60 @@ template <class Form> struct FusedInstance {
     unsigned first_block;
 };
 
-template <class Form, int TX, int TY>
-__global__ void __launch_bounds__(TX *TY)
+template <class Form, int TX, int TY, bool PF>
+__global__ void __launch_bounds__(TX *TY, 2)
 k3_fused_batch(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restrict__ owner,
                unsigned nblocks)
 {
     const unsigned b = xcd_remap(blockIdx.x, nblocks);
     const FusedInstance<Form> &d = inst[owner[b]];
-    fused_tile<Form, TX, TY>(d.kp, d.fp, d.fg, b - d.first_block, d.Fcur, d.Fnext, d.Gin, d.Gout,
-                             d.Bp, d.Bpout);
+    fused_tile<Form, TX, TY, PF>(d.kp, d.fp, d.fg, b - d.first_block, d.Fcur, d.Fnext, d.Gin,
+                                 d.Gout, d.Bp, d.Bpout);
+}
+
+// Run-time tile shapes: index -> (TX, TY).
+constexpr int fused_tx(int tile) { return tile == 0 ? 32 : (tile == 1 ? 16 : 32); }
+constexpr int fused_ty(int tile) { return tile == 0 ? 16 : (tile == 1 ? 16 : 8); }
+
+template <class Form>
+hipError_t launch_fused_form(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg,
+                             hipStream_t s, const typename Form::S *Fcur, typename Form::S *Fnext,
+                             const typename Form::S *Gin, typename Form::S *Gout,
+                             const typename Form::B *Bp, typename Form::B *Bpout)
+{
+#define RBLBM_FUSED(T, PF)                                                                       \
+    k3_fused<Form, fused_tx(T), fused_ty(T), PF><<<fg.nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
+        kp, fp, fg, Fcur, Fnext, Gin, Gout, Bp, Bpout)
+    switch (fg.tile * 2 + (fg.pf ? 1 : 0)) {
+    case 0: RBLBM_FUSED(0, false); break;
+    case 1: RBLBM_FUSED(0, true); break;
+    case 2: RBLBM_FUSED(1, false); break;
+    case 3: RBLBM_FUSED(1, true); break;
+    case 4: RBLBM_FUSED(2, false); break;
+    default: RBLBM_FUSED(2, true); break;
+    }
+#undef RBLBM_FUSED
+    return hipGetLastError();
+}
+
+template <class Form>
+hipError_t launch_fused_batch_form(const FusedGeom &fg0, const void *inst, const uint16_t *owner,
+                                   unsigned nblocks, hipStream_t s)
+{
+    const FusedInstance<Form> *d = (const FusedInstance<Form> *)inst;
+#define RBLBM_FUSED_B(T, PF)                                                                     \
+    k3_fused_batch<Form, fused_tx(T), fused_ty(T), PF><<<nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
+        d, owner, nblocks)
+    switch (fg0.tile * 2 + (fg0.pf ? 1 : 0)) {
+    case 0: RBLBM_FUSED_B(0, false); break;
+    case 1: RBLBM_FUSED_B(0, true); break;
+    case 2: RBLBM_FUSED_B(1, false); break;
+    case 3: RBLBM_FUSED_B(1, true); break;
+    case 4: RBLBM_FUSED_B(2, false); break;
+    default: RBLBM_FUSED_B(2, true); break;
+    }
+#undef RBLBM_FUSED_B
+    return hipGetLastError();
 }
 
 } // namespace rblbm

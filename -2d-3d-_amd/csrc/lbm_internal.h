@@ -30,8 +30,14 @@ hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, 
                         const double *Gpost, const double *aux_or_fy, const double *Gprev,
                         double *mac, double *fo, double *go, int k0, int nk);
 
-// One-pass form: tile of (FUSED_TX - 2) x (FUSED_TY - 2) nodes plus a one-node ring per block.
-constexpr int FUSED_TX = 32, FUSED_TY = 16;
+// One-pass form: tile of (TX - 2) x (TY - 2) nodes plus a one-node ring per block; the thread
+// tile TX x TY is one of three shapes (FusedGeom::tile).
+constexpr int FUSED_TILES = 3;
+inline void fused_tile_dims(int tile, int *tx, int *ty)
+{
+    *tx = tile == 1 ? 16 : 32;
+    *ty = tile == 0 ? 16 : (tile == 1 ? 16 : 8);
+}
 hipError_t launch_fused_f64(const KParams &kp, const FusedGeom &fg, hipStream_t s, const double *Fcur,
                             double *Fnext, const double *Gin, double *Gout, const double *Bp,
                             double *Bp_out);
@@ -92,12 +98,12 @@ struct FusedPtrs {
 size_t fused_instance_bytes_f64();
 void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, const FusedPtrs &p,
                              unsigned first_block);
-hipError_t launch_fused_batch_f64(const void *inst, const uint16_t *owner, unsigned nblocks,
-                                  hipStream_t s);
+hipError_t launch_fused_batch_f64(const FusedGeom &fg0, const void *inst, const uint16_t *owner,
+                                  unsigned nblocks, hipStream_t s);
 size_t fused_instance_bytes_lp(int precision);
 void fused_instance_fill_lp(int precision, void *dst, const KParams &kp, const FusedGeom &fg,
                             const FusedPtrs &p, unsigned first_block);
-hipError_t launch_fused_batch_lp(int precision, const void *inst, const uint16_t *owner,
-                                 unsigned nblocks, hipStream_t s);
+hipError_t launch_fused_batch_lp(int precision, const FusedGeom &fg0, const void *inst,
+                                 const uint16_t *owner, unsigned nblocks, hipStream_t s);
 
 } // namespace rblbm

@@ -776,10 +776,8 @@ hipError_t launch_fused_f64(const KParams &kp, const FusedGeom &fg, hipStream_t 
                             double *Fnext, const double *Gin, double *Gout, const double *Bp,
                             double *Bp_out)
 {
-    FormF64::P p{kp.thot, kp.tcold};
-    k3_fused<FormF64, FUSED_TX, FUSED_TY><<<fg.nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
-        kp, p, fg, Fcur, Fnext, Gin, Gout, Bp, Bp_out);
-    return hipGetLastError();
+    const FormF64::P p{kp.thot, kp.tcold};
+    return launch_fused_form<FormF64>(kp, p, fg, s, Fcur, Fnext, Gin, Gout, Bp, Bp_out);
 }
 
 size_t fused_instance_bytes_f64() { return sizeof(FusedInstance<FormF64>); }
@@ -801,12 +799,10 @@ void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, 
     memcpy(dst, &d, sizeof(d));
 }
 
-hipError_t launch_fused_batch_f64(const void *inst, const uint16_t *owner, unsigned nblocks,
-                                  hipStream_t s)
+hipError_t launch_fused_batch_f64(const FusedGeom &fg0, const void *inst, const uint16_t *owner,
+                                  unsigned nblocks, hipStream_t s)
 {
-    k3_fused_batch<FormF64, FUSED_TX, FUSED_TY><<<nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
-        (const FusedInstance<FormF64> *)inst, owner, nblocks);
-    return hipGetLastError();
+    return launch_fused_batch_form<FormF64>(fg0, inst, owner, nblocks, s);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -646,9 +646,8 @@ static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, 
                                          a.mac, a.fo, a.go, a.k0, a.nk);
         break;
     case LP_FUSED:
-        k3_fused<FormLP<S>, FUSED_TX, FUSED_TY><<<a.fg->nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
-            kp, lp, *a.fg, Fc, (S *)a.Fn, (const S *)a.Gi, (S *)a.Go, a.Bp, a.Bp_out);
-        break;
+        return launch_fused_form<FormLP<S>>(kp, lp, *a.fg, s, Fc, (S *)a.Fn, (const S *)a.Gi,
+                                            (S *)a.Go, a.Bp, a.Bp_out);
     case LP_INIT:
         lp_init<S><<<grid, bx, 0, s>>>(kp, lp, (S *)a.Fn, (S *)a.Go, a.Bp, a.mac);
         break;
@@ -714,16 +713,11 @@ void fused_instance_fill_lp(int precision, void *dst, const KParams &kp, const F
     else fill_lp<_Float16>(dst, kp, fg, p, first_block);
 }
 
-hipError_t launch_fused_batch_lp(int precision, const void *inst, const uint16_t *owner,
-                                 unsigned nblocks, hipStream_t s)
+hipError_t launch_fused_batch_lp(int precision, const FusedGeom &fg0, const void *inst,
+                                 const uint16_t *owner, unsigned nblocks, hipStream_t s)
 {
-    if (precision == 1)
-        k3_fused_batch<FormLP<float>, FUSED_TX, FUSED_TY><<<nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
-            (const FusedInstance<FormLP<float>> *)inst, owner, nblocks);
-    else
-        k3_fused_batch<FormLP<_Float16>, FUSED_TX, FUSED_TY><<<nblocks, FUSED_TX * FUSED_TY, 0, s>>>(
-            (const FusedInstance<FormLP<_Float16>> *)inst, owner, nblocks);
-    return hipGetLastError();
+    if (precision == 1) return launch_fused_batch_form<FormLP<float>>(fg0, inst, owner, nblocks, s);
+    return launch_fused_batch_form<FormLP<_Float16>>(fg0, inst, owner, nblocks, s);
 }
 
 hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a)

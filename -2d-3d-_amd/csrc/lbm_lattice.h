@@ -68,6 +68,8 @@ struct KParams {
 struct FusedGeom {
     unsigned ntx, nty, nzc, nblocks;
     int kchunk;
+    int tile;     // tile shape: 0 = 32 x 16 threads, 1 = 16 x 16, 2 = 32 x 8
+    int pf;       // 1: issue the next plane's loads before the current flow collision
 };
 
 // Blocks are dealt to the 8 XCDs round-robin by hardware.  Renumber them so that each XCD

@@ -22,11 +22,13 @@ VEL_NOSLIP, VEL_PERIODIC = 0, 1
 T_CONST, T_ADIABATIC, T_PERIODIC = 0, 1, 2
 FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_ONE_PASS = 0, 1, 2
+TILE_32X16, TILE_16X16, TILE_32X8 = 0, 1, 2
 
 EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
            "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",
            "lbm_diagnostics", "lbm_gather", "lbm_scatter", "lbm_bytes_per_update",
-           "lbm_set_kernel_variant", "lbm_last_error", "lbm_error_string")
+           "lbm_set_kernel_variant", "lbm_set_fused_tile", "lbm_get_kernel_form",
+           "lbm_last_error", "lbm_error_string")
 
 
 class LbmError(RuntimeError):
@@ -87,6 +89,9 @@ def library(path: str = LIB_PATH):
     L.lbm_scatter.argtypes = [h, dp, dp]
     L.lbm_bytes_per_update.argtypes = [h, dp]
     L.lbm_set_kernel_variant.argtypes = [h, C.c_int32, C.c_int32]
+    L.lbm_set_fused_tile.argtypes = [h, C.c_int32, C.c_int32]
+    ip = C.POINTER(C.c_int32)
+    L.lbm_get_kernel_form.argtypes = [h, ip, ip, ip, ip]
     L.lbm_last_error.argtypes = [h]
     L.lbm_last_error.restype = C.c_char_p
     L.lbm_error_string.argtypes = [C.c_int]
@@ -210,6 +215,16 @@ class Solver:
     def set_kernel_variant(self, variant: int, kchunk: int = 0):
         """KERNEL_AUTO / KERNEL_TWO_PASS / KERNEL_ONE_PASS; kchunk: z-planes per block."""
         self._chk(self.lib.lbm_set_kernel_variant(self._h, int(variant), int(kchunk)))
+
+    def set_fused_tile(self, tile: int, prefetch: int = 0):
+        """TILE_32X16 / TILE_16X16 / TILE_32X8 thread tile of the one-pass form; prefetch 0/1."""
+        self._chk(self.lib.lbm_set_fused_tile(self._h, int(tile), int(prefetch)))
+
+    def kernel_form(self) -> dict:
+        """The kernel form in effect: variant, kchunk, tile, prefetch."""
+        v = [C.c_int32() for _ in range(4)]
+        self._chk(self.lib.lbm_get_kernel_form(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("variant", "kchunk", "tile", "prefetch"), (x.value for x in v)))
 
     def autotune(self, cache: str | None = None, steps: int = 10) -> float:
         """Pick the kernel form and z-chunk for this shape (lbm_autotune); returns ms/step."""

@@ -130,12 +130,26 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 #define LBM_KERNEL_ONE_PASS 2
 int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 
-/* Autotuner (no reference counterpart).  Picks the kernel form and the one-pass z-chunk
- * (nodes marched per thread) for this handle's shape by timing `steps` steps of each candidate
- * on a scratch domain of the same configuration, applies the winner and appends it to the text
- * cache `cache_path` (NULL: no cache).  A shape already in the cache is applied without timing.
- * The handle's own state is not advanced.  *ms_per_step (may be NULL) receives the winner's
- * time per step (0 when it came from the cache). */
+/* Thread tile of the one-pass form (no reference counterpart): 32x16 threads (one 512-thread
+ * block per CU), 16x16 or 32x8 (two independent 256-thread blocks per CU); prefetch = 1 issues
+ * the next plane's loads before the current flow collision.  Results do not depend on it.
+ * Batch domains reject this and lbm_set_kernel_variant (LBM_ERR_STATE): a batch is tuned with
+ * lbm_batch_autotune. */
+#define LBM_TILE_32X16 0
+#define LBM_TILE_16X16 1
+#define LBM_TILE_32X8 2
+int lbm_set_fused_tile(lbm_handle *h, int32_t tile, int32_t prefetch);
+/* The kernel form in effect: variant as requested (LBM_KERNEL_*), z-chunk, tile, prefetch. */
+int lbm_get_kernel_form(lbm_handle *h, int32_t *variant, int32_t *kchunk, int32_t *tile,
+                        int32_t *prefetch);
+
+/* Autotuner (no reference counterpart).  Picks the kernel form, the one-pass z-chunk (nodes
+ * marched per thread), tile shape and prefetch for this handle's shape by timing `steps` steps
+ * of each candidate on a scratch domain of the same configuration (with fewer z-planes when the
+ * device cannot hold a second copy), applies the winner and appends it to the text cache
+ * `cache_path` (NULL: no cache) as "key variant kchunk tile prefetch ms".  A shape already in
+ * the cache is applied without timing.  The handle's own state is not advanced.  *ms_per_step
+ * (may be NULL) receives the winner's time per step (0 when it came from the cache). */
 int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *ms_per_step);
 
 /* ---- Batched ensembles (no reference counterpart: the reference runs one cavity per
