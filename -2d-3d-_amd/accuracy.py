@@ -3,8 +3,8 @@
 
 Runs the same seeded cavity in the FP64 form and in each requested reduced-precision form,
 samples the diagnostics every `--every` steps, and reports for each form:
-  * time-averaged Nu_vol, Nu_global, Nu_hot, Re_vol over the samples and their relative error
-    against the FP64 run,
+  * time-averaged Nu_vol, Nu_global, Nu_hot, Re_vol over the samples (all samples, and the
+    samples from --avg-from on) and their relative error against the FP64 run,
   * relative L2 error of u and T against the FP64 run at the end,
   * throughput of each run (GLUPS over the whole run, diagnostics included).
 
@@ -62,8 +62,8 @@ def main():
     ap.add_argument("--mach", type=float, default=0.1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--every", type=int, default=500)
-    ap.add_argument("--avg-from", type=int, default=0,
-                    help="first step included in the time averages")
+    ap.add_argument("--avg-from", type=int, default=-1,
+                    help="first step of the second set of time averages (default: half the run)")
     ap.add_argument("--forms", default="fp32")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--out", default="")
@@ -72,6 +72,8 @@ def main():
                          "after averaging the fine run over blocks of (n/ref_n)^3 nodes")
     a = ap.parse_args()
 
+    if a.avg_from < 0:
+        a.avg_from = a.steps // 2
     ref_n = a.ref_n or a.n
     ratio = a.n // ref_n
     assert ref_n * ratio == a.n, "--n must be a multiple of --ref-n"
@@ -93,12 +95,13 @@ def main():
         return x.reshape(nz // ratio, ratio, ny // ratio, ratio, nx // ratio, ratio).mean(
             axis=(1, 3, 5))
 
-    def tavg(series, key):
-        v = [s[key] for s in series if s["itc"] >= a.avg_from]
+    def tavg(series, key, start=0):
+        v = [s[key] for s in series if s["itc"] >= start]
         return float(np.mean(v))
 
     for k in KEYS:
         result["fp64"][f"tavg_{k}"] = tavg(ref, k)
+        result["fp64"][f"tavg2_{k}"] = tavg(ref, k, a.avg_from)
     for form in a.forms.split(","):
         series, fields, glups = run(form, a.n, a.ra, a.steps, a.every, a.seed, True, a.mach)
         fields = {k: coarsen(v) for k, v in fields.items()}
@@ -107,9 +110,10 @@ def main():
             series = [s for s in series if s["itc"] in times]
         r = {"glups_incl_diag": glups, "series": series}
         for k in KEYS:
-            r[f"tavg_{k}"] = tavg(series, k)
-            r[f"tavg_{k}_relerr"] = abs(r[f"tavg_{k}"] - result["fp64"][f"tavg_{k}"]) / \
-                abs(result["fp64"][f"tavg_{k}"])
+            for tag, start in (("tavg", 0), ("tavg2", a.avg_from)):
+                r[f"{tag}_{k}"] = tavg(series, k, start)
+                r[f"{tag}_{k}_relerr"] = abs(r[f"{tag}_{k}"] - result["fp64"][f"{tag}_{k}"]) / \
+                    abs(result["fp64"][f"{tag}_{k}"])
         num = sum(np.sum((fields[k] - ref_fields[k]) ** 2) for k in ("u", "v", "w"))
         den = sum(np.sum(ref_fields[k] ** 2) for k in ("u", "v", "w"))
         r["l2_u_relerr"] = float(np.sqrt(num / den))

@@ -46,6 +46,12 @@ struct lbm_handle {
     bool in_batch;          // destroyed with its batch, not by lbm_destroy
     uint64_t instance;      // index within a batch (seeded-noise key)
     size_t dev_bytes;       // device memory held by the domain
+    // one-pass launch table (lbm_fused.h): entry 0 for the state (ftab_fcur, ftab_gcur,
+    // ftab_bp), entry 1 for the state one one-pass step later
+    void *ftab;
+    bool ftab_ok;
+    int ftab_fcur, ftab_gcur;
+    const double *ftab_bp;
     char err[256];
 };
 
@@ -111,6 +117,7 @@ static void release(lbm_handle *h)
     for (auto b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (h->flag) { (void)hipFree(h->flag); h->flag = nullptr; }
+    if (h->ftab) { (void)hipFree(h->ftab); h->ftab = nullptr; }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
@@ -133,6 +140,68 @@ static void set_fused_geom(lbm_handle *h, int kchunk, int tile, int prefetch)
     g.nty = (unsigned)((h->cfg.ny + ty - 3) / (ty - 2));
     g.nzc = (unsigned)((h->cfg.nz + g.kchunk - 1) / g.kchunk);
     g.nblocks = g.ntx * g.nty * g.nzc;
+    h->ftab_ok = false;
+}
+
+// Pointers of one one-pass step from the buffer state (fcur, gcur, Bp) advanced by `par` steps.
+static FusedPtrs fused_ptrs(const lbm_handle *h, int fcur, int gcur, double *bp, double *bp_alt,
+                            int par)
+{
+    const bool sw = par == 1 && h->cfg.use_g;
+    FusedPtrs p;
+    p.Fcur = h->F[fcur ^ par];
+    p.Fnext = h->F[fcur ^ par ^ 1];
+    p.Gin = h->G[gcur ^ par];
+    p.Gout = h->G[gcur ^ par ^ 1];
+    p.Bp = sw ? bp_alt : bp;
+    p.Bpout = sw ? bp : bp_alt;
+    return p;
+}
+
+static size_t fused_entry_bytes(const lbm_handle *h)
+{
+    return h->lp ? fused_instance_bytes_lp(h->cfg.precision) : fused_instance_bytes_f64();
+}
+
+static void fused_fill(const lbm_handle *h, void *dst, const FusedPtrs &p, unsigned first_block)
+{
+    if (h->lp) fused_instance_fill_lp(h->cfg.precision, dst, h->kp, h->fg, p, first_block);
+    else fused_instance_fill_f64(dst, h->kp, h->fg, p, first_block);
+}
+
+// One-pass step of a single domain: launch from the handle's two-entry table, rebuilding it
+// when the buffers are not in either state it describes.
+static int launch_fused_single(lbm_handle *h)
+{
+    const size_t isz = fused_entry_bytes(h);
+    int par = -1;
+    if (h->ftab_ok) {
+        // a one-pass step flips F and G, and swaps the two history buffers when use_g
+        const bool bp_same = h->Bp == h->ftab_bp;
+        if (h->fcur == h->ftab_fcur && h->gcur == h->ftab_gcur && bp_same) par = 0;
+        else if (h->fcur != h->ftab_fcur && h->gcur != h->ftab_gcur && bp_same != (bool)h->cfg.use_g)
+            par = 1;
+    }
+    if (par < 0) {
+        if (!h->ftab && hipMalloc(&h->ftab, 2 * isz) != hipSuccess)
+            return fail(h, LBM_ERR_NOMEM, "one-pass launch table");
+        std::vector<unsigned char> host(2 * isz);
+        for (int q = 0; q < 2; ++q)
+            fused_fill(h, host.data() + q * isz, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, q), 0);
+        // synchronous: the table is rebuilt only after an initialisation, a two-pass step or a
+        // change of the kernel form, never inside a run of one-pass steps
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, hipMemcpy(h->ftab, host.data(), 2 * isz, hipMemcpyHostToDevice));
+        h->ftab_ok = true;
+        h->ftab_fcur = h->fcur;
+        h->ftab_gcur = h->gcur;
+        h->ftab_bp = h->Bp;
+        par = 0;
+    }
+    const void *entry = (const unsigned char *)h->ftab + par * isz;
+    if (h->lp) HIPCHK(h, launch_fused_batch_lp(h->cfg.precision, h->fg, entry, nullptr, h->fg.nblocks, h->stream));
+    else HIPCHK(h, launch_fused_batch_f64(h->fg, entry, nullptr, h->fg.nblocks, h->stream));
+    return LBM_OK;
 }
 
 extern "C" int lbm_create(const lbm_config *cfg, lbm_handle **out)
@@ -207,7 +276,8 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
         {&h->Bp, N * 3 * ea},          {&h->aux[0], N * ea},
         {&h->Bp_alt, (h->fused_ok ? N * 3 : 1) * ea},
         {&h->aux[1], (c.dims == 2 ? N : 1) * ea},
-        {&h->mac, N * 5 * sizeof(double)}, {&h->wall, (size_t)2 * c.nx * sizeof(double)},
+        {&h->mac, N * 5 * sizeof(double)},
+        {&h->wall, (size_t)2 * (c.nx > c.ny ? c.nx : c.ny) * sizeof(double)},
     };
     for (auto &b : plan) {
         if (hipMalloc((void **)b.p, b.bytes) != hipSuccess) { rc = LBM_ERR_NOMEM; break; }
@@ -215,7 +285,8 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     }
     h->red_blocks = 1024;
     if (rc == LBM_OK) {
-        size_t nred = (size_t)2 * 8 * (h->red_blocks > (unsigned)c.nx ? h->red_blocks : c.nx);
+        const unsigned nw = (unsigned)(c.nx > c.ny ? c.nx : c.ny);   // wall-profile blocks
+        size_t nred = (size_t)2 * 8 * (h->red_blocks > nw ? h->red_blocks : nw);
         if (hipMalloc((void **)&h->red, nred * sizeof(double)) != hipSuccess ||
             hipMalloc((void **)&h->flag, sizeof(int)) != hipSuccess)
             rc = LBM_ERR_NOMEM;
@@ -315,9 +386,16 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
         double *Fn = h->F[h->fcur ^ 1];
         const double *Gi = h->G[h->gcur];
         double *Go = h->G[h->gcur ^ 1];
-        // the first step after (re)initialisation reads the plain state: two-pass form
-        const bool fused = h->fused_ok && !h->first_thermal && h->variant != LBM_KERNEL_TWO_PASS;
-        if (h->lp) {
+        // the first step after (re)initialisation reads the plain state: two-pass form.  AUTO
+        // runs the two-pass form, the faster one at 512^3 on MI355X in every precision form;
+        // lbm_autotune measures both per shape.
+        const bool fused = h->fused_ok && !h->first_thermal && h->variant == LBM_KERNEL_ONE_PASS;
+        if (fused) {
+            int rc = launch_fused_single(h);
+            if (rc) return rc;
+            h->aux_valid = 0;
+            if (h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
+        } else if (h->lp) {
             LPArgs a;
             a.Fc = Fc;
             a.Fn = Fn;
@@ -327,15 +405,8 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
             a.aux = (float *)h->aux[0];
             a.mac = h->mac;
             a.first = h->first_thermal;
-            a.fg = &h->fg;
-            a.Bp_out = (float *)h->Bp_alt;
-            HIPCHK(h, launch_lp(h->cfg.precision, fused ? LP_FUSED : LP_STEP, h->kp, h->bx, h->stream, a));
-            h->aux_valid = !fused;
-            if (fused && h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
-        } else if (d3 && fused) {
-            HIPCHK(h, launch_fused_f64(h->kp, h->fg, h->stream, Fc, Fn, Gi, Go, h->Bp, h->Bp_alt));
-            h->aux_valid = 0;
-            if (h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
+            HIPCHK(h, launch_lp(h->cfg.precision, LP_STEP, h->kp, h->bx, h->stream, a));
+            h->aux_valid = 1;
         } else if (d3) {
             HIPCHK(h, launch_step3(h->kp, h->bx, h->stream, h->first_thermal, Fc, Fn, Gi, Go, h->Bp,
                                    h->aux[0], h->mac));
@@ -415,8 +486,10 @@ extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
 static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefetch)
 {
     if (variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < 0 ||
-        tile < LBM_TILE_32X16 || tile > LBM_TILE_32X8 || prefetch < 0 || prefetch > 1)
+        tile < LBM_TILE_32X16 || tile > LBM_TILE_32X32 || prefetch < 0 || prefetch > 1)
         return LBM_ERR_ARG;
+    if (tile == LBM_TILE_32X32 && !h->lp)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the 32x32 tile needs a reduced-precision form");
     if (variant == LBM_KERNEL_ONE_PASS && !h->fused_ok)
         return fail(h, LBM_ERR_UNSUPPORTED, "one-pass form needs 3D and non-periodic thermal walls");
     h->variant = variant;
@@ -514,6 +587,7 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
     if (t->fused_ok)
         for (int tile = 0; tile < FUSED_TILES; ++tile)
             for (int pf = 0; pf < 2; ++pf)
+                if (tile != LBM_TILE_32X32 || h->lp)
                 for (int kc : {8, 16, 32, 64, 128})
                     if (kc <= sc.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc, tile, pf});
     double best = 1e300;
@@ -704,11 +778,14 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
     const lbm_config &c = h->cfg;
     const double L = h->p.L, dy = 1.0 / L, coef = L / h->p.diffusivity;
     const double dT = c.thot - c.tcold, N = (double)h->N;
-    const int nx = c.nx;
+    // Rayleigh-Benard cell: hot wall y = 0, wall profile along x.  Side-heated cell: hot wall
+    // x = 0, wall profile along y (SideHeatedcalc_Nu_wall_avg, 2DRB/3DRB.F90:1889-2038).
+    const int side = c.side_heated ? 1 : 0;
+    const int nx = side ? c.ny : c.nx;   // length of the hot-wall profile
     const unsigned nb = h->red_blocks;
     memset(d, 0, sizeof(*d));
 
-    HIPCHK(h, launch_reduce_volume(h->kp, h->stream, h->mac, coef, dy, h->red, nb));
+    HIPCHK(h, launch_reduce_volume(h->kp, h->stream, h->mac, coef, dy, side, h->red, nb));
     std::vector<double> part((size_t)nb * 6 * 2);
     HIPCHK(h, hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double),
                              hipMemcpyDeviceToHost, h->stream));
@@ -722,7 +799,8 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
     d->heat = s[4];
     d->kinetic = s[5];
 
-    HIPCHK(h, launch_reduce_walls(h->kp, h->stream, h->mac, coef, dy, h->wall, h->wall + nx, h->red));
+    HIPCHK(h, launch_reduce_walls(h->kp, h->stream, h->mac, coef, dy, side, h->wall, h->wall + nx,
+                                  h->red));
     std::vector<double> wpart((size_t)nx * 4 * 2), prof((size_t)2 * nx);
     HIPCHK(h, hipMemcpyAsync(wpart.data(), h->red, wpart.size() * sizeof(double),
                              hipMemcpyDeviceToHost, h->stream));
@@ -741,7 +819,8 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
     d->nu_cold = cold.val() / ((double)nx * c.nz);
     d->nu_middle = mid.val() / ((double)nx * c.nz);
 
-    // Corner extension and 5-point fits of the hot-wall profile (2DRB/3DRB.F90:2089-2150).
+    // Corner extension and 5-point fits of the hot-wall profile (2DRB/3DRB.F90:2089-2150;
+    // side-heated cell :1929-1989, along y).
     std::vector<double> xp(nx + 2), ext(nx + 2);
     xp[0] = 0.0;
     xp[nx + 1] = (double)nx / L;
@@ -757,7 +836,8 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
     }
     for (int pass = 0; pass < 2; ++pass) {
         const int ic = pass == 0 ? imax : imin;
-        const int i0 = ic <= 2 ? 0 : (ic >= nx - 1 ? nx - 3 : ic - 2);
+        int i0 = ic <= 2 ? 0 : (ic >= nx - 1 ? nx - 3 : ic - 2);
+        if (side && pass == 1) i0 = ic >= 4 ? ic - 4 : 0;   // the side-heated minimum's window
         double fs, ys;
         fit_parabola5(&xp[i0], &ext[i0], pass == 0 ? 1 : -1, &fs, &ys);
         if (pass == 0) { d->nu_hot_max = fs; d->nu_hot_max_pos = ys; }
@@ -1068,8 +1148,7 @@ extern "C" int lbm_batch_set_mode(lbm_batch *b, int32_t mode)
 static int batch_tables(lbm_batch *b)
 {
     lbm_handle *h0 = b->dom[0];
-    const bool lp = h0->lp;
-    const size_t isz = lp ? fused_instance_bytes_lp(b->precision) : fused_instance_bytes_f64();
+    const size_t isz = fused_entry_bytes(h0);
     unsigned total = 0;
     for (int i = 0; i < b->n; ++i) total += b->dom[i]->fg.nblocks;
     if (!b->owner) {
@@ -1089,17 +1168,8 @@ static int batch_tables(lbm_batch *b)
         unsigned first = 0;
         for (int i = 0; i < b->n; ++i) {
             lbm_handle *h = b->dom[i];
-            const int fc = h->fcur ^ par, gc = h->gcur ^ par;
-            const bool sw = par == 1 && h->cfg.use_g;
-            FusedPtrs p;
-            p.Fcur = h->F[fc];
-            p.Fnext = h->F[fc ^ 1];
-            p.Gin = h->G[gc];
-            p.Gout = h->G[gc ^ 1];
-            p.Bp = sw ? h->Bp_alt : h->Bp;
-            p.Bpout = sw ? h->Bp : h->Bp_alt;
-            if (lp) fused_instance_fill_lp(b->precision, host.data() + isz * i, h->kp, h->fg, p, first);
-            else fused_instance_fill_f64(host.data() + isz * i, h->kp, h->fg, p, first);
+            fused_fill(h, host.data() + isz * i, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, par),
+                       first);
             first += h->fg.nblocks;
         }
         HIPCHK(h0, hipMemcpy(b->table[par], host.data(), host.size(), hipMemcpyHostToDevice));

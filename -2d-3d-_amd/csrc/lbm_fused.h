@@ -163,22 +163,11 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     }
 }
 
-// At least two waves per SIMD: with 256 registers per lane at most, a 512-thread tile fits one
-// CU and 256-thread tiles two per CU.
-template <class Form, int TX, int TY, bool PF>
-__global__ void __launch_bounds__(TX *TY, 2)
-k3_fused(KParams kp, typename Form::P fp, FusedGeom fg, const typename Form::S *__restrict__ Fcur,
-         typename Form::S *__restrict__ Fnext, const typename Form::S *__restrict__ Gin,
-         typename Form::S *__restrict__ Gout, const typename Form::B *__restrict__ Bp,
-         typename Form::B *__restrict__ Bpout)
-{
-    fused_tile<Form, TX, TY, PF>(kp, fp, fg, xcd_remap(blockIdx.x, fg.nblocks), Fcur, Fnext, Gin,
-                                 Gout, Bp, Bpout);
-}
-
-// Batched form: many independent domains advance in one launch.  Block b (after the XCD remap)
-// looks up its domain in `owner` and its domain's parameters and buffers in `inst`; the first
-// block of domain d is inst[d].first_block.
+// The launch's domains and their parameters come from a table in device memory: block b (after
+// the XCD remap) belongs to domain owner[b] (domain 0 when owner is null), whose first block is
+// inst[d].first_block.  Reading the parameters through a pointer, rather than as kernel
+// arguments, lets the compiler reload them with scalar loads instead of holding them in vector
+// registers (FP64: about 200 instead of 256 registers, no spills).
 template <class Form> struct FusedInstance {
     KParams kp;
     typename Form::P fp;
@@ -192,49 +181,31 @@ template <class Form> struct FusedInstance {
     unsigned first_block;
 };
 
+// At least two waves per SIMD: with 256 registers per lane at most, a 512-thread tile fits one
+// CU and 256-thread tiles two per CU.
 template <class Form, int TX, int TY, bool PF>
 __global__ void __launch_bounds__(TX *TY, 2)
-k3_fused_batch(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restrict__ owner,
-               unsigned nblocks)
+k3_fused(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restrict__ owner,
+         unsigned nblocks)
 {
     const unsigned b = xcd_remap(blockIdx.x, nblocks);
-    const FusedInstance<Form> &d = inst[owner[b]];
+    const FusedInstance<Form> &d = inst[owner ? owner[b] : 0];
     fused_tile<Form, TX, TY, PF>(d.kp, d.fp, d.fg, b - d.first_block, d.Fcur, d.Fnext, d.Gin,
                                  d.Gout, d.Bp, d.Bpout);
 }
 
 // Run-time tile shapes: index -> (TX, TY).
-constexpr int fused_tx(int tile) { return tile == 0 ? 32 : (tile == 1 ? 16 : 32); }
-constexpr int fused_ty(int tile) { return tile == 0 ? 16 : (tile == 1 ? 16 : 8); }
+constexpr int fused_tx(int tile) { return tile == 1 ? 16 : 32; }
+constexpr int fused_ty(int tile) { return tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : 32)); }
 
+// One launch over `nblocks` blocks of the domains in table `inst` (owner null: one domain).
 template <class Form>
-hipError_t launch_fused_form(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg,
-                             hipStream_t s, const typename Form::S *Fcur, typename Form::S *Fnext,
-                             const typename Form::S *Gin, typename Form::S *Gout,
-                             const typename Form::B *Bp, typename Form::B *Bpout)
-{
-#define RBLBM_FUSED(T, PF)                                                                       \
-    k3_fused<Form, fused_tx(T), fused_ty(T), PF><<<fg.nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
-        kp, fp, fg, Fcur, Fnext, Gin, Gout, Bp, Bpout)
-    switch (fg.tile * 2 + (fg.pf ? 1 : 0)) {
-    case 0: RBLBM_FUSED(0, false); break;
-    case 1: RBLBM_FUSED(0, true); break;
-    case 2: RBLBM_FUSED(1, false); break;
-    case 3: RBLBM_FUSED(1, true); break;
-    case 4: RBLBM_FUSED(2, false); break;
-    default: RBLBM_FUSED(2, true); break;
-    }
-#undef RBLBM_FUSED
-    return hipGetLastError();
-}
-
-template <class Form>
-hipError_t launch_fused_batch_form(const FusedGeom &fg0, const void *inst, const uint16_t *owner,
-                                   unsigned nblocks, hipStream_t s)
+hipError_t launch_fused_table(const FusedGeom &fg0, const void *inst, const uint16_t *owner,
+                              unsigned nblocks, hipStream_t s)
 {
     const FusedInstance<Form> *d = (const FusedInstance<Form> *)inst;
 #define RBLBM_FUSED_B(T, PF)                                                                     \
-    k3_fused_batch<Form, fused_tx(T), fused_ty(T), PF><<<nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
+    k3_fused<Form, fused_tx(T), fused_ty(T), PF><<<nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
         d, owner, nblocks)
     switch (fg0.tile * 2 + (fg0.pf ? 1 : 0)) {
     case 0: RBLBM_FUSED_B(0, false); break;
@@ -242,7 +213,16 @@ hipError_t launch_fused_batch_form(const FusedGeom &fg0, const void *inst, const
     case 2: RBLBM_FUSED_B(1, false); break;
     case 3: RBLBM_FUSED_B(1, true); break;
     case 4: RBLBM_FUSED_B(2, false); break;
-    default: RBLBM_FUSED_B(2, true); break;
+    case 5: RBLBM_FUSED_B(2, true); break;
+    default:
+        // 32 x 32 = 1024 threads leave 128 registers per lane: reduced-precision forms only
+        if constexpr (sizeof(typename Form::R) == 4) {
+            if (fg0.pf) RBLBM_FUSED_B(3, true);
+            else RBLBM_FUSED_B(3, false);
+        } else {
+            return hipErrorInvalidValue;
+        }
+        break;
     }
 #undef RBLBM_FUSED_B
     return hipGetLastError();

@@ -31,27 +31,27 @@ hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, 
                         double *mac, double *fo, double *go, int k0, int nk);
 
 // One-pass form: tile of (TX - 2) x (TY - 2) nodes plus a one-node ring per block; the thread
-// tile TX x TY is one of three shapes (FusedGeom::tile).
-constexpr int FUSED_TILES = 3;
+// tile TX x TY is one of four shapes (FusedGeom::tile): 32x16, 16x16, 32x8, and 32x32 for the
+// reduced-precision forms only.
+constexpr int FUSED_TILES = 4;
 inline void fused_tile_dims(int tile, int *tx, int *ty)
 {
     *tx = tile == 1 ? 16 : 32;
-    *ty = tile == 0 ? 16 : (tile == 1 ? 16 : 8);
+    *ty = tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : 32));
 }
-hipError_t launch_fused_f64(const KParams &kp, const FusedGeom &fg, hipStream_t s, const double *Fcur,
-                            double *Fnext, const double *Gin, double *Gout, const double *Bp,
-                            double *Bp_out);
 hipError_t launch_init(const KParams &kp, int bx, hipStream_t s, const InitParams &ip, double *F,
                        double *G, double *Bp, double *mac, int qf, int qt);
 hipError_t launch_transpose(bool to_soa, hipStream_t s, const double *in, double *out, int q,
                             uint64_t N, uint64_t c0, uint64_t n);
 hipError_t launch_reconstruct(const KParams &kp, hipStream_t s, const double *F, const double *G,
                               double *Bp, double *mac, int qf, int qt, int *bad);
+// side = 1 selects the side-heated cell's formulas (hot wall x = 0; wall profile along y).
 hipError_t launch_reduce_volume(const KParams &kp, hipStream_t s, const double *mac,
-                                double L_over_kappa, double dy, double *out, unsigned blocks);
+                                double L_over_kappa, double dy, int side, double *out,
+                                unsigned blocks);
 hipError_t launch_reduce_walls(const KParams &kp, hipStream_t s, const double *mac,
-                               double L_over_kappa, double dy, double *tbot, double *nubot,
-                               double *out);
+                               double L_over_kappa, double dy, int side, double *tbot,
+                               double *nubot, double *out);
 hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *mac, double *prev,
                                double *out, unsigned blocks);
 
@@ -66,7 +66,6 @@ enum {
     LP_BPREV,         // Bp (3 float planes) -> fo (FP64), n values
     LP_RECON_AOS,     // reconstruct_macro_from_fg on the AoS chunk fo/go -> mac, *flag on rho <= 0
     LP_BP_FROM_MAC,   // Bp = u T from mac (zero without use_g)
-    LP_FUSED,         // one-pass step: Fc, Gi -> Fn, Go; Bp updated (aux not written)
 };
 struct LPArgs {
     const void *Fc = nullptr;
@@ -79,14 +78,12 @@ struct LPArgs {
     uint64_t c0 = 0, n = 0;
     int *flag = nullptr;
     const void *Gprev = nullptr;   // LP_STATE: g_post^{n-1} when aux is not valid
-    const FusedGeom *fg = nullptr; // LP_FUSED
-    float *Bp_out = nullptr;       // LP_FUSED: updated history
 };
 hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a);
 
-// Batched one-pass launch (many domains per launch).  The per-domain table entry is
-// form-specific; these helpers size and fill it on the host, then launch over `nblocks` blocks
-// whose owner domain is `owner[block]`.
+// One-pass launches read their domains from a table in device memory (lbm_fused.h).  The
+// per-domain entry is form-specific; these helpers size and fill it on the host, then launch
+// over `nblocks` blocks whose domain is `owner[block]` (owner null: the table's one domain).
 struct FusedPtrs {
     const void *Fcur;
     void *Fnext;

@@ -606,8 +606,10 @@ __device__ __forceinline__ void block_reduce_store(KS (&v)[NQ], double *out)
 // Volume sums over the macro planes: calNuRe, RBcalc_Nu_global and the conservation sums.
 //   0: sum v T   1: sum |u|^2   2: sum qy (global Nu integrand)   3: sum rho   4: sum T
 //   5: sum 0.5 rho |u|^2
+// side = 1: the side-heated cell's formulas (hot wall x = 0): Nu_vol from u T
+// (2DRB/3DRB.F90:1732-1741) and SideHeatedcalc_Nu_global (2DRB/3DRB.F90:1789-1828).
 __global__ void __launch_bounds__(256) k_reduce_volume(KParams kp, const double *__restrict__ mac,
-                                                       double L_over_kappa, double dy,
+                                                       double L_over_kappa, double dy, int side,
                                                        double *__restrict__ out)
 {
     const uint64_t N = kp.N;
@@ -617,15 +619,25 @@ __global__ void __launch_bounds__(256) k_reduce_volume(KParams kp, const double 
          c += (uint64_t)gridDim.x * blockDim.x) {
         const double rho = mac[c], u = mac[N + c], v = mac[2 * N + c], w = mac[3 * N + c];
         const double T = mac[4 * N + c];
-        const int j = (int)((c / (uint64_t)nx) % (uint64_t)ny);
-        ks_add(acc[0], v * T);
+        const double *Tc = mac + 4 * N + c;
+        ks_add(acc[0], (side ? u : v) * T);
         const double q2 = u * u + v * v + w * w;
         ks_add(acc[1], q2);
-        double dTdy;
-        if (j == 0) dTdy = (3.0 * T + mac[4 * N + c + nx] - 4.0 * kp.thot) / (3.0 * dy);
-        else if (j == ny - 1) dTdy = (4.0 * kp.tcold - 3.0 * T - mac[4 * N + c - nx]) / (3.0 * dy);
-        else dTdy = (mac[4 * N + c + nx] - mac[4 * N + c - nx]) / (2.0 * dy);
-        ks_add(acc[2], L_over_kappa * v * (T - kp.tref) - dTdy);
+        if (side) {
+            const int i = (int)(c % (uint64_t)nx);
+            double dT;
+            if (i == 0) dT = (-3.0 * T - Tc[1] + 4.0 * kp.thot) / (3.0 * dy);
+            else if (i == nx - 1) dT = (-4.0 * kp.tcold + 3.0 * T + Tc[-1]) / (3.0 * dy);
+            else dT = (Tc[-1] - Tc[1]) / (2.0 * dy);
+            ks_add(acc[2], L_over_kappa * u * (T - kp.tref) + dT);
+        } else {
+            const int j = (int)((c / (uint64_t)nx) % (uint64_t)ny);
+            double dTdy;
+            if (j == 0) dTdy = (3.0 * T + Tc[nx] - 4.0 * kp.thot) / (3.0 * dy);
+            else if (j == ny - 1) dTdy = (4.0 * kp.tcold - 3.0 * T - Tc[-nx]) / (3.0 * dy);
+            else dTdy = (Tc[nx] - Tc[-nx]) / (2.0 * dy);
+            ks_add(acc[2], L_over_kappa * v * (T - kp.tref) - dTdy);
+        }
         ks_add(acc[3], rho);
         ks_add(acc[4], T);
         ks_add(acc[5], 0.5 * rho * q2);
@@ -636,8 +648,10 @@ __global__ void __launch_bounds__(256) k_reduce_volume(KParams kp, const double 
 // Wall sums of RBcalc_Nu_wall_avg: one block per x index i, reducing over z.
 //   per-i outputs: T(i,1,:) average and hot-wall Nu average (plain arrays, length nx)
 //   per-block pairs: 0 sum cold-wall Nu, 1 sum mid-plane Nu
+// side = 1: SideHeatedcalc_Nu_wall_avg (2DRB/3DRB.F90:1889-2038), one block per y index j:
+// hot wall x = 0, cold wall x = Lx, mid-plane x = Lx / 2 with u.
 __global__ void __launch_bounds__(256) k_reduce_walls(KParams kp, const double *__restrict__ mac,
-                                                      double L_over_kappa, double dy,
+                                                      double L_over_kappa, double dy, int side,
                                                       double *__restrict__ tbot,
                                                       double *__restrict__ nubot,
                                                       double *__restrict__ out)
@@ -645,10 +659,31 @@ __global__ void __launch_bounds__(256) k_reduce_walls(KParams kp, const double *
     const int i = blockIdx.x;
     const uint64_t N = kp.N;
     const int nx = kp.nx, ny = kp.ny, nz = kp.nz;
-    const double *T = mac + 4 * N, *v = mac + 2 * N;
+    const double *T = mac + 4 * N, *v = mac + 2 * N, *u = mac + N;
     const double dT = kp.thot - kp.tcold;
     KS acc[4] = {};
-    for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+    if (side) {
+        const int j = blockIdx.x;
+        const double dx = dy;
+        for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+            const uint64_t b = (uint64_t)nx * ((uint64_t)j + (uint64_t)ny * k);  // (0, j, k)
+            ks_add(acc[0], T[b]);
+            ks_add(acc[1], ((8.0 * kp.thot - 9.0 * T[b] + T[b + 1]) / (3.0 * dx)) / dT);
+            const uint64_t t = b + nx - 1;
+            ks_add(acc[2], ((-8.0 * kp.tcold + 9.0 * T[t] - T[t - 1]) / (3.0 * dx)) / dT);
+            double mid;
+            if (nx % 2 == 1) {
+                const uint64_t m = b + (nx + 1) / 2 - 1;
+                mid = (L_over_kappa * u[m] * (T[m] - kp.tref) + (T[m - 1] - T[m + 1]) / (2.0 * dx)) / dT;
+            } else {
+                const uint64_t ml = b + nx / 2 - 1, mr = ml + 1;
+                mid = (L_over_kappa * 0.5 * (u[ml] * (T[ml] - kp.tref) + u[mr] * (T[mr] - kp.tref)) +
+                       (T[ml] - T[mr]) / dx) / dT;
+            }
+            ks_add(acc[3], mid);
+        }
+    }
+    for (int k = side ? nz : (int)threadIdx.x; k < nz; k += blockDim.x) {
         const uint64_t b = (uint64_t)i + (uint64_t)nx * (uint64_t)ny * k;  // (i, 0, k)
         ks_add(acc[0], T[b]);
         ks_add(acc[1], ((8.0 * kp.thot - 9.0 * T[b] + T[b + nx]) / (3.0 * dy)) / dT);
@@ -772,14 +807,6 @@ struct FormF64 {
 
 namespace rblbm {
 
-hipError_t launch_fused_f64(const KParams &kp, const FusedGeom &fg, hipStream_t s, const double *Fcur,
-                            double *Fnext, const double *Gin, double *Gout, const double *Bp,
-                            double *Bp_out)
-{
-    const FormF64::P p{kp.thot, kp.tcold};
-    return launch_fused_form<FormF64>(kp, p, fg, s, Fcur, Fnext, Gin, Gout, Bp, Bp_out);
-}
-
 size_t fused_instance_bytes_f64() { return sizeof(FusedInstance<FormF64>); }
 
 void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, const FusedPtrs &p,
@@ -802,7 +829,7 @@ void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, 
 hipError_t launch_fused_batch_f64(const FusedGeom &fg0, const void *inst, const uint16_t *owner,
                                   unsigned nblocks, hipStream_t s)
 {
-    return launch_fused_batch_form<FormF64>(fg0, inst, owner, nblocks, s);
+    return launch_fused_table<FormF64>(fg0, inst, owner, nblocks, s);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -873,17 +900,19 @@ hipError_t launch_reconstruct(const KParams &kp, hipStream_t s, const double *F,
 }
 
 hipError_t launch_reduce_volume(const KParams &kp, hipStream_t s, const double *mac,
-                                double L_over_kappa, double dy, double *out, unsigned blocks)
+                                double L_over_kappa, double dy, int side, double *out,
+                                unsigned blocks)
 {
-    k_reduce_volume<<<blocks, 256, 0, s>>>(kp, mac, L_over_kappa, dy, out);
+    k_reduce_volume<<<blocks, 256, 0, s>>>(kp, mac, L_over_kappa, dy, side, out);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_walls(const KParams &kp, hipStream_t s, const double *mac,
-                               double L_over_kappa, double dy, double *tbot, double *nubot,
-                               double *out)
+                               double L_over_kappa, double dy, int side, double *tbot,
+                               double *nubot, double *out)
 {
-    k_reduce_walls<<<kp.nx, 256, 0, s>>>(kp, mac, L_over_kappa, dy, tbot, nubot, out);
+    k_reduce_walls<<<side ? kp.ny : kp.nx, 256, 0, s>>>(kp, mac, L_over_kappa, dy, side, tbot,
+                                                       nubot, out);
     return hipGetLastError();
 }
 

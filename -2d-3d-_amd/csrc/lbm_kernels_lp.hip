@@ -645,9 +645,6 @@ static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, 
         lp3_state<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux, (const S *)a.Gprev,
                                          a.mac, a.fo, a.go, a.k0, a.nk);
         break;
-    case LP_FUSED:
-        return launch_fused_form<FormLP<S>>(kp, lp, *a.fg, s, Fc, (S *)a.Fn, (const S *)a.Gi,
-                                            (S *)a.Go, a.Bp, a.Bp_out);
     case LP_INIT:
         lp_init<S><<<grid, bx, 0, s>>>(kp, lp, (S *)a.Fn, (S *)a.Go, a.Bp, a.mac);
         break;
@@ -716,8 +713,8 @@ void fused_instance_fill_lp(int precision, void *dst, const KParams &kp, const F
 hipError_t launch_fused_batch_lp(int precision, const FusedGeom &fg0, const void *inst,
                                  const uint16_t *owner, unsigned nblocks, hipStream_t s)
 {
-    if (precision == 1) return launch_fused_batch_form<FormLP<float>>(fg0, inst, owner, nblocks, s);
-    return launch_fused_batch_form<FormLP<_Float16>>(fg0, inst, owner, nblocks, s);
+    if (precision == 1) return launch_fused_table<FormLP<float>>(fg0, inst, owner, nblocks, s);
+    return launch_fused_table<FormLP<_Float16>>(fg0, inst, owner, nblocks, s);
 }
 
 hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a)

@@ -22,7 +22,7 @@ VEL_NOSLIP, VEL_PERIODIC = 0, 1
 T_CONST, T_ADIABATIC, T_PERIODIC = 0, 1, 2
 FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_ONE_PASS = 0, 1, 2
-TILE_32X16, TILE_16X16, TILE_32X8 = 0, 1, 2
+TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32 = 0, 1, 2, 3
 
 EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
            "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",
@@ -217,7 +217,8 @@ class Solver:
         self._chk(self.lib.lbm_set_kernel_variant(self._h, int(variant), int(kchunk)))
 
     def set_fused_tile(self, tile: int, prefetch: int = 0):
-        """TILE_32X16 / TILE_16X16 / TILE_32X8 thread tile of the one-pass form; prefetch 0/1."""
+        """TILE_32X16 / TILE_16X16 / TILE_32X8 / TILE_32X32 (FP32, FP16 only) thread tile of the
+        one-pass form; prefetch 0/1."""
         self._chk(self.lib.lbm_set_fused_tile(self._h, int(tile), int(prefetch)))
 
     def kernel_form(self) -> dict:

@@ -7,11 +7,15 @@ noise +-1e-3 dT), rho=1, u=0.  A "step" is one full time step of both lattices o
 domain; inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line:
-  value      coupled GLUPS = nodes * steps / seconds / 1e9 (device time of the timed steps)
+  value      coupled GLUPS = nodes * steps / seconds / 1e9 (device time of the timed steps,
+             HIP events on the solver's stream)
   roofline   algorithmic bytes per node update (464 B in FP64: both lattices read and written
-             once plus the M1G history) * nodes / step time, against the 8 TB/s HBM3E peak;
-             traffic = measured HBM bytes per step from rocprofv3 FETCH_SIZE/WRITE_SIZE
-             (profiles/traffic_fp64_512.json), or null when that file is absent
+             once plus the M1G history) * nodes / step time, against the 8 TB/s HBM3E peak.
+             The unit is one step: the two launches of the two-pass form (k3_thermal, k3_flow)
+             or the one launch of the one-pass form (k3_fused).  traffic = measured HBM bytes
+             per step from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy kernel's readings
+             (profiles/traffic_fp64_512.json), or null when that file has no entry for the form;
+             copy_gbs = a device copy measured in the same run (the practical ceiling)
   cpu_baseline  the FP64 OpenMP oracle (a restatement of the reference's CPU path) timed on
              this host over a bounded sample of the same workload
 """
@@ -43,7 +47,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-nz", type=int, default=64)
     ap.add_argument("--variant", choices=["auto", "two-pass", "one-pass"], default="auto",
-                    help="kernel form of the step (auto: one-pass where it applies)")
+                    help="kernel form of the step (auto: the two-pass form)")
+    ap.add_argument("--tile", type=int, default=0, help="one-pass thread tile (0: 32x16, 1: 16x16, 2: 32x8)")
+    ap.add_argument("--prefetch", type=int, default=0, help="one-pass prefetching (0/1)")
     ap.add_argument("--kchunk", type=int, default=0, help="z-planes per one-pass block (0: default)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run a device-to-device copy of GIB GiB (counter calibration)")
@@ -93,12 +99,34 @@ def cpu_baseline(size, nz, ra, steps):
                       f"oracle, {steps} timed steps after 1 warm-up step ({dt:.1f} s)"}
 
 
-def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0):
+def copy_gbs(torch, gib=2.0, reps=5):
+    """Bandwidth of a device-to-device copy (read + write) of `gib` GiB: the practical ceiling."""
+    if torch is None:
+        return None
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms / 1e3) / 1e9
+
+
+def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=0, prefetch=0):
     """Time `steps` steps of one storage form on the n^3 cavity (inputs resident in HBM)."""
     cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=ra, prandtl=0.71, mach=0.1,
                        precision=prec)
     sol = rblbm.Solver(cfg)
     sol.set_kernel_variant(variant, kchunk)
+    sol.set_fused_tile(tile, prefetch)
+    form = sol.kernel_form()
     sol.initial_seeded(SEED, 1e-3)
     mass0 = sol.diagnostics()["mass"]
     sol.step(warmup)
@@ -115,7 +143,10 @@ def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0):
     bpu = sol.bytes_per_update()
     diag = sol.diagnostics()
     sol.close()
-    return {"glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms_step,
+    one = form["variant"] == rblbm.KERNEL_ONE_PASS
+    form["kernels"] = "k3_fused (one-pass step)" if one else "k3_thermal + k3_flow (two-pass step)"
+    form["name"] = "one-pass" if one else "two-pass"
+    return {"glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms_step, "form": form,
             "bytes_per_update": bpu, "achieved_gbs": bpu * nodes / (ms_step / 1e3) / 1e9,
             "roofline_frac": bpu * nodes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
             "wall_seconds": t_wall, "nu_vol": diag["nu_vol"], "diag": diag, "mass0": mass0,
@@ -148,7 +179,7 @@ def main():
         variant = {"auto": rblbm.KERNEL_AUTO, "two-pass": rblbm.KERNEL_TWO_PASS,
                    "one-pass": rblbm.KERNEL_ONE_PASS}[a.variant]
         forms[name] = measure(rblbm, torch if have_torch else None, n, a.rayleigh, prec,
-                              a.steps, a.warmup, variant, a.kchunk)
+                              a.steps, a.warmup, variant, a.kchunk, a.tile, a.prefetch)
     main_form = forms[a.forms.split(",")[0]]
     ms_step = main_form["ms_per_step"]
     glups = main_form["glups"]
@@ -162,11 +193,15 @@ def main():
     for f in forms.values():
         f.pop("diag", None)
 
+    main_name = a.forms.split(",")[0]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_fp64_512.json")
-    if os.path.exists(tpath) and n == 512:
+    if os.path.exists(tpath) and n == 512 and main_name == "fp64":
         with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_step")
+            entry = json.load(fh).get(main_form["form"]["name"])
+        if entry:
+            traffic = entry["hbm_bytes_per_step"]
+    ceiling = copy_gbs(torch if have_torch else None)
 
     cpu = None
     if not a.no_cpu:
@@ -178,15 +213,17 @@ def main():
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16/f32"}[a.forms.split(",")[0]],
         "data": "synthetic",
-        "config": {"workload": f"3D RB cavity {n}^3 Ra={a.rayleigh:g} Pr=0.71 Ma=0.1, FP64 "
-                               "populations, seeded T noise 1e-3",
-                   "nodes": nodes, "precision": "fp64", "wall_seconds": t_wall},
+        "config": {"workload": f"3D RB cavity {n}^3 Ra={a.rayleigh:g} Pr=0.71 Ma=0.1, "
+                               f"{main_name} storage form, seeded T noise 1e-3",
+                   "nodes": nodes, "precision": main_name, "kernel_form": main_form["form"],
+                   "wall_seconds": t_wall},
         "forms": forms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_update": bpu,
-                     "kernel": ("k3_thermal + k3_flow (two-pass step)" if a.variant == "two-pass"
-                                else "k3_fused (one-pass step)")},
+                     "traffic_unit": "bytes per step", "bytes_per_update": bpu,
+                     "algorithmic_bytes_per_step": bpu * nodes,
+                     "kernel": main_form["form"]["kernels"], "copy_gbs": ceiling,
+                     "frac_of_copy": achieved / ceiling if ceiling else None},
         "cpu_baseline": cpu,
         "checks": {"nu_vol": diag["nu_vol"], "re_vol": diag["re_vol"],
                    "mass_rel_change": (diag["mass"] - mass0) / mass0},

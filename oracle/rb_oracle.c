@@ -1109,17 +1109,146 @@ void orc_fit_parabola_ls5(const double *y, const double *f, int mode, double *fs
     *fstar = A + B * ys + C * ys * ys;
 }
 
+/* Side-heated cell (hot wall x = 0, cold wall x = L, L = nx): the SideHeatedCell branch of
+ * calNuRe (3DRB.F90:1732-1741, 2DRB.F90:1631-1640; Nu_vol from u T), SideHeatedcalc_Nu_global
+ * (3DRB.F90:1789-1828, 2DRB.F90:1686-1734) and SideHeatedcalc_Nu_wall_avg (3DRB.F90:1889-2038,
+ * 2DRB.F90:1746-1941): the hot-wall profile runs along y (averaged over z), its corner values
+ * come from adiabatic-wall fits at y = 0 and y = Ly, and the minimum's 5-point window is
+ * jmin-4 .. jmin (0 .. 4 when jmin < 4). */
+static void diagnostics_side_heated(const orc_state *s, orc_diag *d)
+{
+    const orc_params *p = &s->p;
+    const double Th = s->c.thot, Tc = s->c.tcold, Tref = p->Tref, L = p->L;
+    const double dx = 1.0 / L, deltaT = Th - Tc, coef = L / p->diffusivity;
+    const double N = (double)s->n;
+    const int nx = NX, ny = NY, nz = NZ;
+
+    ksum snu = {0, 0}, sre = {0, 0}, sqx = {0, 0}, sm = {0, 0}, sh = {0, 0}, ske = {0, 0};
+    for (int k = 0; k < nz; ++k) {
+        ksum a = {0, 0}, b = {0, 0}, c = {0, 0}, m = {0, 0}, h = {0, 0}, e = {0, 0};
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                long n = IDX(s, i, j, k);
+                double u = s->u[n], v = s->v[n], w = s->w[n], T = s->T[n];
+                ks_add(&a, u * T);
+                double q2 = u * u + v * v;
+                if (s->c.dims == 3) q2 = q2 + w * w;
+                ks_add(&b, q2);
+                double dT;
+                if (i == 0) dT = (-3.0 * T - s->T[IDX(s, 1, j, k)] + 4.0 * Th) / (3.0 * dx);
+                else if (i == nx - 1)
+                    dT = (-4.0 * Tc + 3.0 * T + s->T[IDX(s, nx - 2, j, k)]) / (3.0 * dx);
+                else dT = (s->T[IDX(s, i - 1, j, k)] - s->T[IDX(s, i + 1, j, k)]) / (2.0 * dx);
+                ks_add(&c, coef * u * (T - Tref) + dT);
+                ks_add(&m, s->rho[n]);
+                ks_add(&h, T);
+                ks_add(&e, 0.5 * s->rho[n] * q2);
+            }
+        ks_merge(&snu, a);
+        ks_merge(&sre, b);
+        ks_merge(&sqx, c);
+        ks_merge(&sm, m);
+        ks_merge(&sh, h);
+        ks_merge(&ske, e);
+    }
+    d->nu_vol = ks_val(snu) / N * L / p->diffusivity + 1.0;
+    d->re_vol = sqrt(ks_val(sre) / N) * L / p->viscosity;
+    d->nu_global = (ks_val(sqx) / N) / deltaT;
+    d->mass = ks_val(sm);
+    d->heat = ks_val(sh);
+    d->kinetic = ks_val(ske);
+
+    /* Hot-wall profile along y: per-j averages over z of T(1,j,k) and of the wall flux. */
+    double *Nu_ext = (double *)calloc((size_t)ny + 2, sizeof(double));
+    double *Tleft = (double *)calloc((size_t)ny + 2, sizeof(double));
+    ksum shot = {0, 0}, scold = {0, 0}, smid = {0, 0};
+    for (int j = 0; j < ny; ++j) {
+        ksum tl = {0, 0}, nl = {0, 0};
+        for (int k = 0; k < nz; ++k) {
+            ks_add(&tl, s->T[IDX(s, 0, j, k)]);
+            double qx = (8.0 * Th - 9.0 * s->T[IDX(s, 0, j, k)] + s->T[IDX(s, 1, j, k)]) / (3.0 * dx);
+            ks_add(&nl, qx / deltaT);
+            double qc = (-8.0 * Tc + 9.0 * s->T[IDX(s, nx - 1, j, k)] - s->T[IDX(s, nx - 2, j, k)]) /
+                        (3.0 * dx);
+            ks_add(&scold, qc / deltaT);
+            double mid;
+            if (nx % 2 == 1) {
+                int im = (nx + 1) / 2 - 1;
+                long nm = IDX(s, im, j, k);
+                mid = (coef * s->u[nm] * (s->T[nm] - Tref) +
+                       (s->T[IDX(s, im - 1, j, k)] - s->T[IDX(s, im + 1, j, k)]) / (2.0 * dx)) /
+                      deltaT;
+            } else {
+                int il = nx / 2 - 1, ir = il + 1;
+                long nl_ = IDX(s, il, j, k), nr_ = IDX(s, ir, j, k);
+                mid = (coef * 0.5 * (s->u[nl_] * (s->T[nl_] - Tref) + s->u[nr_] * (s->T[nr_] - Tref)) +
+                       (s->T[nl_] - s->T[nr_]) / dx) /
+                      deltaT;
+            }
+            ks_add(&smid, mid);
+        }
+        Tleft[j + 1] = ks_val(tl) / (double)nz;
+        Nu_ext[j + 1] = ks_val(nl) / (double)nz;
+        ks_add(&shot, Nu_ext[j + 1]);
+    }
+    d->nu_hot = ks_val(shot) / (double)ny;
+    d->nu_cold = ks_val(scold) / ((double)ny * nz);
+    d->nu_middle = ks_val(smid) / ((double)ny * nz);
+
+    const double *yp = s->yp;
+    double yfit[4], Tfit[4], Twb, Twt;
+    for (int m = 0; m < 4; ++m) {
+        yfit[m] = yp[1 + m];
+        Tfit[m] = Tleft[1 + m];
+    }
+    orc_fit_adiabatic_wall_T4(0.0, yfit, Tfit, &Twb);
+    Nu_ext[0] = (2.0 * (Th - Twb) / dx) / deltaT;
+    for (int m = 0; m < 4; ++m) {
+        yfit[m] = yp[ny - 3 + m];
+        Tfit[m] = Tleft[ny - 3 + m];
+    }
+    orc_fit_adiabatic_wall_T4(yp[ny + 1], yfit, Tfit, &Twt);
+    Nu_ext[ny + 1] = (2.0 * (Th - Twt) / dx) / deltaT;
+
+    int jmax = 0, jmin = 0;
+    double vmax = Nu_ext[0], vmin = Nu_ext[0];
+    for (int j = 1; j <= ny + 1; ++j) {
+        if (Nu_ext[j] > vmax) { vmax = Nu_ext[j]; jmax = j; }
+        if (Nu_ext[j] < vmin) { vmin = Nu_ext[j]; jmin = j; }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        int j0;
+        if (pass == 0) j0 = jmax <= 2 ? 0 : (jmax >= ny - 1 ? ny - 3 : jmax - 2);
+        else j0 = jmin >= 4 ? jmin - 4 : 0;
+        double yk[5], fk[5], fs, ys;
+        for (int m = 0; m < 5; ++m) {
+            yk[m] = yp[j0 + m];
+            fk[m] = Nu_ext[j0 + m];
+        }
+        orc_fit_parabola_ls5(yk, fk, pass == 0 ? 1 : -1, &fs, &ys);
+        if (pass == 0) { d->nu_hot_max = fs; d->nu_hot_max_pos = ys; }
+        else { d->nu_hot_min = fs; d->nu_hot_min_pos = ys; }
+    }
+    free(Nu_ext);
+    free(Tleft);
+}
+
 /* Diagnostics: calNuRe (3DRB.F90:1712-1778, 2DRB.F90:1610-1675), RBcalc_Nu_global
  * (3DRB.F90:1839-1878, 2DRB.F90:2251-2297), RBcalc_Nu_wall_avg (3DRB.F90:2049-2201,
- * 2DRB.F90:2308-2473), plus mass, heat content and kinetic energy. */
+ * 2DRB.F90:2308-2473), plus mass, heat content and kinetic energy; the side-heated cell's
+ * formulas when side_heated is set. */
 void orc_diagnostics(const orc_state *s, orc_diag *d)
 {
+    memset(d, 0, sizeof(*d));
+    if (s->c.side_heated) {
+        diagnostics_side_heated(s, d);
+        return;
+    }
     const orc_params *p = &s->p;
     const double Th = s->c.thot, Tc = s->c.tcold, Tref = p->Tref, L = p->L;
     const double dy = 1.0 / L, deltaT = Th - Tc, coef = L / p->diffusivity;
     const double N = (double)s->n;
     const int nx = NX, ny = NY, nz = NZ;
-    memset(d, 0, sizeof(*d));
 
     ksum snu = {0, 0}, sre = {0, 0}, sqy = {0, 0}, sm = {0, 0}, sh = {0, 0}, ske = {0, 0};
     for (int k = 0; k < nz; ++k) {

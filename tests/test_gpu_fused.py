@@ -48,7 +48,8 @@ def test_one_pass_fp64_bitwise(name, kchunk):
         np.testing.assert_array_equal(m["T"], s["T"])
 
 
-@pytest.mark.parametrize("tile", [rblbm.TILE_32X16, rblbm.TILE_16X16, rblbm.TILE_32X8])
+@pytest.mark.parametrize("tile", [rblbm.TILE_32X16, rblbm.TILE_16X16, rblbm.TILE_32X8,
+                                  rblbm.TILE_32X32])
 @pytest.mark.parametrize("prefetch", [0, 1])
 @pytest.mark.parametrize("prec", [rblbm.FP64, rblbm.FP32_SHIFT])
 def test_one_pass_tiles_and_prefetch(tile, prefetch, prec):
@@ -60,6 +61,10 @@ def test_one_pass_tiles_and_prefetch(tile, prefetch, prec):
     dims = dict(dims=3, nx=45, ny=37, nz=29, rayleigh=1e5, mach=0.3, precision=prec)
     sol = rblbm.Solver(rblbm.Config(**dims))
     sol.set_kernel_variant(rblbm.KERNEL_ONE_PASS, 7)
+    if tile == rblbm.TILE_32X32 and prec == rblbm.FP64:
+        with pytest.raises(rblbm.LbmError):         # 1024 threads: FP64 does not fit
+            sol.set_fused_tile(tile, prefetch)
+        return
     sol.set_fused_tile(tile, prefetch)
     assert sol.kernel_form() == dict(variant=rblbm.KERNEL_ONE_PASS, kchunk=7, tile=tile,
                                      prefetch=prefetch)

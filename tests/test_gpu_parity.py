@@ -142,6 +142,18 @@ def test_diagnostics_match_oracle_2d():
     _compare_diag(sol, ora)
 
 
+@pytest.mark.parametrize("nx", [31, 32])
+def test_side_heated_diagnostics_match_oracle(nx):
+    """Side-heated cell (hot wall x = 0): Nu_vol from u T, the x-direction Nu_global, and the
+    hot-wall profile along y with its own minimum window; odd and even nx (mid-plane rule)."""
+    kw = dict(side_heated=1, temp_x=0, temp_y=1, temp_z=1, rayleigh=1e6)
+    sol, ora = _run_pair(3, nx, 24, 12, (300,), **kw)
+    _compare_diag(sol, ora)
+    sol2, ora2 = _run_pair(2, nx, 40, 1, (400,), rayleigh=1e5, side_heated=1, vel_x=0, temp_x=0,
+                           temp_y=1, thot=1.0, tcold=0.0)
+    _compare_diag(sol2, ora2)
+
+
 def test_c2_128cubed_1000_steps():
     """128^3, Ra=1e6, Pr=0.71, FP64: relative max-norm <= 1e-12 after 1000 steps."""
     sol, ora = _run_pair(3, 128, 128, 128, (1000,), rayleigh=1e6)

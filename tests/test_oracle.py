@@ -250,6 +250,78 @@ def test_volume_diagnostics_closed_form():
     assert d["nu_middle"] == pytest.approx(mid, rel=1e-12)
 
 
+def _side_state(cfg, U, T):
+    """f, g of a uniform flow U and temperature field T (Fortran-order arrays)."""
+    m = Model(cfg.dims, cfg.nx, cfg.ny, cfg.nz)
+    U = np.array(U, dtype=float)
+    if cfg.dims == 2:
+        U[2] = 0.0
+    f = m.feq(np.ones(cfg.shape), np.broadcast_to(U, cfg.shape + (3,)).copy())
+    g = m.geq(T, np.zeros(cfg.shape + (3,)))
+    return f, g
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_side_heated_diagnostics_closed_form(dims):
+    # Side-heated cell (hot wall x = 0, L = nx): a uniform flow U through the conduction profile
+    # T = Thot - (x/L) dT.  The x differences are exact on it, so Nu_global, Nu_hot, Nu_cold are
+    # closed forms, and Nu_vol uses u T (2DRB/3DRB.F90:1732-1741), not v T.  The reload's half
+    # force acts on v only (gravity along y), so u = U_x exactly.
+    cfg = orc.Config(dims=dims, nx=10, ny=9, nz=6, rayleigh=1e5, mach=0.1, thot=1.0, tcold=0.0,
+                     side_heated=1, temp_x=orc.T_CONST, temp_y=orc.T_ADIABATIC)
+    p = orc.params(cfg)
+    assert p["L"] == cfg.nx
+    U = np.array([0.01, -0.02, 0.005])
+    x = (np.arange(cfg.nx) + 0.5) / cfg.nx
+    T = (cfg.thot - x * (cfg.thot - cfg.tcold))[None, None, :] * np.ones(cfg.shape)
+    f, g = _side_state(cfg, U, T)
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    assert o.set_fg(f, g) == 0
+    d = o.diagnostics()
+    L, kappa, dT = p["L"], p["diffusivity"], cfg.thot - cfg.tcold
+    assert d["nu_vol"] == pytest.approx(1 + L / kappa * U[0] * T.mean(), rel=1e-12)
+    conv = L / kappa * U[0] * (T - p["Tref"])
+    assert d["nu_global"] == pytest.approx((conv.mean() + dT) / dT, rel=1e-12)
+    assert d["nu_hot"] == pytest.approx(1.0, rel=1e-12)
+    assert d["nu_cold"] == pytest.approx(1.0, rel=1e-12)
+    il, ir = cfg.nx // 2 - 1, cfg.nx // 2
+    mid = 0.5 * (conv[..., il] + conv[..., ir]).mean() / dT + 1.0
+    assert d["nu_middle"] == pytest.approx(mid, rel=1e-12)
+    # the hot-wall profile is flat (corner fits included): its extrema are 1
+    assert d["nu_hot_max"] == pytest.approx(1.0, rel=1e-12)
+    assert d["nu_hot_min"] == pytest.approx(1.0, rel=1e-12)
+
+
+def test_side_heated_hot_wall_minimum_window():
+    # The side-heated minimum is fitted on the five profile points jmin-4 .. jmin
+    # (3DRB.F90:1975-1980), not on a window centred on jmin as for the maximum.  The first two
+    # columns carry an upward parabola in y whose vertex y0 lies just above the grid point
+    # nearest to it: a centred window contains y0 and returns it, the reference's window
+    # jmin-4 .. jmin ends below y0, so the fit falls back to the grid point and its value.
+    cfg = orc.Config(dims=3, nx=8, ny=16, nz=4, rayleigh=1e5, mach=0.1, thot=1.0, tcold=0.0,
+                     side_heated=1, temp_x=orc.T_CONST, temp_y=orc.T_ADIABATIC)
+    p = orc.params(cfg)
+    L, dT = p["L"], cfg.thot - cfg.tcold
+    yp = (np.arange(cfg.ny) + 0.5) / L
+    jstar = 9
+    y0 = yp[jstar] + 0.2 / L                      # nearest grid point: jstar
+    T1 = 0.9 - 0.3 * (yp - y0) ** 2               # colder towards both corners
+    T = np.broadcast_to(np.linspace(0.9, 0.1, cfg.nx)[None, None, :], cfg.shape).copy()
+    T[:, :, 0] = T1[None, :]
+    T[:, :, 1] = T1[None, :]
+    f, g = _side_state(cfg, np.zeros(3), T)
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    assert o.set_fg(f, g) == 0
+    d = o.diagnostics()
+    nu_left = (8 * cfg.thot - 9 * T1 + T1) / (3 / L) / dT
+    assert np.argmin(nu_left) == jstar
+    assert d["nu_hot_min_pos"] == pytest.approx(yp[jstar], rel=1e-12)
+    assert d["nu_hot_min"] == pytest.approx(nu_left[jstar], rel=1e-12)
+    assert d["nu_hot"] == pytest.approx(nu_left.mean(), rel=1e-12)
+
+
 def test_restart_reconstruction_roundtrip():
     cfg = orc.Config(dims=3, nx=7, ny=6, nz=5, rayleigh=1e5, mach=0.3)
     o = orc.Oracle(cfg)
