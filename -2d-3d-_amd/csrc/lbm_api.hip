@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -844,6 +845,138 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
         else { d->nu_hot_min = fs; d->nu_hot_min_pos = ys; }
     }
     return LBM_OK;
+}
+
+// ---- mid-plane velocity maxima --------------------------------------------------------------
+// find_bracketing_index (3DRB.F90:2525-2547) on 1-based coordinates coord[0..n+1].
+static void bracket_index(const std::vector<double> &coord, int n, double target, int *iL, int *iR,
+                          double *weight)
+{
+    int l = 1;
+    while (l < n && coord[l + 1] < target) ++l;
+    const int r = l + 1 < n ? l + 1 : n;
+    double w;
+    if (fabs(coord[r] - coord[l]) <= 1.0e-14) w = 0.0;
+    else w = (target - coord[l]) / (coord[r] - coord[l]);
+    *iL = l;
+    *iR = r;
+    *weight = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+}
+
+// The 2D routines' 5-point least-squares parabola around the grid maximum of line[1..n].
+static void line_max_fit(const std::vector<double> &coord, const std::vector<double> &line, int n,
+                         double *fmax, double *pos)
+{
+    int i0 = 1;
+    for (int i = 2; i <= n; ++i)
+        if (line[i] > line[i0]) i0 = i;
+    const int a = i0 <= 2 ? 1 : (i0 >= n - 1 ? n - 4 : i0 - 2);
+    fit_parabola5(&coord[a], &line[a], 1, fmax, pos);
+}
+
+extern "C" int lbm_midplane_extrema(lbm_handle *h, lbm_midplane *m)
+{
+    if (!h || !m) return LBM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc = refresh_macros(h);
+    if (rc) return rc;
+    const lbm_config &c = h->cfg;
+    if (c.nx < 5 || c.ny < 5) return fail(h, LBM_ERR_ARG, "mid-plane fits need nx, ny >= 5");
+    const double L = h->p.L, coef = L / h->p.diffusivity;
+    const int n[3] = {c.nx, c.ny, c.dims == 3 ? c.nz : 1};
+    std::vector<double> coord[3];
+    for (int a = 0; a < 3; ++a) {   // node coordinates / L, walls at 0 and n / L
+        coord[a].assign(n[a] + 2, 0.0);
+        coord[a][n[a] + 1] = (double)n[a] / L;
+        for (int i = 1; i <= n[a]; ++i) coord[a][i] = ((double)i - 0.5) / L;
+    }
+    const size_t np = (size_t)std::max(n[1] * n[2], std::max(n[0] * n[2], n[0] * n[1]));
+    double *dplane = nullptr;
+    if (hipMalloc((void **)&dplane, np * sizeof(double)) != hipSuccess)
+        return fail(h, LBM_ERR_NOMEM, "mid-plane buffer");
+    std::vector<double> plane(np);
+    // one plane of component `comp` (1 u, 2 v, 3 w) across `axis` -> host vector
+    auto extract = [&](int comp, int axis, int l, int r, double w, int mode, size_t cnt) -> int {
+        hipError_t e = launch_plane(h->kp, h->stream, h->mac + (size_t)comp * h->N, axis, l - 1,
+                                    r - 1, w, mode, dplane);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(plane.data(), dplane, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        return e == hipSuccess ? LBM_OK : fail(h, LBM_ERR_HIP, "mid-plane: %s", hipGetErrorString(e));
+    };
+    memset(m, 0, sizeof(*m));
+    if (c.dims == 3) {
+        // calc_{u,v,w}mid_max_common (3DRB.F90:2399-2518): the first grid maximum in loop order
+        double *out[3] = {&m->u_max, &m->v_max, &m->w_max};
+        for (int a = 0; a < 3 && !rc; ++a) {
+            const int pa = a == 0 ? 1 : 0, qa = a == 2 ? 1 : 2;
+            const double target = 0.5 * coord[a][n[a] + 1];
+            int iL, iR;
+            double w;
+            bracket_index(coord[a], n[a], target, &iL, &iR, &w);
+            const size_t cnt = (size_t)n[pa] * n[qa];
+            rc = extract(a + 1, a, iL, iR, w, 0, cnt);
+            if (rc) break;
+            size_t best = 0;
+            for (size_t t = 1; t < cnt; ++t)
+                if (plane[t] > plane[best]) best = t;
+            out[a][0] = plane[best] * coef;
+            out[a][1] = coord[pa][1 + best % n[pa]];
+            out[a][2] = coord[qa][1 + best / n[pa]];
+            out[a][3] = target;
+        }
+    } else {
+        const int nx = n[0], ny = n[1];
+        std::vector<double> line;
+        double fmax = 0.0, pos = 0.0;
+        // u along y: the middle column (side-heated, 2DRB.F90:2081-2153) or x = 0.5
+        // (Rayleigh-Benard, 2DRB.F90:2484-2564, position folded into the lower half)
+        if (c.side_heated) {
+            if (nx % 2 == 1) {
+                rc = extract(1, 0, (nx + 1) / 2, (nx + 1) / 2, 0.0, 0, ny);
+                m->u_plane = coord[0][(nx + 1) / 2];
+            } else {
+                rc = extract(1, 0, nx / 2, nx / 2 + 1, 0.0, 1, ny);
+                m->u_plane = 0.5 * (coord[0][nx / 2] + coord[0][nx / 2 + 1]);
+            }
+        } else {
+            const double target = 0.5;
+            int il = 1;
+            while (il < nx && coord[0][il + 1] < target) ++il;
+            const int ir = il + 1 < nx ? il + 1 : nx;
+            if (fabs(coord[0][il] - target) <= 1.0e-14 || ir == il)
+                rc = extract(1, 0, il, il, 0.0, 0, ny);
+            else
+                rc = extract(1, 0, il, ir, (target - coord[0][il]) / (coord[0][ir] - coord[0][il]), 0, ny);
+            m->u_plane = target;
+        }
+        if (!rc) {
+            line.assign(plane.begin(), plane.begin() + ny);
+            line.insert(line.begin(), 0.0);
+            line_max_fit(coord[1], line, ny, &fmax, &pos);
+            if (!c.side_heated && pos > 0.5 * coord[1][ny + 1]) pos = coord[1][ny + 1] - pos;
+            m->u_max = fmax * coef;
+            m->u_pos1 = pos;
+            // v along x on the middle row (2DRB.F90:2575-2642, :2166-2238)
+            if (ny % 2 == 1) {
+                rc = extract(2, 1, (ny + 1) / 2, (ny + 1) / 2, 0.0, 0, nx);
+                m->v_plane = coord[1][(ny + 1) / 2];
+            } else {
+                rc = extract(2, 1, ny / 2, ny / 2 + 1, 0.0, 1, nx);
+                m->v_plane = 0.5 * (coord[1][ny / 2] + coord[1][ny / 2 + 1]);
+            }
+        }
+        if (!rc) {
+            line.assign(plane.begin(), plane.begin() + nx);
+            line.insert(line.begin(), 0.0);
+            line_max_fit(coord[0], line, nx, &fmax, &pos);
+            m->v_max = fmax * coef;
+            m->v_pos1 = pos;
+        }
+    }
+    (void)hipFree(dplane);
+    return rc;
 }
 
 // Chunk of z-planes whose AoS f and g fit a staging buffer of about 256 MiB.

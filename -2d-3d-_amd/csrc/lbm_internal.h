@@ -52,6 +52,9 @@ hipError_t launch_reduce_volume(const KParams &kp, hipStream_t s, const double *
 hipError_t launch_reduce_walls(const KParams &kp, hipStream_t s, const double *mac,
                                double L_over_kappa, double dy, int side, double *tbot,
                                double *nubot, double *out);
+// Plane of a macro field across `axis` at 0-based indices (l, r), see k_plane.
+hipError_t launch_plane(const KParams &kp, hipStream_t s, const double *f, int axis, int l, int r,
+                        double w, int mode, double *out);
 hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *mac, double *prev,
                                double *out, unsigned blocks);
 

@@ -710,6 +710,37 @@ __global__ void __launch_bounds__(256) k_reduce_walls(KParams kp, const double *
     }
 }
 
+// One plane of a macro field across `axis` (0: x, 1: y, 2: z) at the 0-based index pair
+// (l, r): mode 0 interpolates (1 - w) f(l) + w f(r), or takes f(l) when l == r
+// (interp_scalar_x/y/z, 3DRB.F90:2550-2600); mode 1 averages 0.5 (f(l) + f(r)) (the 2D
+// mid-line averages, 2DRB.F90:2097-2108).  Output: the two in-plane axes, the lower one fastest.
+__global__ void __launch_bounds__(256) k_plane(KParams kp, const double *__restrict__ f, int axis,
+                                               int l, int r, double w, int mode,
+                                               double *__restrict__ out)
+{
+    const uint64_t nx = kp.nx, ny = kp.ny, nz = kp.nz;
+    const uint64_t np = axis == 0 ? ny * nz : (axis == 1 ? nx * nz : nx * ny);
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t cl, cr;
+        if (axis == 0) {
+            const uint64_t j = t % ny, k = t / ny;
+            cl = l + nx * (j + ny * k);
+            cr = r + nx * (j + ny * k);
+        } else if (axis == 1) {
+            const uint64_t i = t % nx, k = t / nx;
+            cl = i + nx * (l + ny * k);
+            cr = i + nx * (r + ny * k);
+        } else {
+            const uint64_t i = t % nx, j = t / nx;
+            cl = i + nx * (j + ny * l);
+            cr = i + nx * (j + ny * r);
+        }
+        const double fl = f[cl], fr = f[cr];
+        out[t] = mode ? 0.5 * (fl + fr) : (l == r ? fl : (1.0 - w) * fl + w * fr);
+    }
+}
+
 // `check` sums: 0 sum |u - u_p|^2, 1 sum |u|^2, 2 sum |T - T_p|, 3 sum |T|; then u_p = u.
 __global__ void __launch_bounds__(256) k_reduce_check(KParams kp, const double *__restrict__ mac,
                                                       double *__restrict__ prev,
@@ -913,6 +944,16 @@ hipError_t launch_reduce_walls(const KParams &kp, hipStream_t s, const double *m
 {
     k_reduce_walls<<<side ? kp.ny : kp.nx, 256, 0, s>>>(kp, mac, L_over_kappa, dy, side, tbot,
                                                        nubot, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_plane(const KParams &kp, hipStream_t s, const double *f, int axis, int l, int r,
+                        double w, int mode, double *out)
+{
+    const uint64_t np = axis == 0 ? (uint64_t)kp.ny * kp.nz
+                                  : (axis == 1 ? (uint64_t)kp.nx * kp.nz : (uint64_t)kp.nx * kp.ny);
+    const unsigned blocks = (unsigned)((np + 255) / 256 < 4096 ? (np + 255) / 256 : 4096);
+    k_plane<<<blocks, 256, 0, s>>>(kp, f, axis, l, r, w, mode, out);
     return hipGetLastError();
 }
 

@@ -34,6 +34,12 @@ module lbm_rb
     real(c_double) :: mass, heat, kinetic
   end type lbm_diag
 
+  type, bind(C) :: lbm_midplane
+    real(c_double) :: u_max, u_pos1, u_pos2, u_plane
+    real(c_double) :: v_max, v_pos1, v_pos2, v_plane
+    real(c_double) :: w_max, w_pos1, w_pos2, w_plane
+  end type lbm_midplane
+
   interface
     integer(c_int) function lbm_create(cfg, h) bind(C, name="lbm_create")
       import :: c_int, c_ptr, lbm_config
@@ -81,6 +87,11 @@ module lbm_rb
       import :: c_int, c_ptr, lbm_diag
       type(c_ptr), value :: h
       type(lbm_diag), intent(out) :: d
+    end function
+    integer(c_int) function lbm_midplane_extrema(h, m) bind(C, name="lbm_midplane_extrema")
+      import :: c_int, c_ptr, lbm_midplane
+      type(c_ptr), value :: h
+      type(lbm_midplane), intent(out) :: m
     end function
     integer(c_int) function lbm_gather(h, f, g, bprev) bind(C, name="lbm_gather")
       import :: c_int, c_ptr

@@ -47,6 +47,7 @@ program rb_main
   type(lbm_config) :: cfg
   type(lbm_params) :: prm
   type(lbm_diag) :: dg
+  type(lbm_midplane) :: mp
   type(c_ptr) :: h = c_null_ptr
   integer :: itc, nchunk, qf, qt, ios
   real(8) :: errorU, errorT
@@ -90,6 +91,7 @@ program rb_main
   seconds = dble(t1 - t0) / dble(rate)
 
   call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
+  call lbm_require(lbm_midplane_extrema(h, mp), 'lbm_midplane_extrema', h, 6)
   if (outputBinFile == 1) call output_binary()
   if (backupAtEnd == 1) call backupData()
 
@@ -101,6 +103,7 @@ program rb_main
   write(*,'(a,1x,es16.8,2x,a,1x,es16.8)') 'Nu_hot_min =', dg%nu_hot_min, 'x_min =', dg%nu_hot_min_pos
   write(*,'(a,1x,es16.8)') 'NuVolAvg =', dg%nu_vol
   write(*,'(a,1x,es16.8)') 'ReVolAvg =', dg%re_vol
+  call report_midplane()
   write(*,'(a,1x,i0)') 'itc =', itc
 
   open(unit=00, file=trim(settingsFile), status='unknown', position='append')
@@ -119,6 +122,31 @@ program rb_main
   call lbm_require(lbm_destroy(h), 'lbm_destroy', c_null_ptr, 6)
 
 contains
+
+  !------------------------------------------------------------------------------------------
+  ! Mid-plane velocity maxima, printed as calc_*mid_max_common (2DRB/3DRB.F90:2399-2518) and
+  ! the 2D RBcalc_/SideHeatedcalc_ umid/vmid routines (2DRB/2DRB.F90:2081-2238, :2484-2642)
+  ! report them, and appended to the settings log.
+  !------------------------------------------------------------------------------------------
+  subroutine report_midplane()
+    character(len=*), parameter :: fmt3 = '(A,1X,ES16.8,2X,A,1X,ES16.8,2X,A,1X,ES16.8,2X,A,1X,ES16.8)'
+    character(len=*), parameter :: fmt2 = '(A,1X,ES16.8,2X,A,1X,ES16.8,2X,A,1X,ES16.8)'
+    open(unit=00, file=trim(settingsFile), status='unknown', position='append')
+    if (dims == 3) then
+      write(*,fmt3) 'u_mid_max =', mp%u_max, 'at y =', mp%u_pos1, 'z =', mp%u_pos2, 'on x_mid =', mp%u_plane
+      write(*,fmt3) 'v_mid_max =', mp%v_max, 'at x =', mp%v_pos1, 'z =', mp%v_pos2, 'on y_mid =', mp%v_plane
+      write(*,fmt3) 'w_mid_max =', mp%w_max, 'at x =', mp%w_pos1, 'y =', mp%w_pos2, 'on z_mid =', mp%w_plane
+      write(00,*) 'u_mid_max =', mp%u_max, ' y_pos =', mp%u_pos1, ' z_pos =', mp%u_pos2
+      write(00,*) 'v_mid_max =', mp%v_max, ' x_pos =', mp%v_pos1, ' z_pos =', mp%v_pos2
+      write(00,*) 'w_mid_max =', mp%w_max, ' x_pos =', mp%w_pos1, ' y_pos =', mp%w_pos2
+    else
+      write(*,fmt2) 'u_mid_max* =', mp%u_max, 'y =', mp%u_pos1, 'x_mid =', mp%u_plane
+      write(*,fmt2) 'v_mid_max* =', mp%v_max, 'x =', mp%v_pos1, 'y_mid =', mp%v_plane
+      write(00,fmt2) 'u_mid_max* =', mp%u_max, 'y =', mp%u_pos1, 'x_mid =', mp%u_plane
+      write(00,fmt2) 'v_mid_max* =', mp%v_max, 'x =', mp%v_pos1, 'y_mid =', mp%v_plane
+    endif
+    close(00)
+  end subroutine report_midplane
 
   !------------------------------------------------------------------------------------------
   ! initial (2DRB/3DRB.F90:342-643): parameters, allocation, exact or restart initialisation

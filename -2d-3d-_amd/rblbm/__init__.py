@@ -26,7 +26,7 @@ TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32 = 0, 1, 2, 3
 
 EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
            "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",
-           "lbm_diagnostics", "lbm_gather", "lbm_scatter", "lbm_bytes_per_update",
+           "lbm_diagnostics", "lbm_midplane_extrema", "lbm_gather", "lbm_scatter", "lbm_bytes_per_update",
            "lbm_set_kernel_variant", "lbm_set_fused_tile", "lbm_get_kernel_form",
            "lbm_last_error", "lbm_error_string")
 
@@ -58,6 +58,12 @@ class _Diag(C.Structure):
         "nu_hot_max_pos", "nu_hot_min", "nu_hot_min_pos", "mass", "heat", "kinetic")]
 
 
+class _Mid(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "u_max", "u_pos1", "u_pos2", "u_plane", "v_max", "v_pos1", "v_pos2", "v_plane",
+        "w_max", "w_pos1", "w_pos2", "w_plane")]
+
+
 _lib = None
 
 
@@ -85,6 +91,7 @@ def library(path: str = LIB_PATH):
     L.lbm_macros.argtypes = [h, dp, dp, dp, dp, dp]
     L.lbm_check.argtypes = [h, dp, dp]
     L.lbm_diagnostics.argtypes = [h, C.POINTER(_Diag)]
+    L.lbm_midplane_extrema.argtypes = [h, C.POINTER(_Mid)]
     L.lbm_gather.argtypes = [h, dp, dp, dp]
     L.lbm_scatter.argtypes = [h, dp, dp]
     L.lbm_bytes_per_update.argtypes = [h, dp]
@@ -256,6 +263,12 @@ class Solver:
         d = _Diag()
         self._chk(self.lib.lbm_diagnostics(self._h, C.byref(d)))
         return {n: getattr(d, n) for n, _ in _Diag._fields_}
+
+    def midplane(self) -> dict:
+        """Mid-plane velocity maxima (scaled by L / kappa) and their positions."""
+        m = _Mid()
+        self._chk(self.lib.lbm_midplane_extrema(self._h, C.byref(m)))
+        return {n: getattr(m, n) for n, _ in _Mid._fields_}
 
     def gather(self, with_bprev: bool = True) -> dict:
         sh = self.cfg.shape

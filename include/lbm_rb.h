@@ -74,6 +74,15 @@ typedef struct lbm_diag {
     double mass, heat, kinetic;   /* sum rho, sum T, 0.5 sum rho|u|^2 (conservation checks) */
 } lbm_diag;
 
+/* Mid-plane velocity maxima, scaled by velocityScaleCompare = L / kappa as the reference
+ * reports them.  Per component: the maximum, its position (3D: the two in-plane coordinates;
+ * 2D: one, pos2 = 0) and the plane's coordinate, all divided by L. */
+typedef struct lbm_midplane {
+    double u_max, u_pos1, u_pos2, u_plane;   /* u on the x mid-plane (3D: pos = y, z) */
+    double v_max, v_pos1, v_pos2, v_plane;   /* v on the y mid-plane (3D: pos = x, z) */
+    double w_max, w_pos1, w_pos2, w_plane;   /* w on the z mid-plane (3D: pos = x, y; 2D: 0) */
+} lbm_midplane;
+
 typedef struct lbm_handle lbm_handle;
 
 /* Allocate the domain on cfg->device.  Replaces the allocate block of `initial`
@@ -108,6 +117,13 @@ int lbm_check(lbm_handle *h, double *error_u, double *error_t);
 
 /* Nusselt / Reynolds / conservation diagnostics of the current state. */
 int lbm_diagnostics(lbm_handle *h, lbm_diag *d);
+/* Mid-plane velocity maxima of the current state.  3D: calc_umid_max_common,
+ * calc_vmid_max_common, calc_wmid_max_common (2DRB/3DRB.F90:2399-2518; the field interpolated
+ * onto the plane at half the domain extent, the first grid maximum).  2D: RBcalc_umid_max /
+ * RBcalc_vmid_max (2DRB/2DRB.F90:2484-2642) or, for the side-heated cell,
+ * SideHeatedcalc_umid_max / SideHeatedcalc_vmid_max (:2081-2238): a 5-point least-squares
+ * parabola around the grid maximum of the sampled line.  Needs nx, ny >= 5. */
+int lbm_midplane_extrema(lbm_handle *h, lbm_midplane *m);
 
 /* f, g (reference AoS order, absolute FP64) and, if non-NULL, B_prev (3 planes of nx*ny*nz:
  * Bx_prev, By_prev, Bz_prev, 2DRB/3DRB.F90:163).  Restart data of backupData

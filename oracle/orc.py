@@ -46,6 +46,12 @@ class _Diag(C.Structure):
         "nu_hot_max_pos", "nu_hot_min", "nu_hot_min_pos", "mass", "heat", "kinetic")]
 
 
+class _Mid(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "u_max", "u_pos1", "u_pos2", "u_plane", "v_max", "v_pos1", "v_pos2", "v_plane",
+        "w_max", "w_pos1", "w_pos2", "w_plane")]
+
+
 def build() -> str:
     """Compile liborc.so with the committed Makefile (gcc, -ffp-contract=off, OpenMP)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -78,6 +84,7 @@ def lib():
         L.orc_set_fg.restype = C.c_int
         L.orc_check.argtypes = [C.c_void_p, dp, dp]
         L.orc_diagnostics.argtypes = [C.c_void_p, C.POINTER(_Diag)]
+        L.orc_midplane.argtypes = [C.c_void_p, C.POINTER(_Mid)]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = C.c_int
         L.orc_fit_adiabatic_wall_T4.argtypes = [C.c_double, dp, dp, dp]
@@ -217,3 +224,9 @@ class Oracle:
         d = _Diag()
         lib().orc_diagnostics(self.h, C.byref(d))
         return {n: getattr(d, n) for n, _ in _Diag._fields_}
+
+    def midplane(self) -> dict:
+        """Mid-plane velocity maxima (scaled by L / kappa) and their positions."""
+        m = _Mid()
+        lib().orc_midplane(self.h, C.byref(m))
+        return {n: getattr(m, n) for n, _ in _Mid._fields_}

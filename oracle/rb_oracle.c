@@ -1361,3 +1361,147 @@ void orc_diagnostics(const orc_state *s, orc_diag *d)
     free(Nu_ext);
     free(Tbot);
 }
+
+/* ---------------------------------------------------------------------------------------
+ * Mid-plane velocity maxima
+ * 3D (both cells): calc_umid_max_common / calc_vmid_max_common / calc_wmid_max_common
+ *   (3DRB.F90:2399-2518) with find_bracketing_index and interp_scalar_x/y/z (:2525-2600):
+ *   the field is interpolated linearly onto the plane at half the domain's extent and the
+ *   first grid maximum in loop order (outer index slowest) is reported with its coordinates.
+ * 2D Rayleigh-Benard cell: RBcalc_umid_max (2DRB.F90:2484-2564; u on x = 0.5, a fixed
+ *   nondimensional coordinate, the fitted position folded into the lower half) and
+ *   RBcalc_vmid_max (:2575-2642; v on the middle row).
+ * 2D side-heated cell: SideHeatedcalc_umid_max (2DRB.F90:2081-2153; u on the middle column)
+ *   and SideHeatedcalc_vmid_max (:2166-2238).
+ * The 2D maxima come from the 5-point least-squares parabola around the grid maximum.
+ * ------------------------------------------------------------------------------------- */
+static void bracket(const double *coord, int n, double target, int *iL, int *iR, double *weight)
+{
+    int l = 1;
+    while (l < n && coord[l + 1] < target) ++l;
+    int r = l + 1 < n ? l + 1 : n;
+    double w;
+    if (fabs(coord[r] - coord[l]) <= 1.0e-14) w = 0.0;
+    else w = (target - coord[l]) / (coord[r] - coord[l]);
+    w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+    *iL = l;
+    *iR = r;
+    *weight = w;
+}
+
+/* value of field at (a = interpolated index pair along `axis`, other two indices) */
+static double interp(const orc_state *s, const double *f, int axis, int L_, int R_, double w,
+                     int p, int q)
+{
+    long nl, nr;
+    if (axis == 0) { nl = IDX(s, L_ - 1, p - 1, q - 1); nr = IDX(s, R_ - 1, p - 1, q - 1); }
+    else if (axis == 1) { nl = IDX(s, p - 1, L_ - 1, q - 1); nr = IDX(s, p - 1, R_ - 1, q - 1); }
+    else { nl = IDX(s, p - 1, q - 1, L_ - 1); nr = IDX(s, p - 1, q - 1, R_ - 1); }
+    if (L_ == R_) return f[nl];
+    return (1.0 - w) * f[nl] + w * f[nr];
+}
+
+/* 5-point window around the grid maximum i0 of a line of n values (1-based), as the 2D
+ * routines choose it, then the least-squares parabola through it. */
+static void line_fit(const double *coord, const double *line, int n, double *fmax, double *pos)
+{
+    int i0 = 1;
+    double best = line[1];
+    for (int i = 2; i <= n; ++i)
+        if (line[i] > best) { best = line[i]; i0 = i; }
+    int a = i0 <= 2 ? 1 : (i0 >= n - 1 ? n - 4 : i0 - 2);
+    double xk[5], fk[5];
+    for (int m = 0; m < 5; ++m) { xk[m] = coord[a + m]; fk[m] = line[a + m]; }
+    orc_fit_parabola_ls5(xk, fk, 1, fmax, pos);
+}
+
+void orc_midplane(const orc_state *s, orc_mid *m)
+{
+    const orc_params *p = &s->p;
+    const double coef = p->L / p->diffusivity;
+    const int nx = NX, ny = NY, nz = NZ;
+    memset(m, 0, sizeof(*m));
+    if (s->c.dims == 3) {
+        const double *fields[3] = {s->u, s->v, s->w};
+        const double *coords[3] = {s->xp, s->yp, s->zp};
+        const int n[3] = {nx, ny, nz};
+        double *out[3] = {&m->u_max, &m->v_max, &m->w_max};
+        for (int a = 0; a < 3; ++a) {
+            /* in-plane axes: (p, q) = (y, z) for u, (x, z) for v, (x, y) for w; q outermost */
+            const int pa = a == 0 ? 1 : 0, qa = a == 2 ? 1 : 2;
+            const double target = 0.5 * coords[a][n[a] + 1];
+            int iL, iR;
+            double w;
+            bracket(coords[a], n[a], target, &iL, &iR, &w);
+            double best = -HUGE_VAL;
+            int pb = 1, qb = 1;
+            for (int q = 1; q <= n[qa]; ++q)
+                for (int pp = 1; pp <= n[pa]; ++pp) {
+                    double val = interp(s, fields[a], a, iL, iR, w, pp, q);
+                    if (val > best) { best = val; pb = pp; qb = q; }
+                }
+            out[a][0] = best * coef;
+            out[a][1] = coords[pa][pb];
+            out[a][2] = coords[qa][qb];
+            out[a][3] = target;
+        }
+        return;
+    }
+    /* 2D */
+    double *uline = (double *)calloc((size_t)ny + 1, sizeof(double));
+    double *vline = (double *)calloc((size_t)nx + 1, sizeof(double));
+    const double *xp = s->xp, *yp = s->yp;
+    double xmid;
+    if (s->c.side_heated) {
+        if (nx % 2 == 1) {
+            int im = (nx + 1) / 2;
+            xmid = xp[im];
+            for (int j = 1; j <= ny; ++j) uline[j] = s->u[IDX(s, im - 1, j - 1, 0)];
+        } else {
+            int il = nx / 2, ir = il + 1;
+            xmid = 0.5 * (xp[il] + xp[ir]);
+            for (int j = 1; j <= ny; ++j)
+                uline[j] = 0.5 * (s->u[IDX(s, il - 1, j - 1, 0)] + s->u[IDX(s, ir - 1, j - 1, 0)]);
+        }
+    } else {
+        const double target = 0.5;
+        int il = 1;
+        while (il < nx && xp[il + 1] < target) ++il;
+        int ir = il + 1 < nx ? il + 1 : nx;
+        xmid = target;
+        if (fabs(xp[il] - target) <= 1.0e-14 || ir == il) {
+            for (int j = 1; j <= ny; ++j) uline[j] = s->u[IDX(s, il - 1, j - 1, 0)];
+        } else {
+            double w = (target - xp[il]) / (xp[ir] - xp[il]);
+            for (int j = 1; j <= ny; ++j)
+                uline[j] = (1.0 - w) * s->u[IDX(s, il - 1, j - 1, 0)] + w * s->u[IDX(s, ir - 1, j - 1, 0)];
+        }
+    }
+    double fmax, pos;
+    line_fit(yp, uline, ny, &fmax, &pos);
+    if (!s->c.side_heated) {
+        double yLen = yp[ny + 1];
+        if (pos > 0.5 * yLen) pos = yLen - pos;
+    }
+    m->u_max = fmax * coef;
+    m->u_pos1 = pos;
+    m->u_plane = xmid;
+
+    double ymid;
+    if (ny % 2 == 1) {
+        int jm = (ny + 1) / 2;
+        ymid = yp[jm];
+        for (int i = 1; i <= nx; ++i) vline[i] = s->v[IDX(s, i - 1, jm - 1, 0)];
+    } else {
+        int jb = ny / 2, jt = jb + 1;
+        ymid = 0.5 * (yp[jb] + yp[jt]);
+        for (int i = 1; i <= nx; ++i)
+            vline[i] = 0.5 * (s->v[IDX(s, i - 1, jb - 1, 0)] + s->v[IDX(s, i - 1, jt - 1, 0)]);
+    }
+    line_fit(xp, vline, nx, &fmax, &pos);
+    m->v_max = fmax * coef;
+    m->v_pos1 = pos;
+    m->v_plane = ymid;
+    free(uline);
+    free(vline);
+}

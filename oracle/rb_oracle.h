@@ -51,6 +51,15 @@ typedef struct orc_diag {
     double mass, heat, kinetic;              /* sum rho, sum T, 0.5 sum rho |u|^2 */
 } orc_diag;
 
+/* Mid-plane velocity maxima, scaled by velocityScaleCompare = L / kappa as the reference
+ * reports them.  For each component: the maximum, its position (two in-plane coordinates in
+ * 3D; one in 2D, the second is 0) and the plane's coordinate. */
+typedef struct orc_mid {
+    double u_max, u_pos1, u_pos2, u_plane;
+    double v_max, v_pos1, v_pos2, v_plane;
+    double w_max, w_pos1, w_pos2, w_plane;   /* 3D only */
+} orc_mid;
+
 typedef struct orc_state orc_state;
 
 int orc_params_compute(const orc_cfg *cfg, orc_params *p);
@@ -76,6 +85,7 @@ int orc_set_fg(orc_state *s, const double *f, const double *g);
 
 void orc_check(orc_state *s, double *error_u, double *error_t);
 void orc_diagnostics(const orc_state *s, orc_diag *d);
+void orc_midplane(const orc_state *s, orc_mid *m);
 
 void orc_fit_adiabatic_wall_T4(double y0, const double *y, const double *tt, double *t_wall);
 void orc_fit_parabola_ls5(const double *y, const double *f, int mode, double *fstar,

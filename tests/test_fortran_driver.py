@@ -61,6 +61,14 @@ def test_driver_matches_oracle(tmp_path, dims):
     vals = dict(re.findall(r"^(\w+)\s*(?:\(\w+\))?\s*=\s*(\S+)$", out, re.M))
     assert float(vals["NuVolAvg"]) == pytest.approx(d["nu_vol"], rel=1e-7)   # printed es16.8
     assert float(vals["Nu_global"]) == pytest.approx(d["nu_global"], rel=1e-7)
+    # mid-plane maxima, printed as the reference's end-of-run report does
+    mp = o.midplane()
+    tag = "" if dims == 3 else r"\*"
+    got = re.search(rf"^u_mid_max{tag} =\s+(\S+)", out, re.M)
+    assert got, out
+    assert float(got.group(1)) == pytest.approx(mp["u_max"], rel=1e-7)
+    got = re.search(rf"^v_mid_max{tag} =\s+(\S+)", out, re.M)
+    assert float(got.group(1)) == pytest.approx(mp["v_max"], rel=1e-7)
     recs = _read_records(tmp_path / "backupFile3D-100.bin")
     s = o.state()
     qf, qt = (19, 7) if dims == 3 else (9, 5)

@@ -154,6 +154,36 @@ def test_side_heated_diagnostics_match_oracle(nx):
     _compare_diag(sol2, ora2)
 
 
+def _compare_midplane(sol, ora):
+    a, b = sol.midplane(), ora.midplane()
+    for k in b:
+        if k.endswith("_max"):
+            assert a[k] == pytest.approx(b[k], rel=REL_DIAG, abs=1e-300), (k, a[k], b[k])
+        else:       # positions and plane coordinates: grid values or fitted vertices
+            assert a[k] == pytest.approx(b[k], rel=1e-9, abs=1e-12), (k, a[k], b[k])
+
+
+@pytest.mark.parametrize("case", ["rb_3d", "side_3d", "rb_2d_periodic", "rb_2d_walls", "side_2d_odd",
+                                  "side_2d_even"])
+def test_midplane_maxima_match_oracle(case):
+    runs = {
+        "rb_3d": (3, 20, 17, 12, (200,), dict(rayleigh=1e6)),
+        "side_3d": (3, 19, 16, 10, (200,), dict(side_heated=1, temp_x=0, temp_y=1, temp_z=1,
+                                               rayleigh=1e6)),
+        "rb_2d_periodic": (2, 64, 33, 1, (400,), dict(rayleigh=1e5, vel_x=1, temp_x=2, thot=1.0,
+                                                      tcold=0.0)),
+        "rb_2d_walls": (2, 48, 32, 1, (400,), dict(rayleigh=1e5, vel_x=0, temp_x=1, thot=1.0,
+                                                   tcold=0.0)),
+        "side_2d_odd": (2, 33, 40, 1, (400,), dict(rayleigh=1e5, side_heated=1, vel_x=0,
+                                                  temp_x=0, temp_y=1, thot=1.0, tcold=0.0)),
+        "side_2d_even": (2, 32, 41, 1, (400,), dict(rayleigh=1e5, side_heated=1, vel_x=0,
+                                                   temp_x=0, temp_y=1, thot=1.0, tcold=0.0)),
+    }
+    dims, nx, ny, nz, steps, kw = runs[case]
+    sol, ora = _run_pair(dims, nx, ny, nz, steps, **kw)
+    _compare_midplane(sol, ora)
+
+
 def test_c2_128cubed_1000_steps():
     """128^3, Ra=1e6, Pr=0.71, FP64: relative max-norm <= 1e-12 after 1000 steps."""
     sol, ora = _run_pair(3, 128, 128, 128, (1000,), rayleigh=1e6)

@@ -322,6 +322,78 @@ def test_side_heated_hot_wall_minimum_window():
     assert d["nu_hot"] == pytest.approx(nu_left.mean(), rel=1e-12)
 
 
+def _velocity_state(cfg, u, v, w):
+    """f of the given velocity field at rho = 1, g at T = Tref (no buoyancy in the reload)."""
+    m = Model(cfg.dims, cfg.nx, cfg.ny, cfg.nz)
+    U = np.stack([u, v, w], axis=-1)
+    f = m.feq(np.ones(cfg.shape), U)
+    Tref = 0.5 * (cfg.thot + cfg.tcold)
+    g = m.geq(np.full(cfg.shape, Tref), np.zeros(cfg.shape + (3,)))
+    return f, g
+
+
+def test_midplane_maxima_3d_closed_form():
+    # u varies along y only, v along z only, w along x only, each a parabola whose vertex lies
+    # between grid points.  On its mid-plane every component is uniform along one in-plane axis,
+    # so the reported maximum is the grid point nearest the vertex and the first index (1) on
+    # the other axis (3DRB.F90:2399-2518); values are scaled by L / kappa.
+    cfg = orc.Config(dims=3, nx=12, ny=10, nz=9, rayleigh=1e5, mach=0.1)
+    p = orc.params(cfg)
+    L, coef = p["L"], p["L"] / p["diffusivity"]
+    xp, yp, zp = [(np.arange(n) + 0.5) / L for n in (cfg.nx, cfg.ny, cfg.nz)]
+    prof = lambda s, s0: 0.02 - 0.05 * (s - s0) ** 2
+    z3, y3, x3 = np.meshgrid(zp, yp, xp, indexing="ij")
+    u, v, w = prof(y3, yp[6] + 0.01), prof(z3, zp[2] - 0.02), prof(x3, xp[4] + 0.03)
+    zero = np.zeros(cfg.shape)
+    o = orc.Oracle(cfg)
+    o.init_exact()
+
+    def mid(uu, vv, ww):        # one component at a time: the others stay exactly zero
+        assert o.set_fg(*_velocity_state(cfg, uu, vv, ww)) == 0
+        return o.midplane()
+
+    m = mid(u, zero, zero)
+    assert m["u_plane"] == pytest.approx(0.5 * cfg.nx / L, rel=1e-15)
+    assert (m["u_pos1"], m["u_pos2"]) == (pytest.approx(yp[6]), pytest.approx(zp[0]))
+    assert m["u_max"] == pytest.approx(coef * u[0, 6, 0], rel=1e-12)
+    m = mid(zero, v, zero)
+    assert (m["v_pos1"], m["v_pos2"]) == (pytest.approx(xp[0]), pytest.approx(zp[2]))
+    assert m["v_max"] == pytest.approx(coef * v[2, 0, 0], rel=1e-12)
+    m = mid(zero, zero, w)
+    assert (m["w_pos1"], m["w_pos2"]) == (pytest.approx(xp[4]), pytest.approx(yp[0]))
+    assert m["w_max"] == pytest.approx(coef * w[0, 0, 4], rel=1e-12)
+    assert m["w_plane"] == pytest.approx(0.5 * cfg.nz / L, rel=1e-15)
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_midplane_maxima_2d_parabola_fit(side):
+    # 2D: the maxima come from a 5-point least-squares parabola, exact on parabolic profiles.
+    # The Rayleigh-Benard cell samples u on x = 0.5 (2DRB.F90:2497-2520) and folds the fitted
+    # position into the lower half (:2549-2550); the side-heated cell samples the middle
+    # column and does not fold (:2081-2153).  v is sampled on the middle row in both.
+    cfg = orc.Config(dims=2, nx=24, ny=20, rayleigh=1e5, mach=0.1, side_heated=side,
+                     temp_x=0 if side else 1, temp_y=1 if side else 0)
+    p = orc.params(cfg)
+    L, coef = p["L"], p["L"] / p["diffusivity"]
+    xp, yp = (np.arange(cfg.nx) + 0.5) / L, (np.arange(cfg.ny) + 0.5) / L
+    y0, x0 = 0.73 * cfg.ny / L, 0.41 * cfg.nx / L
+    yy, xx = np.meshgrid(yp, xp, indexing="ij")
+    u = (0.03 - 0.4 * (yy - y0) ** 2)[None]
+    v = (0.02 - 0.3 * (xx - x0) ** 2)[None]
+    f, g = _velocity_state(cfg, u, v, np.zeros_like(u))
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    assert o.set_fg(f, g) == 0
+    m = o.midplane()
+    ylen = cfg.ny / L
+    assert m["u_max"] == pytest.approx(0.03 * coef, rel=1e-9)
+    assert m["u_pos1"] == pytest.approx(y0 if side else ylen - y0, rel=1e-9)
+    assert m["v_max"] == pytest.approx(0.02 * coef, rel=1e-9)
+    assert m["v_pos1"] == pytest.approx(x0, rel=1e-9)
+    assert m["v_plane"] == pytest.approx(0.5 * (yp[cfg.ny // 2 - 1] + yp[cfg.ny // 2]))
+    assert m["u_plane"] == pytest.approx(0.5 * (xp[11] + xp[12]) if side else 0.5)
+
+
 def test_restart_reconstruction_roundtrip():
     cfg = orc.Config(dims=3, nx=7, ny=6, nz=5, rayleigh=1e5, mach=0.3)
     o = orc.Oracle(cfg)
