@@ -979,6 +979,104 @@ extern "C" int lbm_midplane_extrema(lbm_handle *h, lbm_midplane *m)
     return rc;
 }
 
+// ---- mid-plane slices: fields, stream function, vorticity (3D) ------------------------------
+// d/ds of a line q[0], q[st], ..., q[(n-1) st] with spacing h: central inside, second-order
+// one-sided at the ends, first order for n = 2, zero for n = 1 (3DRB.F90:2776-2805).
+static double line_derivative(const double *q, int n, ptrdiff_t st, int t, double h)
+{
+    if (n == 1) return 0.0;
+    if (t == 0) {
+        if (n == 2) return (q[st] - q[0]) / h;
+        return (-3.0 * q[0] + 4.0 * q[st] - q[2 * st]) / (2.0 * h);
+    }
+    if (t == n - 1) {
+        if (n == 2) return (q[t * st] - q[(t - 1) * st]) / h;
+        return (3.0 * q[t * st] - 4.0 * q[(t - 1) * st] + q[(t - 2) * st]) / (2.0 * h);
+    }
+    return (q[(t + 1) * st] - q[(t - 1) * st]) / (2.0 * h);
+}
+
+extern "C" int lbm_midplane_slice(lbm_handle *h, int32_t axis, double *us, double *vs, double *ws,
+                                  double *ts, double *psi, double *vort)
+{
+    if (!h || axis < 0 || axis > 2) return LBM_ERR_ARG;
+    if (h->cfg.dims != 3) return fail(h, LBM_ERR_UNSUPPORTED, "mid-plane slices are 3D");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc = refresh_macros(h);
+    if (rc) return rc;
+    const lbm_config &c = h->cfg;
+    const double L = h->p.L, coef = L / h->p.diffusivity;
+    const int n[3] = {c.nx, c.ny, c.nz};
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    const int n1 = n[a1], n2 = n[a2];
+    const size_t cnt = (size_t)n1 * n2;
+    std::vector<double> coord(n[axis] + 2, 0.0);
+    coord[n[axis] + 1] = (double)n[axis] / L;
+    for (int i = 1; i <= n[axis]; ++i) coord[i] = ((double)i - 0.5) / L;
+    int iL, iR;
+    double wgt;
+    bracket_index(coord, n[axis], 0.5 * coord[n[axis] + 1], &iL, &iR, &wgt);
+
+    // write_midplane_x/y/z (3DRB.F90:2611-2713): components scaled by L / kappa, T as is
+    std::vector<double> slice[5];
+    double *dplane = nullptr;
+    if (hipMalloc((void **)&dplane, cnt * sizeof(double)) != hipSuccess)
+        return fail(h, LBM_ERR_NOMEM, "mid-plane buffer");
+    for (int comp = 1; comp <= 4 && !rc; ++comp) {
+        slice[comp].resize(cnt);
+        hipError_t e = launch_plane(h->kp, h->stream, h->mac + (size_t)comp * h->N, axis, iL - 1,
+                                    iR - 1, wgt, 0, dplane);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(slice[comp].data(), dplane, cnt * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(h, LBM_ERR_HIP, "mid-plane slice: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(dplane);
+    if (rc) return rc;
+    double *outs[4] = {us, vs, ws, ts};
+    for (int comp = 1; comp <= 4; ++comp)
+        if (outs[comp - 1])
+            for (size_t t = 0; t < cnt; ++t)
+                outs[comp - 1][t] = comp == 4 ? slice[4][t] : coef * slice[comp][t];
+
+    // write_midplane_stream_x/y/z (3DRB.F90:2721-3011): psi integrates the component along the
+    // first in-plane axis over the second one from the wall; the vorticity is the plane-normal
+    // component (sign of the right-handed axis triple)
+    const std::vector<double> &c1 = slice[1 + a1], &c2 = slice[1 + a2];
+    const double d = 1.0 / L;
+    if (psi) {
+        for (int p1 = 0; p1 < n1; ++p1) {
+            const double *q = c1.data() + p1;   // stride n1 along the second axis
+            double *o = psi + p1;
+            if (n2 == 1) { o[0] = 0.0; continue; }
+            const double q1 = q[0] * coef, q2 = q[n1] * coef;
+            o[0] = d * (21.0 * q1 - q2) / 72.0;
+            for (int k = 2; k <= n2; ++k) {
+                const int m = k - 1;   // 1-based midpoint between m and m + 1
+                double mid;
+                if (n2 == 2) mid = 0.5 * (q[0] + q[n1]) * coef;
+                else if (m == 1)
+                    mid = (3.0 / 8.0 * q[0] + 3.0 / 4.0 * q[n1] - 1.0 / 8.0 * q[2 * n1]) * coef;
+                else
+                    mid = (-1.0 / 8.0 * q[(m - 2) * n1] + 3.0 / 4.0 * q[(m - 1) * n1] +
+                           3.0 / 8.0 * q[m * n1]) * coef;
+                const double inc = d / 6.0 * (q[(k - 2) * n1] * coef + 4.0 * mid + q[(k - 1) * n1] * coef);
+                o[(k - 1) * n1] = o[(k - 2) * n1] + inc;
+            }
+        }
+    }
+    if (vort) {
+        for (int p2 = 0; p2 < n2; ++p2)
+            for (int p1 = 0; p1 < n1; ++p1) {
+                const double dc2 = line_derivative(c2.data() + (size_t)p2 * n1, n1, 1, p1, d);
+                const double dc1 = line_derivative(c1.data() + p1, n2, n1, p2, d);
+                vort[p1 + (size_t)n1 * p2] = axis == 1 ? coef * (dc1 - dc2) : coef * (dc2 - dc1);
+            }
+    }
+    return LBM_OK;
+}
+
 // Chunk of z-planes whose AoS f and g fit a staging buffer of about 256 MiB.
 static int chunk_planes(const lbm_handle *h)
 {

@@ -29,7 +29,8 @@
 // loads overlaps them with the current flow collision.
 //
 // The including translation unit provides `Form` with the storage type S, arithmetic type R,
-// history type B and the per-node operations (pulls, macro, collisions) of its precision form.
+// history type B, a per-column constant record Node, and the per-node operations (pulls, the
+// pull of a neighbour's value exchanged through LDS, macro, collisions) of its precision form.
 #pragma once
 
 #include "lbm_lattice.h"
@@ -66,6 +67,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     const uint64_t N = kp.N;
     const uint64_t col = (uint64_t)i + (uint64_t)nx * (uint64_t)j;   // valid when in_dom
     const uint64_t plane = (uint64_t)nx * (uint64_t)ny;
+    const typename Form::Node nd = Form::node(kp, fp, i, j);          // per-column constants
 
     R fprev[19];
     R rho_p = 0, u_p = 0, v_p = 0, w_p = 0;
@@ -112,8 +114,8 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         if (have) {
             const uint64_t c = col + plane * (uint64_t)kk;
             const R Tn = Form::tsum(lg);
-            Form::macro(fnew, Tn, kp, fp, rho_n, u_n, v_n, w_n);
-            Form::thermal(lg, gnew, Tn, u_n, v_n, w_n, bp, kp, fp);
+            Form::macro(fnew, Tn, kp, fp, nd, rho_n, u_n, v_n, w_n);
+            Form::thermal(lg, gnew, Tn, u_n, v_n, w_n, bp, kp, fp, nd);
             if (tile && kk >= k0 && kk < k1) {
                 Form::store_g(Gout, N, c, gnew);
                 if (kp.use_g) {
@@ -137,15 +139,21 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
             // T^{n+1}(i, j, p): sum of the pulled post-wall g_post^n in direction order 0..6
             R gin[7];
             gin[0] = gprev[0];
-            gin[1] = i > 0 ? lds[rb][0][tid - 1] : Form::wall_g(1, gprev, kp.temp_x, fp);
-            gin[2] = i < nx - 1 ? lds[rb][1][tid + 1] : Form::wall_g(2, gprev, kp.temp_x, fp);
-            gin[3] = j > 0 ? lds[rb][2][tid - TX] : Form::wall_g(3, gprev, kp.temp_y, fp);
-            gin[4] = j < ny - 1 ? lds[rb][3][tid + TX] : Form::wall_g(4, gprev, kp.temp_y, fp);
-            gin[5] = (p > 0 && have_pp) ? g5pp : Form::wall_g(5, gprev, kp.temp_z, fp);
-            gin[6] = p < nz - 1 ? gnew[6] : Form::wall_g(6, gprev, kp.temp_z, fp);
+            gin[1] = i > 0 ? Form::pulled(1, lds[rb][0][tid - 1], fp)
+                           : Form::wall_g(1, gprev, kp.temp_x, fp, nd);
+            gin[2] = i < nx - 1 ? Form::pulled(2, lds[rb][1][tid + 1], fp)
+                                : Form::wall_g(2, gprev, kp.temp_x, fp, nd);
+            gin[3] = j > 0 ? Form::pulled(3, lds[rb][2][tid - TX], fp)
+                           : Form::wall_g(3, gprev, kp.temp_y, fp, nd);
+            gin[4] = j < ny - 1 ? Form::pulled(4, lds[rb][3][tid + TX], fp)
+                                : Form::wall_g(4, gprev, kp.temp_y, fp, nd);
+            gin[5] = (p > 0 && have_pp) ? Form::pulled(5, g5pp, fp)
+                                        : Form::wall_g(5, gprev, kp.temp_z, fp, nd);
+            gin[6] = p < nz - 1 ? Form::pulled(6, gnew[6], fp)
+                                : Form::wall_g(6, gprev, kp.temp_z, fp, nd);
             const R T1 = Form::tsum(gin);
             R out[19];
-            Form::flow(fprev, out, rho_p, u_p, v_p, w_p, T1, kp, fp);
+            Form::flow(fprev, out, rho_p, u_p, v_p, w_p, T1, kp, fp, nd);
             Form::store_f(Fnext, N, col + plane * (uint64_t)p, out);
         }
         // shift the plane pipeline

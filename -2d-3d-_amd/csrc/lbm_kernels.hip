@@ -774,6 +774,10 @@ struct FormF64 {
     struct P {
         double thot, tcold;
     };
+    struct Node {};
+    static __device__ __forceinline__ Node node(const KParams &, const P &, int, int) { return Node{}; }
+    // a neighbour's post-collision population of direction a pulled into this node: unchanged
+    static __device__ __forceinline__ double pulled(int, double v, const P &) { return v; }
     static __device__ __forceinline__ void pull_g(const double *G, uint64_t N, int i, int j, int k,
                                                   uint64_t c, bool border, const KParams &kp,
                                                   const P &, double *g)
@@ -791,8 +795,8 @@ struct FormF64 {
         return g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
     }
     static __device__ __forceinline__ void macro(const double *f, double Tn, const KParams &kp,
-                                                 const P &, double &rho, double &u, double &v,
-                                                 double &w)
+                                                 const P &, const Node &, double &rho, double &u,
+                                                 double &v, double &w)
     {
         rho = sum19(f);
         const double Fy = rho * kp.gbeta * (Tn - kp.tref);
@@ -802,13 +806,13 @@ struct FormF64 {
     }
     static __device__ __forceinline__ void thermal(const double *g, double *p, double T, double u,
                                                    double v, double w, double *bp,
-                                                   const KParams &kp, const P &)
+                                                   const KParams &kp, const P &, const Node &)
     {
         ref_collide_d3q7(g, p, T, u, v, w, bp, kp);
     }
     static __device__ __forceinline__ void flow(const double *f, double *out, double rho, double u,
                                                 double v, double w, double T1, const KParams &kp,
-                                                const P &)
+                                                const P &, const Node &)
     {
         ref_collide_d3q19(f, out, rho, u, v, w, T1, kp);
     }
@@ -823,7 +827,8 @@ struct FormF64 {
         for (int a = 0; a < 19; ++a) F[a * N + c] = f[a];
     }
     // Wall value of thermal direction a at the node itself (adiabatic or constant T).
-    static __device__ __forceinline__ double wall_g(int a, const double *gp, int kind, const P &p)
+    static __device__ __forceinline__ double wall_g(int a, const double *gp, int kind, const P &p,
+                                                    const Node &)
     {
         const double back = gp[D3Q7::opp[a]];
         if (kind == T_ADIABATIC) return back;

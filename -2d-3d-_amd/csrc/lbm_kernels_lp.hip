@@ -2,7 +2,15 @@
 //
 // Populations are stored as deviations from the rest equilibrium, which keeps their magnitude
 // near the signal and away from the constant weight:
-//   h_a = f_a - w_a rho0 (rho0 = 1)            k_a = g_a - wT_a T0 (T0 = Tref)
+//   h_a = f_a - w_a rho0 (rho0 = 1)            k_a = g_a - wT_a T0(x)
+// T0(x) is the conduction profile between the two isothermal walls (linear along the wall
+// normal; a constant when no axis has two isothermal walls).  Near a wall the temperature then
+// differs from T0 by the thermal boundary layer's deviation only, so the few-ulp rounding of a
+// stored population is measured against that deviation instead of against T itself: the wall
+// heat flux (a difference of nearly equal temperatures) keeps its precision.  T0 is a fixed
+// linear function whose slope and offset carry few mantissa bits, so T0 and every difference
+// of T0 between neighbours are exact in FP32; a pulled deviation is corrected by
+// wT_a (T0(x - e_a) - T0(x)) = -wT_a (e_a . n) s, exactly (wT_a is a power of two).
 // LBM_FP32_SHIFT stores h, k as float; LBM_FP16_SHIFT stores h_a / w_a and k_a / wT_a as
 // _Float16 (per-direction scaling puts every direction in the same numeric range).  All
 // collision arithmetic is FP32 (FMA contraction on); the history B_prev and the stored previous
@@ -46,9 +54,24 @@ template <> struct Pop<_Float16> {
 
 struct LPParams {
     float gbeta, sv, sq, qk, qnu, sg;
-    float dthot, dtcold;        // wall temperatures minus T0
-    double tref;                // T0
+    float thot, tcold;          // wall temperatures
+    float t0a, t0s;             // storage shift T0 = t0a + t0s * (node index along t0axis)
+    int t0axis;                 // 0: T0 = t0a everywhere; 1: along x; 2: along y
+    double tref;                // reference temperature of the buoyancy force
 };
+
+// Storage shift T0 at node (i, j) (exact in FP32, see above).
+__device__ __forceinline__ float t0_of(const LPParams &lp, int i, int j)
+{
+    return lp.t0axis == 1 ? lp.t0a + lp.t0s * (float)i
+                          : (lp.t0axis == 2 ? lp.t0a + lp.t0s * (float)j : lp.t0a);
+}
+// Correction of a pulled thermal deviation of direction A: wT_A (T0(x - e_A) - T0(x)).
+template <int A> __device__ __forceinline__ float t0_pull(const LPParams &lp)
+{
+    const int e = lp.t0axis == 1 ? D3Q7::ex[A] : (lp.t0axis == 2 ? D3Q7::ey[A] : 0);
+    return -(float)w7(A) * ((float)e * lp.t0s);
+}
 
 // -------------------------------------------------------------------------------------------
 // pulls (walls as in lbm_kernels.hip: last outside axis decides for the flow lattice)
@@ -84,7 +107,8 @@ __device__ __forceinline__ float heat_in(const S *__restrict__ P, uint64_t N, in
 {
     constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
     const int sy = kp.nx, sz = kp.nx * kp.ny;
-    if (!border) return Pop<S>::template ld<L::Q, A>(P + A * N + c - (ex + ey * sy + ez * sz));
+    if (!border)
+        return Pop<S>::template ld<L::Q, A>(P + A * N + c - (ex + ey * sy + ez * sz)) + t0_pull<A>(lp);
     int kind = -1;
     int64_t wrap = 0;
     if (ex != 0) {
@@ -94,12 +118,15 @@ __device__ __forceinline__ float heat_in(const S *__restrict__ P, uint64_t N, in
     } else if (ez != 0) {
         if ((unsigned)(k - ez) >= (unsigned)kp.nz) { kind = kp.temp_z; wrap = (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz; }
     }
-    if (kind < 0) return Pop<S>::template ld<L::Q, A>(P + A * N + c - (int64_t)(ex + ey * sy + ez * sz));
-    if (kind == T_PERIODIC) return Pop<S>::template ld<L::Q, A>(P + A * N + c + wrap);
+    if (kind < 0)
+        return Pop<S>::template ld<L::Q, A>(P + A * N + c - (int64_t)(ex + ey * sy + ez * sz)) +
+               t0_pull<A>(lp);
+    // a periodic pair of walls is never the shift axis (it has two isothermal walls)
+    if (kind == T_PERIODIC) return Pop<S>::template ld<L::Q, A>(P + A * N + c + wrap) + t0_pull<A>(lp);
     const float back = Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
     if (kind == T_ADIABATIC) return back;
     constexpr bool low_wall = (ex + ey + ez) > 0;
-    const float tw = low_wall ? lp.dthot : lp.dtcold;
+    const float tw = (low_wall ? lp.thot : lp.tcold) - t0_of(lp, i, j);
     return -back + 2.0f * (float)wq<L::Q>(A) * tw;
 }
 
@@ -324,10 +351,11 @@ __global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, cons
     if (!node_of_lp(kp, i, j, k, c)) return;
     const uint64_t N = kp.N;
     const bool border = on_border_lp(i, j, k, kp);
-    float g[7], dT;
+    const float t0 = t0_of(lp, i, j), b0 = (float)((double)t0 - lp.tref);
+    float g[7], dT;   // dT = T - T0(x)
     if (FIRST) {
         load_all<D3Q7, S>(Gin, N, c, g, SEQ(D3Q7));
-        dT = (float)(mac[4 * N + c] - lp.tref);
+        dT = (float)(mac[4 * N + c] - (double)t0);
     } else {
         pull_heat_lp<D3Q7, S>(Gin, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
         dT = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
@@ -338,18 +366,19 @@ __global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, cons
     moments19(f, drho, jx, jy, jz);
     const float rho = 1.0f + drho;
     const float ir = 1.0f / rho;
-    const float Fy = rho * lp.gbeta * dT;
+    const float bT = b0 + dT;   // T - Tref
+    const float Fy = rho * lp.gbeta * bT;
     const float u = jx * ir, v = (jy + 0.5f * Fy) * ir, w = jz * ir;
     float bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
     float p[7];
-    collide7_dev(g, p, dT, (float)lp.tref + dT, u, v, w, bp, kp.use_g, lp);
+    collide7_dev(g, p, dT, t0 + dT, u, v, w, bp, kp.use_g, lp);
     store_all<D3Q7, S>(Gout, N, c, p, SEQ(D3Q7));
     if (kp.use_g) {
         Bp[c] = bp[0];
         Bp[N + c] = bp[1];
         Bp[2 * N + c] = bp[2];
     }
-    aux[c] = dT;
+    aux[c] = bT;
 }
 
 template <typename S>
@@ -365,6 +394,7 @@ __global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S
     float g[7];
     pull_heat_lp<D3Q7, S>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
     const float dT1 = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
+    const float b0 = (float)((double)t0_of(lp, i, j) - lp.tref);
     float f[19];
     pull_flow_lp<D3Q19, S>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
     float drho, jx, jy, jz;
@@ -373,7 +403,7 @@ __global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S
     const float ir = 1.0f / rho;
     const float Fy = rho * lp.gbeta * aux[c];
     const float u = jx * ir, v = (jy + 0.5f * Fy) * ir, w = jz * ir;
-    collide19_dev(f, f, drho, u, v, w, dT1, lp);
+    collide19_dev(f, f, drho, u, v, w, b0 + dT1, lp);
     store_all<D3Q19, S>(Fnext, N, c, f, SEQ(D3Q19));
 }
 
@@ -404,29 +434,32 @@ __global__ void __launch_bounds__(256) lp3_state(KParams kp, LPParams lp, const 
 #pragma unroll
     for (int a = 0; a < 7; ++a) dT += (double)g[a];
     const double rho = 1.0 + drho;
+    const float t0 = t0_of(lp, i, j);
     if (mac) {
-        // dT^n of the half force: stored by the two-pass form, else pull-sum of g_post^{n-1}
-        float dTn;
+        // T^n - Tref of the half force: stored by the two-pass form, else the pull-sum of
+        // g_post^{n-1}
+        float bTn;
         if (aux) {
-            dTn = aux[c];
+            bTn = aux[c];
         } else {
             float gq[7];
             pull_heat_lp<D3Q7, S>(Gprev, N, i, j, k, c, border, kp, lp, gq, SEQ(D3Q7));
-            dTn = (gq[0] + gq[1]) + (gq[2] + gq[3]) + (gq[4] + gq[5]) + gq[6];
+            bTn = (float)((double)t0 - lp.tref) +
+                  ((gq[0] + gq[1]) + (gq[2] + gq[3]) + (gq[4] + gq[5]) + gq[6]);
         }
-        const double Fy = rho * (double)lp.gbeta * (double)dTn;
+        const double Fy = rho * (double)lp.gbeta * (double)bTn;
         mac[c] = rho;
         mac[N + c] = jx / rho;
         mac[2 * N + c] = (jy + 0.5 * Fy) / rho;
         mac[3 * N + c] = jz / rho;
-        mac[4 * N + c] = lp.tref + dT;
+        mac[4 * N + c] = (double)t0 + dT;
     }
     if (fo && k >= k0 && k < k0 + nk) {
         const uint64_t r = c - (uint64_t)k0 * kp.nx * kp.ny;
 #pragma unroll
         for (int a = 0; a < 19; ++a) fo[r * 19 + a] = w19(a) + (double)f[a];
 #pragma unroll
-        for (int a = 0; a < 7; ++a) go[r * 7 + a] = w7(a) * lp.tref + (double)g[a];
+        for (int a = 0; a < 7; ++a) go[r * 7 + a] = w7(a) * (double)t0 + (double)g[a];
     }
 }
 
@@ -440,9 +473,9 @@ __global__ void __launch_bounds__(256) lp_init(KParams kp, LPParams lp, S *__res
     if (!node_of_lp(kp, i, j, k, c)) return;
     const uint64_t N = kp.N;
     float f[19] = {}, g[7];
-    const double T = mac[4 * N + c];
+    const double T = mac[4 * N + c], t0 = (double)t0_of(lp, i, j);
 #pragma unroll
-    for (int a = 0; a < 7; ++a) g[a] = (float)(w7(a) * T - w7(a) * lp.tref);
+    for (int a = 0; a < 7; ++a) g[a] = (float)(w7(a) * T - w7(a) * t0);
     store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
     store_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
     Bp[c] = (float)(mac[N + c] * T);
@@ -458,11 +491,13 @@ __global__ void lp_from_abs(KParams kp, LPParams lp, const double *__restrict__ 
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const uint64_t c = c0 + r, N = kp.N;
+    const double t0 = (double)t0_of(lp, (int)(c % (uint64_t)kp.nx),
+                                    (int)((c / (uint64_t)kp.nx) % (uint64_t)kp.ny));
     float f[19], g[7];
 #pragma unroll
     for (int a = 0; a < 19; ++a) f[a] = (float)(Fd[r * 19 + a] - w19(a));
 #pragma unroll
-    for (int a = 0; a < 7; ++a) g[a] = (float)(Gd[r * 7 + a] - w7(a) * lp.tref);
+    for (int a = 0; a < 7; ++a) g[a] = (float)(Gd[r * 7 + a] - w7(a) * t0);
     store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
     store_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
 }
@@ -478,10 +513,12 @@ __global__ void lp_to_abs(KParams kp, LPParams lp, const S *__restrict__ F, cons
     float f[19], g[7];
     load_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
     load_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
+    const double t0 = (double)t0_of(lp, (int)(c % (uint64_t)kp.nx),
+                                    (int)((c / (uint64_t)kp.nx) % (uint64_t)kp.ny));
 #pragma unroll
     for (int a = 0; a < 19; ++a) Fd[r * 19 + a] = w19(a) + (double)f[a];
 #pragma unroll
-    for (int a = 0; a < 7; ++a) Gd[r * 7 + a] = w7(a) * lp.tref + (double)g[a];
+    for (int a = 0; a < 7; ++a) Gd[r * 7 + a] = w7(a) * t0 + (double)g[a];
 }
 
 // reconstruct_macro_from_fg (2DRB/3DRB.F90:1392-1463) on an absolute FP64 AoS chunk.
@@ -540,6 +577,21 @@ template <typename SS> struct FormLP {
     using S = SS;
     using B = float;
     using P = LPParams;
+    // per-column constants: the storage shift T0 and T0 - Tref
+    struct Node {
+        float t0, b0;
+    };
+    static __device__ __forceinline__ Node node(const KParams &, const P &lp, int i, int j)
+    {
+        const float t0 = t0_of(lp, i, j);
+        return Node{t0, (float)((double)t0 - lp.tref)};
+    }
+    // a neighbour's post-collision deviation of direction a pulled into this node
+    static __device__ __forceinline__ float pulled(int a, float v, const P &lp)
+    {
+        const int e = lp.t0axis == 1 ? D3Q7::ex[a] : (lp.t0axis == 2 ? D3Q7::ey[a] : 0);
+        return v + -(float)w7(a) * ((float)e * lp.t0s);
+    }
     static __device__ __forceinline__ void pull_g(const S *G, uint64_t N, int i, int j, int k,
                                                   uint64_t c, bool border, const KParams &kp,
                                                   const P &lp, float *g)
@@ -552,36 +604,36 @@ template <typename SS> struct FormLP {
     {
         pull_flow_lp<D3Q19, S>(F, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
     }
-    // temperature deviation dT = T - T0
+    // temperature deviation dT = T - T0(x)
     static __device__ __forceinline__ float tsum(const float *g)
     {
         return (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
     }
     // returns drho (density deviation) in the rho slot
     static __device__ __forceinline__ void macro(const float *f, float dTn, const KParams &,
-                                                 const P &lp, float &drho, float &u, float &v,
-                                                 float &w)
+                                                 const P &lp, const Node &nd, float &drho, float &u,
+                                                 float &v, float &w)
     {
         float jx, jy, jz;
         moments19(f, drho, jx, jy, jz);
         const float rho = 1.0f + drho;
         const float ir = 1.0f / rho;
-        const float Fy = rho * lp.gbeta * dTn;
+        const float Fy = rho * lp.gbeta * (nd.b0 + dTn);
         u = jx * ir;
         v = (jy + 0.5f * Fy) * ir;
         w = jz * ir;
     }
     static __device__ __forceinline__ void thermal(const float *g, float *p, float dT, float u,
                                                    float v, float w, float *bp, const KParams &kp,
-                                                   const P &lp)
+                                                   const P &lp, const Node &nd)
     {
-        collide7_dev(g, p, dT, (float)lp.tref + dT, u, v, w, bp, kp.use_g, lp);
+        collide7_dev(g, p, dT, nd.t0 + dT, u, v, w, bp, kp.use_g, lp);
     }
     static __device__ __forceinline__ void flow(const float *f, float *out, float drho, float u,
                                                 float v, float w, float dT1, const KParams &,
-                                                const P &lp)
+                                                const P &lp, const Node &nd)
     {
-        collide19_dev(f, out, drho, u, v, w, dT1, lp);
+        collide19_dev(f, out, drho, u, v, w, nd.b0 + dT1, lp);
     }
     static __device__ __forceinline__ void store_g(S *G, uint64_t N, uint64_t c, const float *g)
     {
@@ -591,11 +643,12 @@ template <typename SS> struct FormLP {
     {
         store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
     }
-    static __device__ __forceinline__ float wall_g(int a, const float *gp, int kind, const P &lp)
+    static __device__ __forceinline__ float wall_g(int a, const float *gp, int kind, const P &lp,
+                                                   const Node &nd)
     {
         const float back = gp[D3Q7::opp[a]];
         if (kind == T_ADIABATIC) return back;
-        const float tw = (a & 1) ? lp.dthot : lp.dtcold;
+        const float tw = ((a & 1) ? lp.thot : lp.tcold) - nd.t0;
         return -back + 2.0f * (float)w7(a) * tw;
     }
 };
@@ -619,8 +672,29 @@ static LPParams lp_params(const KParams &kp)
     lp.qnu = (float)kp.qnu;
     lp.sg = (float)kp.sg;
     lp.tref = kp.tref;
-    lp.dthot = (float)(kp.thot - kp.tref);
-    lp.dtcold = (float)(kp.tcold - kp.tref);
+    lp.thot = (float)kp.thot;
+    lp.tcold = (float)kp.tcold;
+    // storage shift: the conduction profile between two isothermal walls, y before x
+    lp.t0axis = kp.temp_y == T_CONST ? 2 : (kp.temp_x == T_CONST ? 1 : 0);
+    if (lp.t0axis == 0) {
+        lp.t0a = (float)kp.tref;
+        lp.t0s = 0.0f;
+    } else {
+        const int n = lp.t0axis == 2 ? kp.ny : kp.nx;
+        const double s = (kp.tcold - kp.thot) / n;   // per node, from the low (hot) wall
+        if (s == 0.0) {
+            lp.t0a = (float)kp.thot;
+            lp.t0s = 0.0f;
+        } else {
+            // 10 significant bits for the slope and the offset on the slope's last bit: every
+            // T0 = t0a + t0s * idx (idx < 2^13) and its differences are exact in FP32
+            int e;
+            frexp(s, &e);
+            const double q = ldexp(1.0, e - 10);
+            lp.t0s = (float)(nearbyint(s / q) * q);
+            lp.t0a = (float)(nearbyint((kp.thot + 0.5 * (double)lp.t0s) / q) * q);
+        }
+    }
     return lp;
 }
 

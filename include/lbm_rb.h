@@ -124,6 +124,14 @@ int lbm_diagnostics(lbm_handle *h, lbm_diag *d);
  * SideHeatedcalc_umid_max / SideHeatedcalc_vmid_max (:2081-2238): a 5-point least-squares
  * parabola around the grid maximum of the sampled line.  Needs nx, ny >= 5. */
 int lbm_midplane_extrema(lbm_handle *h, lbm_midplane *m);
+/* 3D mid-plane slice across `axis` (0: x, 1: y, 2: z) at half the domain extent:
+ * u, v, w scaled by L / kappa and T, interpolated onto the plane (write_midplane_x/y/z,
+ * 2DRB/3DRB.F90:2611-2713), and the slice's stream function and plane-normal vorticity
+ * (write_midplane_stream_x/y/z, :2721-3011: psi integrates the first in-plane axis's velocity
+ * along the second axis from the wall).  Each output holds n1 * n2 values, first in-plane axis
+ * fastest: x-plane (y, z), y-plane (x, z), z-plane (x, y).  Any output may be NULL. */
+int lbm_midplane_slice(lbm_handle *h, int32_t axis, double *u, double *v, double *w, double *T,
+                       double *psi, double *vort);
 
 /* f, g (reference AoS order, absolute FP64) and, if non-NULL, B_prev (3 planes of nx*ny*nz:
  * Bx_prev, By_prev, Bz_prev, 2DRB/3DRB.F90:163).  Restart data of backupData

@@ -89,3 +89,40 @@ def test_lowprec_bytes_per_update():
         assert sol.bytes_per_update() == b
     with pytest.raises(rblbm.LbmError):
         rblbm.Solver(rblbm.Config(dims=2, nx=8, ny=8, nz=1, precision=rblbm.FP32_SHIFT))
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+def test_lowprec_conduction_state_keeps_unit_wall_nusselt(prec):
+    # The reduced-precision forms store deviations from the conduction profile between the
+    # isothermal walls, so the conduction state is held to rounding level even near the walls,
+    # where the wall Nusselt number differences nearly equal temperatures (amplified by L / 3).
+    # Ra far below onset and a small Mach number keep the discrete hydrostatic flow negligible.
+    cfg = rblbm.Config(dims=3, nx=16, ny=48, nz=8, rayleigh=0.01, mach=0.001, precision=prec)
+    sol = rblbm.Solver(cfg)
+    y = (np.arange(cfg.ny) + 0.5) / cfg.ny
+    T0 = np.broadcast_to((cfg.thot - y * (cfg.thot - cfg.tcold))[None, :, None], cfg.shape)
+    sol.initial(np.ascontiguousarray(T0))
+    sol.step(3000)
+    d = sol.diagnostics()
+    tol = 1e-5 if prec == rblbm.FP32_SHIFT else 1e-3
+    for k in ("nu_hot", "nu_cold", "nu_global", "nu_middle"):
+        assert d[k] == pytest.approx(1.0, abs=tol), (k, d[k])
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+def test_lowprec_wall_nusselt_tracks_fp64(prec):
+    # Convecting cavity, 64^3, Ra = 1e7: the wall Nusselt numbers of the reduced-precision form
+    # follow the FP64 form (bitwise equal to the reference's arithmetic) over 1500 steps.
+    kw = dict(dims=3, nx=64, ny=64, nz=64, rayleigh=1e7)
+    ref = rblbm.Solver(rblbm.Config(**kw))
+    sol = rblbm.Solver(rblbm.Config(precision=prec, **kw))
+    for s in (ref, sol):
+        s.initial_seeded(11, 1e-3)
+    for n in (500, 1000):
+        ref.step(n)
+        sol.step(n)
+        d, e = sol.diagnostics(), ref.diagnostics()
+        print(prec, n, {k: (d[k], e[k]) for k in ("nu_hot", "nu_cold", "nu_vol")}, flush=True)
+        tol = 2e-4 if prec == rblbm.FP32_SHIFT else 2e-2
+        for k in ("nu_hot", "nu_cold"):
+            assert d[k] == pytest.approx(e[k], rel=tol), (n, k, d[k], e[k])
