@@ -93,6 +93,13 @@ module lbm_rb
       type(c_ptr), value :: h
       type(lbm_midplane), intent(out) :: m
     end function
+    integer(c_int) function lbm_midplane_slice(h, axis, u, v, w, t, psi, vort) &
+         bind(C, name="lbm_midplane_slice")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: axis
+      type(c_ptr), value :: u, v, w, t, psi, vort
+    end function
     integer(c_int) function lbm_gather(h, f, g, bprev) bind(C, name="lbm_gather")
       import :: c_int, c_ptr
       type(c_ptr), value :: h, f, g, bprev

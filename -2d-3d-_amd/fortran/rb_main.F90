@@ -4,8 +4,9 @@
 !
 !   initial   -> lbm_create + lbm_init_exact / lbm_init_seeded   (or restart: read f, g + scatter)
 !   time loop -> lbm_step (the eight sweeps), check every checkInterval steps
-!   end       -> lbm_diagnostics (RBcalc_Nu_global, RBcalc_Nu_wall_avg, calNuRe), MLUPS log,
-!                optional output_binary / backupData files in the reference's formats
+!   end       -> lbm_diagnostics (RBcalc_Nu_global, RBcalc_Nu_wall_avg, calNuRe), the mid-plane
+!                velocity maxima, MLUPS log, optional output_binary / backupData / Tecplot files
+!                in the reference's formats
 !
 ! Parameters keep the reference's names (module commondata3d) and defaults; a namelist file
 ! given as the first command-line argument overrides them at run time:
@@ -16,6 +17,7 @@
 program rb_main
   use iso_c_binding
   use lbm_rb
+  use rb_tecplot
   implicit none
 
   ! ---- commondata3d ------------------------------------------------------------------------
@@ -38,11 +40,14 @@ program rb_main
   integer :: loadInitField = 0
   character(len=256) :: reloadFile = 'backupFile3D-0.bin'
   integer :: outputBinFile = 0
+  integer :: outputPltFile = 0
+  character(len=100) :: pltFolderPrefix = ''
   integer :: backupAtEnd = 0
   character(len=256) :: settingsFile = 'SimulationSettings3D.txt'
   namelist /rb/ dims, nx, ny, nz, Rayleigh, Prandtl, Mach, Thot, Tcold, itc_max, checkInterval, &
        epsU, epsT, sideHeated, velX, velZ, tempX, tempY, tempZ, useG, precision, device, &
-       seeded, seed, noiseAmp, loadInitField, reloadFile, outputBinFile, backupAtEnd, settingsFile
+       seeded, seed, noiseAmp, loadInitField, reloadFile, outputBinFile, backupAtEnd, settingsFile, &
+       outputPltFile, pltFolderPrefix
 
   type(lbm_config) :: cfg
   type(lbm_params) :: prm
@@ -66,6 +71,8 @@ program rb_main
     close(10)
   endif
   if (dims == 2) nz = 1
+  if (len_trim(pltFolderPrefix) == 0) &
+    pltFolderPrefix = merge('buoyancyCavity3DTecplot', 'buoyancyCavityTecplot  ', dims == 3)
   qf = merge(19, 9, dims == 3)
   qt = merge(7, 5, dims == 3)
 
@@ -93,6 +100,7 @@ program rb_main
   call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
   call lbm_require(lbm_midplane_extrema(h, mp), 'lbm_midplane_extrema', h, 6)
   if (outputBinFile == 1) call output_binary()
+  if (outputPltFile == 1) call output_Tecplot()
   if (backupAtEnd == 1) call backupData()
 
   write(*,'(a,1x,es16.8)') 'Nu_global =', dg%nu_global
@@ -231,6 +239,90 @@ contains
     write(03) rho
     close(03)
   end subroutine output_binary
+
+  !------------------------------------------------------------------------------------------
+  ! output_Tecplot (2DRB/3DRB.F90:1676-1701; 2D 2DRB/2DRB.F90:1443-1570).  3D: the full field
+  ! (X, Y, Z, U, V, W, T with lattice velocities) and, on each mid-plane, the fields (velocities
+  ! scaled by L / kappa) and the stream function and vorticity.  2D: X, Y, U_nd, V_nd, T.
+  !------------------------------------------------------------------------------------------
+  subroutine output_Tecplot()
+    real(8), allocatable, target :: rho(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:), T(:,:,:)
+    real(8), allocatable, target :: s(:,:,:)
+    real(8), allocatable :: dat(:,:), c1(:), c2(:)
+    character(len=12) :: tag
+    character(len=1), parameter :: ax(3) = (/ 'X', 'Y', 'Z' /)
+    character(len=2), parameter :: pl(3) = (/ 'YZ', 'XZ', 'XY' /)
+    real(8) :: scale
+    integer :: i, j, k, p, a, n1, n2
+    allocate(rho(nx,ny,nz), u(nx,ny,nz), v(nx,ny,nz), w(nx,ny,nz), T(nx,ny,nz))
+    call lbm_require(lbm_macros(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(w), c_loc(T)), &
+         'lbm_macros', h, 6)
+    scale = prm%L / prm%diffusivity
+    write(tag,'(i12.12)') itc
+    if (dims == 2) then
+      allocate(dat(5, nx*ny))
+      do j = 1, ny
+        do i = 1, nx
+          p = i + nx * (j - 1)
+          dat(:, p) = (/ coordinate(i), coordinate(j), scale * u(i,j,1), scale * v(i,j,1), &
+                         T(i,j,1) /)
+        enddo
+      enddo
+      call plt_write(trim(pltFolderPrefix)//'-'//tag//'.plt', 'MyFirst', &
+           (/ 'X   ', 'Y   ', 'U_nd', 'V_nd', 'T   ' /), nx, ny, 1, dat)
+      return
+    endif
+    allocate(dat(7, nx*ny*nz))
+    do k = 1, nz
+      do j = 1, ny
+        do i = 1, nx
+          p = i + nx * (j - 1 + ny * (k - 1))
+          dat(:, p) = (/ coordinate(i), coordinate(j), coordinate(k), &
+                         u(i,j,k), v(i,j,k), w(i,j,k), T(i,j,k) /)
+        enddo
+      enddo
+    enddo
+    call plt_write(trim(pltFolderPrefix)//'-fullField-'//tag//'.plt', 'full-field-3d', &
+         (/ 'X', 'Y', 'Z', 'U', 'V', 'W', 'T' /), nx, ny, nz, dat)
+    deallocate(dat)
+    do a = 1, 3
+      ! in-plane axes: x-plane (y, z), y-plane (x, z), z-plane (x, y)
+      n1 = merge(ny, nx, a == 1)
+      n2 = merge(ny, nz, a == 3)
+      allocate(s(n1, n2, 6), dat(6, n1*n2), c1(n1), c2(n2))
+      call lbm_require(lbm_midplane_slice(h, int(a - 1, c_int32_t), c_loc(s(1,1,1)), &
+           c_loc(s(1,1,2)), c_loc(s(1,1,3)), c_loc(s(1,1,4)), c_loc(s(1,1,5)), c_loc(s(1,1,6))), &
+           'lbm_midplane_slice', h, 6)
+      do i = 1, n1
+        c1(i) = coordinate(i)
+      enddo
+      do j = 1, n2
+        c2(j) = coordinate(j)
+      enddo
+      do j = 1, n2
+        do i = 1, n1
+          dat(:, i + n1 * (j - 1)) = (/ c1(i), c2(j), s(i,j,1), s(i,j,2), s(i,j,3), s(i,j,4) /)
+        enddo
+      enddo
+      call plt_write(trim(pltFolderPrefix)//'-mid'//ax(a)//'-'//tag//'.plt', 'mid'//ax(a)//'-fields', &
+           (/ pl(a)(1:1)//'   ', pl(a)(2:2)//'   ', 'U_nd', 'V_nd', 'W_nd', 'T   ' /), n1, n2, 1, dat)
+      do j = 1, n2
+        do i = 1, n1
+          dat(1:4, i + n1 * (j - 1)) = (/ c1(i), c2(j), s(i,j,5), s(i,j,6) /)
+        enddo
+      enddo
+      call plt_write(trim(pltFolderPrefix)//'-mid'//ax(a)//'psiVort-'//tag//'.plt', &
+           'mid'//ax(a)//'-psi-vort', (/ pl(a)(1:1)//'     ', pl(a)(2:2)//'     ', &
+           'PSI_'//pl(a), 'VORT_'//ax(a) /), n1, n2, 1, dat(1:4, :))
+      deallocate(s, dat, c1, c2)
+    enddo
+  end subroutine output_Tecplot
+
+  ! node coordinate (i - 1/2) / L of node i (2DRB/3DRB.F90:443-462)
+  real(8) function coordinate(i)
+    integer, intent(in) :: i
+    coordinate = (dble(i) - 0.5d0) / prm%L
+  end function coordinate
 
   !------------------------------------------------------------------------------------------
   ! backupData (2DRB/3DRB.F90:1638-1665): f then g, population index outermost

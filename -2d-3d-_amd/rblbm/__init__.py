@@ -264,6 +264,22 @@ class Solver:
         self._chk(self.lib.lbm_diagnostics(self._h, C.byref(d)))
         return {n: getattr(d, n) for n, _ in _Diag._fields_}
 
+    def midplane_slice(self, axis: int) -> dict:
+        """3D mid-plane slice across axis 0/1/2 (lbm_midplane_slice): u, v, w scaled by
+        L / kappa, T, stream function and vorticity, each as (n2, n1) arrays."""
+        n = (self.cfg.nx, self.cfg.ny, self.cfg.nz)
+        n1 = n[1] if axis == 0 else n[0]
+        n2 = n[1] if axis == 2 else n[2]
+        out = {k: np.empty((n2, n1)) for k in ("u", "v", "w", "T", "psi", "vort")}
+        L = self.lib
+        if not getattr(L, "_slice_bound", False):
+            L.lbm_midplane_slice.argtypes = [C.c_void_p, C.c_int32] + [C.POINTER(C.c_double)] * 6
+            L.lbm_midplane_slice.restype = C.c_int
+            L._slice_bound = True
+        self._chk(L.lbm_midplane_slice(self._h, int(axis), *[_p(out[k]) for k in
+                                                            ("u", "v", "w", "T", "psi", "vort")]))
+        return out
+
     def midplane(self) -> dict:
         """Mid-plane velocity maxima (scaled by L / kappa) and their positions."""
         m = _Mid()

@@ -85,6 +85,7 @@ def lib():
         L.orc_check.argtypes = [C.c_void_p, dp, dp]
         L.orc_diagnostics.argtypes = [C.c_void_p, C.POINTER(_Diag)]
         L.orc_midplane.argtypes = [C.c_void_p, C.POINTER(_Mid)]
+        L.orc_midplane_slice.argtypes = [C.c_void_p, C.c_int] + [dp] * 6
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = C.c_int
         L.orc_fit_adiabatic_wall_T4.argtypes = [C.c_double, dp, dp, dp]
@@ -224,6 +225,16 @@ class Oracle:
         d = _Diag()
         lib().orc_diagnostics(self.h, C.byref(d))
         return {n: getattr(d, n) for n, _ in _Diag._fields_}
+
+    def midplane_slice(self, axis: int) -> dict:
+        """3D mid-plane slice across axis 0/1/2: u, v, w (scaled), T, psi, vort as (n2, n1)."""
+        n = (self.cfg.nx, self.cfg.ny, self.cfg.nz)
+        n1 = n[1] if axis == 0 else n[0]
+        n2 = n[1] if axis == 2 else n[2]
+        out = {k: np.empty((n2, n1)) for k in ("u", "v", "w", "T", "psi", "vort")}
+        lib().orc_midplane_slice(self.h, int(axis), *[_ptr(out[k]) for k in
+                                                      ("u", "v", "w", "T", "psi", "vort")])
+        return out
 
     def midplane(self) -> dict:
         """Mid-plane velocity maxima (scaled by L / kappa) and their positions."""

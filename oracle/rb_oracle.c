@@ -1505,3 +1505,73 @@ void orc_midplane(const orc_state *s, orc_mid *m)
     free(uline);
     free(vline);
 }
+
+/* Mid-plane slices: write_midplane_x/y/z (3DRB.F90:2611-2713) and write_midplane_stream_x/y/z
+ * (3DRB.F90:2721-3011).  On the plane across `axis` with in-plane axes (a1, a2) = (y, z),
+ * (x, z), (x, y): psi integrates the a1 velocity component along a2 from the wall (the first
+ * cell by (21 v1 - v2) / 72 dz, then Simpson's rule with quadratic midpoint values), and the
+ * vorticity is the plane-normal component, d c2 / d a1 - d c1 / d a2 for the x and z planes and
+ * its negative for the y plane (omega_y = du/dz - dw/dx), central differences inside and
+ * second-order one-sided differences at the ends. */
+static double slice_d(const double *q, int n, long st, int t, double h)
+{
+    if (n == 1) return 0.0;
+    if (t == 0) return n == 2 ? (q[st] - q[0]) / h : (-3.0 * q[0] + 4.0 * q[st] - q[2 * st]) / (2.0 * h);
+    if (t == n - 1)
+        return n == 2 ? (q[t * st] - q[(t - 1) * st]) / h
+                      : (3.0 * q[t * st] - 4.0 * q[(t - 1) * st] + q[(t - 2) * st]) / (2.0 * h);
+    return (q[(t + 1) * st] - q[(t - 1) * st]) / (2.0 * h);
+}
+
+void orc_midplane_slice(const orc_state *s, int axis, double *uo, double *vo, double *wo,
+                        double *To, double *psi, double *vort)
+{
+    const orc_params *p = &s->p;
+    const double coef = p->L / p->diffusivity, d = 1.0 / p->L;
+    const int n[3] = {NX, NY, NZ};
+    const double *coords[3] = {s->xp, s->yp, s->zp};
+    const double *fields[4] = {s->u, s->v, s->w, s->T};
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    const int n1 = n[a1], n2 = n[a2];
+    int iL, iR;
+    double w;
+    bracket(coords[axis], n[axis], 0.5 * coords[axis][n[axis] + 1], &iL, &iR, &w);
+    double *sl[4];
+    for (int c = 0; c < 4; ++c) {
+        sl[c] = (double *)malloc(sizeof(double) * (size_t)n1 * n2);
+        for (int q2 = 1; q2 <= n2; ++q2)
+            for (int q1 = 1; q1 <= n1; ++q1)
+                sl[c][(q1 - 1) + (size_t)n1 * (q2 - 1)] = interp(s, fields[c], axis, iL, iR, w, q1, q2);
+    }
+    double *outs[4] = {uo, vo, wo, To};
+    for (int c = 0; c < 4; ++c)
+        if (outs[c])
+            for (long t = 0; t < (long)n1 * n2; ++t) outs[c][t] = c == 3 ? sl[3][t] : coef * sl[c][t];
+    const double *c1 = sl[a1], *c2 = sl[a2];
+    if (psi)
+        for (int q1 = 0; q1 < n1; ++q1) {
+            const double *q = c1 + q1;
+            double *o = psi + q1;
+            if (n2 == 1) { o[0] = 0.0; continue; }
+            o[0] = d * (21.0 * (q[0] * coef) - q[n1] * coef) / 72.0;
+            for (int k = 2; k <= n2; ++k) {
+                const int m = k - 1;
+                double mid;
+                if (n2 == 2) mid = 0.5 * (q[0] + q[n1]) * coef;
+                else if (m == 1) mid = (3.0 / 8.0 * q[0] + 3.0 / 4.0 * q[n1] - 1.0 / 8.0 * q[2 * n1]) * coef;
+                else
+                    mid = (-1.0 / 8.0 * q[(long)(m - 2) * n1] + 3.0 / 4.0 * q[(long)(m - 1) * n1] +
+                           3.0 / 8.0 * q[(long)m * n1]) * coef;
+                o[(long)(k - 1) * n1] = o[(long)(k - 2) * n1] +
+                                        d / 6.0 * (q[(long)(k - 2) * n1] * coef + 4.0 * mid + q[(long)(k - 1) * n1] * coef);
+            }
+        }
+    if (vort)
+        for (int q2 = 0; q2 < n2; ++q2)
+            for (int q1 = 0; q1 < n1; ++q1) {
+                const double dc2 = slice_d(c2 + (long)q2 * n1, n1, 1, q1, d);
+                const double dc1 = slice_d(c1 + q1, n2, n1, q2, d);
+                vort[q1 + (long)n1 * q2] = axis == 1 ? coef * (dc1 - dc2) : coef * (dc2 - dc1);
+            }
+    for (int c = 0; c < 4; ++c) free(sl[c]);
+}

@@ -86,6 +86,10 @@ int orc_set_fg(orc_state *s, const double *f, const double *g);
 void orc_check(orc_state *s, double *error_u, double *error_t);
 void orc_diagnostics(const orc_state *s, orc_diag *d);
 void orc_midplane(const orc_state *s, orc_mid *m);
+/* 3D mid-plane slice across axis (0 x, 1 y, 2 z): u, v, w scaled by L/kappa, T, stream function
+ * and vorticity; n1*n2 values each, first in-plane axis fastest (any pointer may be NULL). */
+void orc_midplane_slice(const orc_state *s, int axis, double *u, double *v, double *w, double *T,
+                        double *psi, double *vort);
 
 void orc_fit_adiabatic_wall_T4(double y0, const double *y, const double *tt, double *t_wall);
 void orc_fit_parabola_ls5(const double *y, const double *f, int mode, double *fstar,

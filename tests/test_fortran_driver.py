@@ -94,3 +94,72 @@ def test_driver_restart_continues_bitwise(tmp_path):
     r.step(30)
     t = r.state()
     np.testing.assert_array_equal(recs[0].reshape(19, -1), t["f"].reshape(-1, 19).T)
+
+
+def _read_plt(path):
+    """Parse a one-zone point-format Tecplot v101 binary file written by the driver."""
+    data = open(path, "rb").read()
+    assert data[:8] == b"#!TDV101"
+    pos = 8
+    def i32():
+        nonlocal pos
+        v = int(np.frombuffer(data, "<i4", 1, pos)[0])
+        pos += 4
+        return v
+    def f32():
+        nonlocal pos
+        v = float(np.frombuffer(data, "<f4", 1, pos)[0])
+        pos += 4
+        return v
+    def string():
+        chars = []
+        while True:
+            c = i32()
+            if c == 0:
+                return "".join(chars)
+            chars.append(chr(c))
+    assert i32() == 1
+    title = string()
+    nv = i32()
+    names = [string() for _ in range(nv)]
+    assert f32() == 299.0
+    assert string() == "ZONE 001"
+    assert [i32() for _ in range(5)] == [-1, 0, 1, 0, 0]
+    dims = [i32() for _ in range(3)]
+    assert i32() == 0 and f32() == 357.0 and f32() == 299.0
+    assert [i32() for _ in range(nv)] == [2] * nv
+    assert i32() == 0 and i32() == -1
+    npts = dims[0] * dims[1] * dims[2]
+    vals = np.frombuffer(data, "<f8", nv * npts, pos).reshape(npts, nv)
+    assert pos + 8 * nv * npts == len(data)
+    return title, names, dims, vals
+
+
+def test_driver_tecplot_files_match_oracle(tmp_path):
+    # outputPltFile=1: the full field and, per mid-plane, the interpolated fields and the
+    # stream function / vorticity, in the reference's Tecplot layout (2DRB/3DRB.F90:1676-1701,
+    # :3018-3266), values equal to the oracle's state after the same steps.
+    nx, ny, nz = 14, 12, 10
+    _run(tmp_path, f"dims=3, nx={nx}, ny={ny}, nz={nz}, Rayleigh=1.0d6, itc_max=59, "
+                   "checkInterval=1000, seed=5, outputPltFile=1, pltFolderPrefix='cav'")
+    o = orc.Oracle(orc.Config(dims=3, nx=nx, ny=ny, nz=nz, rayleigh=1e6))
+    o.init_seeded(5, 1e-3)
+    o.step(60)
+    s = o.state()
+    L = float(ny)
+    title, names, dims, vals = _read_plt(tmp_path / "cav-fullField-000000000060.plt")
+    assert (title, names, dims) == ("full-field-3d", ["X", "Y", "Z", "U", "V", "W", "T"], [nx, ny, nz])
+    for col, key in ((3, "u"), (4, "v"), (5, "w"), (6, "T")):
+        np.testing.assert_array_equal(vals[:, col], s[key].reshape(-1))
+    np.testing.assert_allclose(vals[:nx, 0], (np.arange(nx) + 0.5) / L)
+    planes = {0: ("X", "Y", "Z", ny, nz), 1: ("Y", "X", "Z", nx, nz), 2: ("Z", "X", "Y", nx, ny)}
+    for axis, (a, c1, c2, n1, n2) in planes.items():
+        sl = o.midplane_slice(axis)
+        title, names, dims, vals = _read_plt(tmp_path / f"cav-mid{a}-000000000060.plt")
+        assert names == [c1, c2, "U_nd", "V_nd", "W_nd", "T"] and dims == [n1, n2, 1]
+        for col, key in ((2, "u"), (3, "v"), (4, "w"), (5, "T")):
+            np.testing.assert_allclose(vals[:, col], sl[key].reshape(-1), rtol=1e-13, atol=1e-13)
+        title, names, dims, vals = _read_plt(tmp_path / f"cav-mid{a}psiVort-000000000060.plt")
+        assert names == [c1, c2, "PSI_" + c1 + c2, "VORT_" + a]
+        np.testing.assert_allclose(vals[:, 2], sl["psi"].reshape(-1), rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(vals[:, 3], sl["vort"].reshape(-1), rtol=1e-12, atol=1e-12)

@@ -285,3 +285,14 @@ def test_mass_conservation_and_symmetry_512_slab():
     m = sol.macros()
     np.testing.assert_allclose(m["T"][:, :, ::-1], m["T"], rtol=0, atol=1e-12)
     np.testing.assert_allclose(m["u"][:, :, ::-1], -m["u"], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_midplane_slices_match_oracle(axis):
+    sol, ora = _run_pair(3, 20, 17, 12, (200,), rayleigh=1e6)
+    a, b = sol.midplane_slice(axis), ora.midplane_slice(axis)
+    for k in ("u", "v", "w", "T"):
+        np.testing.assert_array_equal(a[k], b[k])
+    for k in ("psi", "vort"):
+        scale = np.abs(b[k]).max()
+        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-13 * scale)

@@ -415,3 +415,45 @@ def test_restart_reconstruction_roundtrip():
     f = s["f"].copy()
     f[0, 0, 0, :] = -1.0
     assert r.set_fg(f, s["g"]) == -1
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_midplane_slice_stream_function_and_vorticity_closed_form(axis):
+    # On each mid-plane the stream function integrates the first in-plane axis's velocity along
+    # the second axis from the wall; with that velocity a*s^2 (s the wall distance along the
+    # second axis) the first-cell rule, Simpson's rule and the quadratic midpoints are exact:
+    # psi = (L/kappa) a s^3 / 3.  The plane-normal vorticity of the same field is exact too
+    # (second-order differences): -(L/kappa) 2 a s for the x and z planes, +(L/kappa) 2 a s for
+    # the y plane (omega_y = du/dz - dw/dx).  3DRB.F90:2721-3011.
+    cfg = orc.Config(dims=3, nx=10, ny=12, nz=9, rayleigh=1e5, mach=0.1)
+    p = orc.params(cfg)
+    L, coef = p["L"], p["L"] / p["diffusivity"]
+    a = 0.03
+    zp, yp = (np.arange(cfg.nz) + 0.5) / L, (np.arange(cfg.ny) + 0.5) / L
+    zero = np.zeros(cfg.shape)
+    if axis == 0:      # (y, z): v(z)
+        s = zp
+        prof = np.broadcast_to((a * zp ** 2)[:, None, None], cfg.shape)
+        u, v, w = zero, prof, zero
+    elif axis == 1:    # (x, z): u(z)
+        s = zp
+        prof = np.broadcast_to((a * zp ** 2)[:, None, None], cfg.shape)
+        u, v, w = prof, zero, zero
+    else:              # (x, y): u(y)
+        s = yp
+        prof = np.broadcast_to((a * yp ** 2)[None, :, None], cfg.shape)
+        u, v, w = prof, zero, zero
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    assert o.set_fg(*_velocity_state(cfg, u, v, w)) == 0
+    sl = o.midplane_slice(axis)
+    psi_exact = coef * a * s ** 3 / 3
+    np.testing.assert_allclose(sl["psi"], np.broadcast_to(psi_exact[:, None], sl["psi"].shape),
+                               rtol=1e-10, atol=1e-12 * coef)
+    sign = 1.0 if axis == 1 else -1.0
+    vort_exact = sign * coef * 2 * a * s
+    np.testing.assert_allclose(sl["vort"], np.broadcast_to(vort_exact[:, None], sl["vort"].shape),
+                               rtol=1e-9, atol=1e-9 * coef * a)
+    comp = {0: "v", 1: "u", 2: "u"}[axis]
+    np.testing.assert_allclose(sl[comp], np.broadcast_to(coef * a * s[:, None] ** 2, sl[comp].shape),
+                               rtol=1e-12)
