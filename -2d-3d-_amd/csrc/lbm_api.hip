@@ -1077,6 +1077,188 @@ extern "C" int lbm_midplane_slice(lbm_handle *h, int32_t axis, double *us, doubl
     return LBM_OK;
 }
 
+// ---- 2D stream function, vorticity and |psi| statistics --------------------------------------
+// natural cubic spline (spline_natural, 2DRB/2DRB.F90:3000-3031) and its evaluation (splint)
+static void spline_natural(int n, const double *x, const double *y, double *y2, double *u)
+{
+    y2[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < n - 1; ++i) {
+        const double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        const double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) /
+                    (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    y2[n - 1] = 0.0;
+    for (int k = n - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+static double splint_host(int n, const double *xa, const double *ya, const double *y2a, double x)
+{
+    int klo = 0, khi = n - 1;
+    while (khi - klo > 1) {
+        const int k = (khi + klo) / 2;
+        if (xa[k] > x) khi = k;
+        else klo = k;
+    }
+    const double h = xa[khi] - xa[klo];
+    if (h == 0.0) return ya[klo];
+    const double a = (xa[khi] - x) / h, b = (x - xa[klo]) / h;
+    return a * ya[klo] + b * ya[khi] + ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[khi]) * (h * h) / 6.0;
+}
+
+static double lagrange4(double xq, const double *xk, const double *fk)
+{
+    double fq = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        double basis = 1.0;
+        for (int b = 0; b < 4; ++b)
+            if (b != a) basis = basis * (xq - xk[b]) / (xk[a] - xk[b]);
+        fq = fq + fk[a] * basis;
+    }
+    return fq;
+}
+
+extern "C" int lbm_psi_vorticity_2d(lbm_handle *h, double *psi_out, double *vort_out, lbm_psi *st)
+{
+    if (!h) return LBM_ERR_ARG;
+    const lbm_config &c = h->cfg;
+    if (c.dims != 2) return fail(h, LBM_ERR_UNSUPPORTED, "stream function: 2D only");
+    if (c.nx < 3 || c.ny < 3) return fail(h, LBM_ERR_ARG, "stream function needs nx, ny >= 3");
+    HIPCHK(h, hipSetDevice(c.device));
+    int rc = refresh_macros(h);
+    if (rc) return rc;
+    const int nx = c.nx, ny = c.ny;
+    const size_t N = (size_t)nx * ny;
+    std::vector<double> uu(N), vv(N), psi(N);
+    HIPCHK(h, hipMemcpyAsync(uu.data(), h->mac + N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(vv.data(), h->mac + 2 * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const double L = h->p.L, dx = 1.0 / L, dy = 1.0 / L, coef = L / h->p.diffusivity;
+    auto U = [&](int i, int j) { return uu[(i - 1) + (size_t)nx * (j - 1)]; };
+    auto V = [&](int i, int j) { return vv[(i - 1) + (size_t)nx * (j - 1)]; };
+    auto P = [&](int i, int j) -> double & { return psi[(i - 1) + (size_t)nx * (j - 1)]; };
+    // calc_psi_vort_and_output (2DRB/2DRB.F90:2657-2770): psi integrates u along y from the wall
+    for (int i = 1; i <= nx; ++i) {
+        const double u1 = U(i, 1) * coef, u2 = U(i, 2) * coef;
+        P(i, 1) = dy * (21.0 * u1 - u2) / 72.0;
+        for (int j = 2; j <= ny; ++j) {
+            const int m = j - 1;
+            double um;
+            if (m == 1) um = (3.0 / 8.0 * U(i, 1) + 3.0 / 4.0 * U(i, 2) - 1.0 / 8.0 * U(i, 3)) * coef;
+            else um = (-1.0 / 8.0 * U(i, m - 1) + 3.0 / 4.0 * U(i, m) + 3.0 / 8.0 * U(i, m + 1)) * coef;
+            const double inc = dy / 6.0 * ((U(i, j - 1) * coef) + 4.0 * um + (U(i, j) * coef));
+            P(i, j) = P(i, j - 1) + inc;
+        }
+    }
+    if (psi_out) memcpy(psi_out, psi.data(), N * sizeof(double));
+    if (vort_out)   // walls at rest: one-sided differences through the wall value 0
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                double dvdx, dudy;
+                if (i == 1) dvdx = (3.0 * V(1, j) + V(2, j) - 4.0 * 0.0) / (3.0 * dx);
+                else if (i == nx) dvdx = (-3.0 * V(nx, j) - V(nx - 1, j) + 4.0 * 0.0) / (3.0 * dx);
+                else dvdx = (V(i + 1, j) - V(i - 1, j)) / (2.0 * dx);
+                if (j == 1) dudy = (3.0 * U(i, 1) + U(i, 2) - 4.0 * 0.0) / (3.0 * dy);
+                else if (j == ny) dudy = (-3.0 * U(i, ny) - U(i, ny - 1) + 4.0 * 0.0) / (3.0 * dy);
+                else dudy = (U(i, j + 1) - U(i, j - 1)) / (2.0 * dy);
+                vort_out[(i - 1) + (size_t)nx * (j - 1)] = coef * (dvdx - dudy);
+            }
+    if (!st) return LBM_OK;
+    memset(st, 0, sizeof(*st));
+    std::vector<double> xp(nx + 2), yp(ny + 2);
+    xp[nx + 1] = (double)nx / L;
+    yp[ny + 1] = (double)ny / L;
+    for (int i = 1; i <= nx; ++i) xp[i] = ((double)i - 0.5) / L;
+    for (int j = 1; j <= ny; ++j) yp[j] = ((double)j - 0.5) / L;
+    // output_psi_center_abs (:3085-3160): 4-point Lagrange interpolation, x then y
+    const double x0 = 0.5 * xp[nx + 1], y0 = 0.5 * yp[ny + 1];
+    double pc;
+    if (nx < 4 || ny < 4) {
+        pc = P((nx + 1) / 2, (ny + 1) / 2);
+    } else {
+        int i0 = 1;
+        while (i0 < nx && xp[i0 + 1] <= x0) ++i0;
+        i0 = std::max(1, std::min(i0 - 1, nx - 3));
+        int j0 = 1;
+        while (j0 < ny && yp[j0 + 1] <= y0) ++j0;
+        j0 = std::max(1, std::min(j0 - 1, ny - 3));
+        double x4[4], y4[4], f4[4], gx[4];
+        for (int q = 0; q < 4; ++q) { x4[q] = xp[i0 + q]; y4[q] = yp[j0 + q]; }
+        for (int q = 0; q < 4; ++q) {
+            for (int pp = 0; pp < 4; ++pp) f4[pp] = P(i0 + pp, j0 + q);
+            gx[q] = lagrange4(x0, x4, f4);
+        }
+        pc = lagrange4(y0, y4, gx);
+    }
+    st->psi_center_coarse = fabs(pc);
+    // calc_psi_absmax_fine_spline (:2873-2989): row splines on the host, the fine grid on the GPU
+    const double xlen = xp[nx + 1], ylen = yp[ny + 1];
+    const int nfx = (int)std::max(2.0, (double)lround(xlen * 10000.0) + 1.0);
+    const int nfy = (int)std::max(2.0, (double)lround(ylen * 10000.0) + 1.0);
+    const int nxe = nx + 2, nye = ny + 2;
+    std::vector<double> xe(nxe), ye(nye), rows((size_t)ny * nxe), y2r((size_t)ny * nxe), wk(std::max(nxe, nye));
+    xe[0] = 0.0; xe[nxe - 1] = xp[nx + 1];
+    for (int i = 1; i <= nx; ++i) xe[i] = xp[i];
+    ye[0] = 0.0; ye[nye - 1] = yp[ny + 1];
+    for (int j = 1; j <= ny; ++j) ye[j] = yp[j];
+    std::vector<double> pcx(ny), colc(nye), y2c(nye);
+    for (int j = 1; j <= ny; ++j) {
+        double *row = &rows[(size_t)(j - 1) * nxe], *d2 = &y2r[(size_t)(j - 1) * nxe];
+        row[0] = 0.0;
+        row[nxe - 1] = 0.0;
+        for (int i = 1; i <= nx; ++i) row[i] = P(i, j);
+        spline_natural(nxe, xe.data(), row, d2, wk.data());
+        pcx[j - 1] = splint_host(nxe, xe.data(), row, d2, 0.5 * xlen);
+    }
+    colc[0] = 0.0;
+    colc[nye - 1] = 0.0;
+    for (int j = 1; j <= ny; ++j) colc[j] = pcx[j - 1];
+    spline_natural(nye, ye.data(), colc.data(), y2c.data(), wk.data());
+    st->psi_center_fine = fabs(splint_host(nye, ye.data(), colc.data(), y2c.data(), 0.5 * ylen));
+
+    const unsigned blocks = 2048;
+    double *d = nullptr;
+    const size_t ncol = (size_t)nfx * nye;
+    const size_t nd = nxe + nye + 2 * (size_t)ny * nxe + 3 * ncol + blocks;
+    if (hipMalloc((void **)&d, nd * sizeof(double) + blocks * sizeof(unsigned long long)) != hipSuccess)
+        return fail(h, LBM_ERR_NOMEM, "stream-function buffers");
+    double *dxe = d, *dye = dxe + nxe, *drows = dye + nye, *dy2r = drows + (size_t)ny * nxe;
+    double *dcols = dy2r + (size_t)ny * nxe, *dy2c = dcols + ncol, *dwk = dy2c + ncol;
+    double *dbv = dwk + ncol;
+    unsigned long long *dbi = (unsigned long long *)(dbv + blocks);
+    std::vector<double> bv(blocks);
+    std::vector<unsigned long long> bi(blocks);
+    hipError_t e = hipMemcpyAsync(dxe, xe.data(), nxe * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dye, ye.data(), nye * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(drows, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dy2r, y2r.data(), y2r.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess)
+        e = launch_psi_absmax(h->stream, nx, ny, nfx, nfy, xlen, ylen, dxe, dye, drows, dy2r, dcols,
+                              dy2c, dwk, dbv, dbi, blocks);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(bv.data(), dbv, blocks * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(bi.data(), dbi, blocks * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(h, LBM_ERR_HIP, "stream function: %s", hipGetErrorString(e));
+    double best = -1.0;
+    unsigned long long idx = ~0ULL;
+    for (unsigned b = 0; b < blocks; ++b)
+        if (bv[b] > best || (bv[b] == best && bi[b] < idx)) {
+            best = bv[b];
+            idx = bi[b];
+        }
+    st->psi_abs_max = best;
+    st->x_at_max = xlen * (double)(idx / (unsigned long long)nfy) / (double)(nfx - 1);
+    st->y_at_max = ylen * (double)(idx % (unsigned long long)nfy) / (double)(nfy - 1);
+    return LBM_OK;
+}
+
 // Chunk of z-planes whose AoS f and g fit a staging buffer of about 256 MiB.
 static int chunk_planes(const lbm_handle *h)
 {

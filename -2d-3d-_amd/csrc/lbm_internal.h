@@ -55,6 +55,13 @@ hipError_t launch_reduce_walls(const KParams &kp, hipStream_t s, const double *m
 // Plane of a macro field across `axis` at 0-based indices (l, r), see k_plane.
 hipError_t launch_plane(const KParams &kp, hipStream_t s, const double *f, int axis, int l, int r,
                         double w, int mode, double *out);
+// Fine-grid |psi| maximum (2D): row splines rows / y2rows (ny x (nx + 2)) sampled at nfx fine
+// x positions into the fine columns, their splines, then |psi| at nfx x nfy points; per block the
+// maximum and the first linear index (column-major fine grid) where it occurs.
+hipError_t launch_psi_absmax(hipStream_t s, int nx, int ny, int nfx, int nfy, double xlen, double ylen,
+                             const double *xe, const double *ye, const double *rows,
+                             const double *y2rows, double *cols, double *y2cols, double *wk,
+                             double *bval, unsigned long long *bidx, unsigned blocks);
 hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *mac, double *prev,
                                double *out, unsigned blocks);
 

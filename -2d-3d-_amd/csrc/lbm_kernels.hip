@@ -741,6 +741,128 @@ __global__ void __launch_bounds__(256) k_plane(KParams kp, const double *__restr
     }
 }
 
+// ---- fine-grid |psi| maximum of the 2D stream function (calc_psi_absmax_fine_spline,
+// 2DRB/2DRB.F90:2873-2989).  Natural cubic splines (spline_natural / splint, :3000-3074) through
+// [0, values, 0] on the node coordinates extended by the two walls; the same operations in the
+// same order as the host restatement, so the maximum and its position agree bit for bit.
+__device__ __forceinline__ double splint_dev(int n, const double *xa, const double *ya,
+                                             const double *y2a, double x)
+{
+    int klo = 0, khi = n - 1;
+    while (khi - klo > 1) {
+        const int k = (khi + klo) / 2;
+        if (xa[k] > x) khi = k;
+        else klo = k;
+    }
+    const double h = xa[khi] - xa[klo];
+    if (h == 0.0) return ya[klo];
+    const double a = (xa[khi] - x) / h, b = (x - xa[klo]) / h;
+    return a * ya[klo] + b * ya[khi] + ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[khi]) * (h * h) / 6.0;
+}
+
+// A: every row spline (ny rows of nxe values) at the nfx fine x positions -> pxf[k * nye + 1 + j]
+// (the fine columns, wall zeros at 0 and nye - 1)
+__global__ void __launch_bounds__(256) k_spline_rows(int nxe, int ny, int nfx, double xlen,
+                                                     const double *__restrict__ xe,
+                                                     const double *__restrict__ rows,
+                                                     const double *__restrict__ y2rows,
+                                                     double *__restrict__ cols)
+{
+    const int nye = ny + 2;
+    const uint64_t total = (uint64_t)nfx * ny;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(t % ny), k = (int)(t / ny);
+        const double xq = xlen * (double)k / (double)(nfx - 1);
+        cols[(uint64_t)k * nye + 1 + j] =
+            splint_dev(nxe, xe, rows + (uint64_t)j * nxe, y2rows + (uint64_t)j * nxe, xq);
+        if (j == 0) {
+            cols[(uint64_t)k * nye] = 0.0;
+            cols[(uint64_t)k * nye + nye - 1] = 0.0;
+        }
+    }
+}
+
+// B: natural spline second derivatives of every fine column (one thread per column)
+__global__ void __launch_bounds__(256) k_spline_cols(int nye, int nfx, const double *__restrict__ ye,
+                                                     const double *__restrict__ cols,
+                                                     double *__restrict__ y2, double *__restrict__ wk)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nfx) return;
+    const double *y = cols + (uint64_t)k * nye;
+    double *d = y2 + (uint64_t)k * nye, *u = wk + (uint64_t)k * nye;
+    d[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < nye - 1; ++i) {
+        const double sig = (ye[i] - ye[i - 1]) / (ye[i + 1] - ye[i - 1]);
+        const double p = sig * d[i - 1] + 2.0;
+        d[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (ye[i + 1] - ye[i]) - (y[i] - y[i - 1]) / (ye[i] - ye[i - 1])) /
+                    (ye[i + 1] - ye[i - 1]) - sig * u[i - 1]) / p;
+    }
+    d[nye - 1] = 0.0;
+    for (int i = nye - 2; i >= 0; --i) d[i] = d[i] * d[i + 1] + u[i];
+}
+
+// C: |psi| on the fine grid (column k outer, y index l inner); per block the largest value and
+// the first (k, l) where it occurs
+__global__ void __launch_bounds__(256) k_spline_absmax(int nye, int nfx, int nfy, double ylen,
+                                                       const double *__restrict__ ye,
+                                                       const double *__restrict__ cols,
+                                                       const double *__restrict__ y2,
+                                                       double *__restrict__ bval,
+                                                       unsigned long long *__restrict__ bidx)
+{
+    __shared__ double sv[256];
+    __shared__ unsigned long long si[256];
+    double best = -1.0;
+    unsigned long long bi = ~0ULL;
+    const uint64_t total = (uint64_t)nfx * nfy;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int l = (int)(t % nfy), k = (int)(t / nfy);
+        const double yq = ylen * (double)l / (double)(nfy - 1);
+        const double v = fabs(splint_dev(nye, ye, cols + (uint64_t)k * nye, y2 + (uint64_t)k * nye, yq));
+        if (v > best || (v == best && t < bi)) {
+            best = v;
+            bi = t;
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = sv[threadIdx.x + s];
+            const unsigned long long oi = si[threadIdx.x + s];
+            if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = o;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bval[blockIdx.x] = sv[0];
+        bidx[blockIdx.x] = si[0];
+    }
+}
+
+hipError_t launch_psi_absmax(hipStream_t s, int nx, int ny, int nfx, int nfy, double xlen, double ylen,
+                             const double *xe, const double *ye, const double *rows,
+                             const double *y2rows, double *cols, double *y2cols, double *wk,
+                             double *bval, unsigned long long *bidx, unsigned blocks)
+{
+    const int nxe = nx + 2, nye = ny + 2;
+    const uint64_t ta = (uint64_t)nfx * ny;
+    k_spline_rows<<<(unsigned)((ta + 255) / 256 < 65536 ? (ta + 255) / 256 : 65536), 256, 0, s>>>(
+        nxe, ny, nfx, xlen, xe, rows, y2rows, cols);
+    k_spline_cols<<<(nfx + 255) / 256, 256, 0, s>>>(nye, nfx, ye, cols, y2cols, wk);
+    k_spline_absmax<<<blocks, 256, 0, s>>>(nye, nfx, nfy, ylen, ye, cols, y2cols, bval, bidx);
+    return hipGetLastError();
+}
+
 // `check` sums: 0 sum |u - u_p|^2, 1 sum |u|^2, 2 sum |T - T_p|, 3 sum |T|; then u_p = u.
 __global__ void __launch_bounds__(256) k_reduce_check(KParams kp, const double *__restrict__ mac,
                                                       double *__restrict__ prev,

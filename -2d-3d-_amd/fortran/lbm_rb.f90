@@ -34,6 +34,10 @@ module lbm_rb
     real(c_double) :: mass, heat, kinetic
   end type lbm_diag
 
+  type, bind(C) :: lbm_psi
+    real(c_double) :: psi_center_coarse, psi_center_fine, psi_abs_max, x_at_max, y_at_max
+  end type lbm_psi
+
   type, bind(C) :: lbm_midplane
     real(c_double) :: u_max, u_pos1, u_pos2, u_plane
     real(c_double) :: v_max, v_pos1, v_pos2, v_plane
@@ -99,6 +103,11 @@ module lbm_rb
       type(c_ptr), value :: h
       integer(c_int32_t), value :: axis
       type(c_ptr), value :: u, v, w, t, psi, vort
+    end function
+    integer(c_int) function lbm_psi_vorticity_2d(h, psi, vort, st) bind(C, name="lbm_psi_vorticity_2d")
+      import :: c_int, c_ptr, lbm_psi
+      type(c_ptr), value :: h, psi, vort
+      type(lbm_psi), intent(out) :: st
     end function
     integer(c_int) function lbm_gather(h, f, g, bprev) bind(C, name="lbm_gather")
       import :: c_int, c_ptr

@@ -112,6 +112,7 @@ program rb_main
   write(*,'(a,1x,es16.8)') 'NuVolAvg =', dg%nu_vol
   write(*,'(a,1x,es16.8)') 'ReVolAvg =', dg%re_vol
   call report_midplane()
+  if (dims == 2) call report_psi_vort()
   write(*,'(a,1x,i0)') 'itc =', itc
 
   open(unit=00, file=trim(settingsFile), status='unknown', position='append')
@@ -239,6 +240,42 @@ contains
     write(03) rho
     close(03)
   end subroutine output_binary
+
+  !------------------------------------------------------------------------------------------
+  ! 2D stream function and vorticity (calc_psi_vort_and_output, 2DRB/2DRB.F90:2657-2770): the
+  ! |psi| statistics in the log and, with outputPltFile, the psi / vorticity Tecplot file
+  ! (output_Tecplot_psi_vort, :2782-2862).
+  !------------------------------------------------------------------------------------------
+  subroutine report_psi_vort()
+    real(8), allocatable, target :: psi(:,:), vort(:,:)
+    real(8), allocatable :: dat(:,:)
+    type(lbm_psi) :: ps
+    character(len=12) :: tag
+    character(len=*), parameter :: fmt1 = '(a,1x,es16.8)'
+    character(len=*), parameter :: fmt3 = '(a,1x,es16.8,2x,a,1x,es16.8,2x,a,1x,es16.8)'
+    integer :: i, j
+    allocate(psi(nx, ny), vort(nx, ny))
+    call lbm_require(lbm_psi_vorticity_2d(h, c_loc(psi), c_loc(vort), ps), 'lbm_psi_vorticity_2d', h, 6)
+    write(*,fmt1) 'abs(psi_center_coarse) =', ps%psi_center_coarse
+    write(*,fmt1) 'abs(psi_center_fine) =', ps%psi_center_fine
+    write(*,fmt3) 'max(|psi|) =', ps%psi_abs_max, 'x* =', ps%x_at_max, 'y* =', ps%y_at_max
+    open(unit=00, file=trim(settingsFile), status='unknown', position='append')
+    write(00,fmt1) 'abs(psi_center_coarse) =', ps%psi_center_coarse
+    write(00,fmt1) 'abs(psi_center_fine) =', ps%psi_center_fine
+    write(00,fmt3) 'max(|psi|) =', ps%psi_abs_max, 'x* =', ps%x_at_max, 'y* =', ps%y_at_max
+    close(00)
+    if (outputPltFile == 1) then
+      allocate(dat(4, nx*ny))
+      do j = 1, ny
+        do i = 1, nx
+          dat(:, i + nx * (j - 1)) = (/ coordinate(i), coordinate(j), psi(i,j), vort(i,j) /)
+        enddo
+      enddo
+      write(tag,'(i12.12)') itc
+      call plt_write(trim(pltFolderPrefix)//'-psiVort-'//tag//'.plt', 'psi-vort', &
+           (/ 'X   ', 'Y   ', 'PSI ', 'VORT' /), nx, ny, 1, dat)
+    endif
+  end subroutine report_psi_vort
 
   !------------------------------------------------------------------------------------------
   ! output_Tecplot (2DRB/3DRB.F90:1676-1701; 2D 2DRB/2DRB.F90:1443-1570).  3D: the full field

@@ -280,6 +280,24 @@ class Solver:
                                                             ("u", "v", "w", "T", "psi", "vort")]))
         return out
 
+    def psi_vorticity_2d(self, stats: bool = True) -> dict:
+        """2D stream function and vorticity (ny, nx) and the |psi| statistics
+        (lbm_psi_vorticity_2d)."""
+        L = self.lib
+        if not getattr(L, "_psi_bound", False):
+            L.lbm_psi_vorticity_2d.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+            L.lbm_psi_vorticity_2d.restype = C.c_int
+            L._psi_bound = True
+        sh = (self.cfg.ny, self.cfg.nx)
+        psi, vort, st = np.empty(sh), np.empty(sh), np.zeros(5)
+        self._chk(L.lbm_psi_vorticity_2d(self._h, _p(psi), _p(vort), _p(st) if stats else None))
+        out = {"psi": psi, "vort": vort}
+        if stats:
+            out.update(zip(("psi_center_coarse", "psi_center_fine", "psi_abs_max", "x_at_max",
+                            "y_at_max"), st.tolist()))
+        return out
+
     def midplane(self) -> dict:
         """Mid-plane velocity maxima (scaled by L / kappa) and their positions."""
         m = _Mid()

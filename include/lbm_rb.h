@@ -83,6 +83,15 @@ typedef struct lbm_midplane {
     double w_max, w_pos1, w_pos2, w_plane;   /* w on the z mid-plane (3D: pos = x, y; 2D: 0) */
 } lbm_midplane;
 
+/* 2D stream-function statistics (calc_psi_vort_and_output, 2DRB/2DRB.F90:2657-2770):
+ * |psi| at the cavity centre from 4x4 Lagrange interpolation (output_psi_center_abs) and from
+ * natural cubic splines, and the largest |psi| on the fine spline grid (10000 intervals per unit
+ * length, calc_psi_absmax_fine_spline) with its position. */
+typedef struct lbm_psi {
+    double psi_center_coarse, psi_center_fine;
+    double psi_abs_max, x_at_max, y_at_max;
+} lbm_psi;
+
 typedef struct lbm_handle lbm_handle;
 
 /* Allocate the domain on cfg->device.  Replaces the allocate block of `initial`
@@ -132,6 +141,11 @@ int lbm_midplane_extrema(lbm_handle *h, lbm_midplane *m);
  * fastest: x-plane (y, z), y-plane (x, z), z-plane (x, y).  Any output may be NULL. */
 int lbm_midplane_slice(lbm_handle *h, int32_t axis, double *u, double *v, double *w, double *T,
                        double *psi, double *vort);
+/* 2D stream function and vorticity (nx*ny each, x fastest; psi integrates u along y from the
+ * bottom wall, the vorticity takes one-sided differences through the resting walls) and, if st
+ * is not NULL, the |psi| statistics; the fine-grid search runs on the GPU.  psi and vort may be
+ * NULL.  2DRB/2DRB.F90:2657-3160. */
+int lbm_psi_vorticity_2d(lbm_handle *h, double *psi, double *vort, lbm_psi *st);
 
 /* f, g (reference AoS order, absolute FP64) and, if non-NULL, B_prev (3 planes of nx*ny*nz:
  * Bx_prev, By_prev, Bz_prev, 2DRB/3DRB.F90:163).  Restart data of backupData

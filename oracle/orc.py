@@ -86,6 +86,10 @@ def lib():
         L.orc_diagnostics.argtypes = [C.c_void_p, C.POINTER(_Diag)]
         L.orc_midplane.argtypes = [C.c_void_p, C.POINTER(_Mid)]
         L.orc_midplane_slice.argtypes = [C.c_void_p, C.c_int] + [dp] * 6
+        L.orc_psi_vort_2d.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_spline_eval.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
+        L.orc_lagrange4.argtypes = [C.c_double, dp, dp]
+        L.orc_lagrange4.restype = C.c_double
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = C.c_int
         L.orc_fit_adiabatic_wall_T4.argtypes = [C.c_double, dp, dp, dp]
@@ -147,6 +151,19 @@ def params(cfg: Config) -> dict:
 def set_threads(n: int) -> int:
     """OpenMP thread count of the oracle's sweeps; returns the count in effect."""
     return lib().orc_set_threads(int(n))
+
+
+def spline_eval(x, y, xq):
+    x, y = np.ascontiguousarray(x, float), np.ascontiguousarray(y, float)
+    xq = np.ascontiguousarray(xq, float)
+    out = np.empty_like(xq)
+    lib().orc_spline_eval(len(x), _ptr(x), _ptr(y), len(xq), _ptr(xq), _ptr(out))
+    return out
+
+
+def lagrange4(xq, xk, fk):
+    xk, fk = np.ascontiguousarray(xk, float), np.ascontiguousarray(fk, float)
+    return lib().orc_lagrange4(float(xq), _ptr(xk), _ptr(fk))
 
 
 def noise(seed: int, instance: int, idx: int) -> float:
@@ -234,6 +251,17 @@ class Oracle:
         out = {k: np.empty((n2, n1)) for k in ("u", "v", "w", "T", "psi", "vort")}
         lib().orc_midplane_slice(self.h, int(axis), *[_ptr(out[k]) for k in
                                                       ("u", "v", "w", "T", "psi", "vort")])
+        return out
+
+    def psi_vorticity_2d(self, stats: bool = True) -> dict:
+        """2D stream function, vorticity (ny, nx) and the |psi| statistics."""
+        sh = (self.cfg.ny, self.cfg.nx)
+        psi, vort, st = np.empty(sh), np.empty(sh), np.zeros(5)
+        lib().orc_psi_vort_2d(self.h, _ptr(psi), _ptr(vort), _ptr(st) if stats else None)
+        out = {"psi": psi, "vort": vort}
+        if stats:
+            out.update(zip(("psi_center_coarse", "psi_center_fine", "psi_abs_max", "x_at_max",
+                            "y_at_max"), st.tolist()))
         return out
 
     def midplane(self) -> dict:

@@ -1575,3 +1575,177 @@ void orc_midplane_slice(const orc_state *s, int axis, double *uo, double *vo, do
             }
     for (int c = 0; c < 4; ++c) free(sl[c]);
 }
+
+/* ---------------------------------------------------------------------------------------
+ * 2D stream function, vorticity and |psi| statistics: calc_psi_vort_and_output
+ * (2DRB.F90:2657-2770), output_psi_center_abs (:3085-3160; 4-point Lagrange interpolation in
+ * x then y around the centre), calc_psi_absmax_fine_spline (:2873-2989; natural cubic splines
+ * through the wall zeros and the node values, along x on every row, then along y on every
+ * column of a fine grid of 10000 intervals per unit length), spline_natural / splint
+ * (:3000-3074).
+ * ------------------------------------------------------------------------------------- */
+static void spline_nat(int n, const double *x, const double *y, double *y2, double *u)
+{
+    y2[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < n - 1; ++i) {
+        const double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        const double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) /
+                    (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    y2[n - 1] = 0.0;
+    for (int k = n - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+static double splint_eval(int n, const double *xa, const double *ya, const double *y2a, double x)
+{
+    int klo = 0, khi = n - 1;
+    while (khi - klo > 1) {
+        const int k = (khi + klo) / 2;
+        if (xa[k] > x) khi = k;
+        else klo = k;
+    }
+    const double h = xa[khi] - xa[klo];
+    if (h == 0.0) return ya[klo];
+    const double a = (xa[khi] - x) / h, b = (x - xa[klo]) / h;
+    return a * ya[klo] + b * ya[khi] + ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[khi]) * (h * h) / 6.0;
+}
+
+static double lagrange4(double xq, const double *xk, const double *fk)
+{
+    double fq = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        double basis = 1.0;
+        for (int b = 0; b < 4; ++b)
+            if (b != a) basis = basis * (xq - xk[b]) / (xk[a] - xk[b]);
+        fq = fq + fk[a] * basis;
+    }
+    return fq;
+}
+
+void orc_psi_vort_2d(const orc_state *s, double *psi_out, double *vort_out, double *stats)
+{
+    const orc_params *p = &s->p;
+    const int nx = NX, ny = NY;
+    const double dx = 1.0 / p->L, dy = 1.0 / p->L, coef = p->L / p->diffusivity;
+    const double *xp = s->xp, *yp = s->yp;
+#define U2(i, j) s->u[IDX(s, (i) - 1, (j) - 1, 0)]
+#define V2(i, j) s->v[IDX(s, (i) - 1, (j) - 1, 0)]
+    double *psi = (double *)malloc(sizeof(double) * (size_t)nx * ny);
+#define PSI(i, j) psi[((i) - 1) + (size_t)nx * ((j) - 1)]
+    for (int i = 1; i <= nx; ++i) {
+        const double u1 = U2(i, 1) * coef, u2 = U2(i, 2) * coef;
+        PSI(i, 1) = dy * (21.0 * u1 - u2) / 72.0;
+        for (int j = 2; j <= ny; ++j) {
+            const int m = j - 1;
+            double um;
+            if (m == 1) um = (3.0 / 8.0 * U2(i, 1) + 3.0 / 4.0 * U2(i, 2) - 1.0 / 8.0 * U2(i, 3)) * coef;
+            else um = (-1.0 / 8.0 * U2(i, m - 1) + 3.0 / 4.0 * U2(i, m) + 3.0 / 8.0 * U2(i, m + 1)) * coef;
+            const double inc = dy / 6.0 * ((U2(i, j - 1) * coef) + 4.0 * um + (U2(i, j) * coef));
+            PSI(i, j) = PSI(i, j - 1) + inc;
+        }
+    }
+    if (psi_out) memcpy(psi_out, psi, sizeof(double) * (size_t)nx * ny);
+    if (vort_out)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                double dvdx, dudy;
+                if (i == 1) dvdx = (3.0 * V2(1, j) + V2(2, j) - 4.0 * 0.0) / (3.0 * dx);
+                else if (i == nx) dvdx = (-3.0 * V2(nx, j) - V2(nx - 1, j) + 4.0 * 0.0) / (3.0 * dx);
+                else dvdx = (V2(i + 1, j) - V2(i - 1, j)) / (2.0 * dx);
+                if (j == 1) dudy = (3.0 * U2(i, 1) + U2(i, 2) - 4.0 * 0.0) / (3.0 * dy);
+                else if (j == ny) dudy = (-3.0 * U2(i, ny) - U2(i, ny - 1) + 4.0 * 0.0) / (3.0 * dy);
+                else dudy = (U2(i, j + 1) - U2(i, j - 1)) / (2.0 * dy);
+                vort_out[(i - 1) + (size_t)nx * (j - 1)] = coef * (dvdx - dudy);
+            }
+    if (stats) {
+        /* coarse centre value */
+        const double x0 = 0.5 * xp[nx + 1], y0 = 0.5 * yp[ny + 1];
+        double pc;
+        if (nx < 4 || ny < 4) {
+            pc = PSI((nx + 1) / 2, (ny + 1) / 2);
+        } else {
+            int i0 = 1;
+            while (i0 < nx && xp[i0 + 1] <= x0) ++i0;
+            i0 = i0 - 1 < nx - 3 ? i0 - 1 : nx - 3;
+            if (i0 < 1) i0 = 1;
+            int j0 = 1;
+            while (j0 < ny && yp[j0 + 1] <= y0) ++j0;
+            j0 = j0 - 1 < ny - 3 ? j0 - 1 : ny - 3;
+            if (j0 < 1) j0 = 1;
+            double x4[4], y4[4], f4[4], gx[4];
+            for (int q = 0; q < 4; ++q) { x4[q] = xp[i0 + q]; y4[q] = yp[j0 + q]; }
+            for (int q = 0; q < 4; ++q) {
+                for (int pp = 0; pp < 4; ++pp) f4[pp] = PSI(i0 + pp, j0 + q);
+                gx[q] = lagrange4(x0, x4, f4);
+            }
+            pc = lagrange4(y0, y4, gx);
+        }
+        stats[0] = fabs(pc);
+        /* fine spline grid */
+        const double xLen = xp[nx + 1], yLen = yp[ny + 1];
+        const int nfx = (int)fmax(2.0, (double)lround(xLen * 10000.0) + 1.0);
+        const int nfy = (int)fmax(2.0, (double)lround(yLen * 10000.0) + 1.0);
+        const int nxe = nx + 2, nye = ny + 2;
+        double *xe = (double *)malloc(sizeof(double) * nxe), *ye = (double *)malloc(sizeof(double) * nye);
+        xe[0] = 0.0; xe[nxe - 1] = xp[nx + 1];
+        for (int i = 1; i <= nx; ++i) xe[i] = xp[i];
+        ye[0] = 0.0; ye[nye - 1] = yp[ny + 1];
+        for (int j = 1; j <= ny; ++j) ye[j] = yp[j];
+        const int nmax = nxe > nye ? nxe : nye;
+        double *row = (double *)malloc(sizeof(double) * nmax), *y2 = (double *)malloc(sizeof(double) * nmax);
+        double *wk = (double *)malloc(sizeof(double) * nmax);
+        double *pcx = (double *)malloc(sizeof(double) * ny);
+        double *pxf = (double *)malloc(sizeof(double) * (size_t)nfx * ny);
+        for (int j = 1; j <= ny; ++j) {
+            row[0] = 0.0; row[nxe - 1] = 0.0;
+            for (int i = 1; i <= nx; ++i) row[i] = PSI(i, j);
+            spline_nat(nxe, xe, row, y2, wk);
+            pcx[j - 1] = splint_eval(nxe, xe, row, y2, 0.5 * xLen);
+            for (int k = 0; k < nfx; ++k) {
+                const double xq = xLen * (double)k / (double)(nfx - 1);
+                pxf[(size_t)k * ny + (j - 1)] = splint_eval(nxe, xe, row, y2, xq);
+            }
+        }
+        row[0] = 0.0; row[nye - 1] = 0.0;
+        for (int j = 1; j <= ny; ++j) row[j] = pcx[j - 1];
+        spline_nat(nye, ye, row, y2, wk);
+        stats[1] = fabs(splint_eval(nye, ye, row, y2, 0.5 * yLen));
+        double best = -1.0, xb = 0.0, yb = 0.0;
+        for (int k = 0; k < nfx; ++k) {
+            row[0] = 0.0; row[nye - 1] = 0.0;
+            for (int j = 1; j <= ny; ++j) row[j] = pxf[(size_t)k * ny + (j - 1)];
+            spline_nat(nye, ye, row, y2, wk);
+            for (int l = 0; l < nfy; ++l) {
+                const double yq = yLen * (double)l / (double)(nfy - 1);
+                const double v = fabs(splint_eval(nye, ye, row, y2, yq));
+                if (v > best) {
+                    best = v;
+                    xb = xLen * (double)k / (double)(nfx - 1);
+                    yb = yq;
+                }
+            }
+        }
+        stats[2] = best;
+        stats[3] = xb;
+        stats[4] = yb;
+        free(xe); free(ye); free(row); free(y2); free(wk); free(pcx); free(pxf);
+    }
+    free(psi);
+#undef PSI
+#undef U2
+#undef V2
+}
+
+void orc_spline_eval(int n, const double *x, const double *y, int nq, const double *xq, double *out)
+{
+    double *y2 = (double *)malloc(sizeof(double) * n), *u = (double *)malloc(sizeof(double) * n);
+    spline_nat(n, x, y, y2, u);
+    for (int q = 0; q < nq; ++q) out[q] = splint_eval(n, x, y, y2, xq[q]);
+    free(y2);
+    free(u);
+}
+
+double orc_lagrange4(double xq, const double *xk, const double *fk) { return lagrange4(xq, xk, fk); }

@@ -88,6 +88,14 @@ void orc_diagnostics(const orc_state *s, orc_diag *d);
 void orc_midplane(const orc_state *s, orc_mid *m);
 /* 3D mid-plane slice across axis (0 x, 1 y, 2 z): u, v, w scaled by L/kappa, T, stream function
  * and vorticity; n1*n2 values each, first in-plane axis fastest (any pointer may be NULL). */
+/* 2D stream function and vorticity (nx*ny, x fastest) and the |psi| statistics
+ * stats = {|psi| at the centre from 4x4 Lagrange interpolation, |psi| at the centre from natural
+ * splines, max |psi| on the fine spline grid, its x, its y}; any pointer may be NULL. */
+void orc_psi_vort_2d(const orc_state *s, double *psi, double *vort, double *stats);
+/* the natural cubic spline through (x, y) evaluated at xq (nq points), and the 4-point
+ * Lagrange interpolant at xq, as used by orc_psi_vort_2d (exposed for their tests) */
+void orc_spline_eval(int n, const double *x, const double *y, int nq, const double *xq, double *out);
+double orc_lagrange4(double xq, const double *xk, const double *fk);
 void orc_midplane_slice(const orc_state *s, int axis, double *u, double *v, double *w, double *T,
                         double *psi, double *vort);
 

@@ -69,6 +69,14 @@ def test_driver_matches_oracle(tmp_path, dims):
     assert float(got.group(1)) == pytest.approx(mp["u_max"], rel=1e-7)
     got = re.search(rf"^v_mid_max{tag} =\s+(\S+)", out, re.M)
     assert float(got.group(1)) == pytest.approx(mp["v_max"], rel=1e-7)
+    if dims == 2:   # the stream-function report of calc_psi_vort_and_output
+        ps = o.psi_vorticity_2d()
+        got = re.search(r"^max\(\|psi\|\) =\s+(\S+)\s+x\* =\s+(\S+)\s+y\* =\s+(\S+)", out, re.M)
+        assert got, out
+        assert float(got.group(1)) == pytest.approx(ps["psi_abs_max"], rel=1e-7)
+        assert float(got.group(2)) == pytest.approx(ps["x_at_max"], rel=1e-7, abs=1e-12)
+        got = re.search(r"^abs\(psi_center_fine\) =\s+(\S+)", out, re.M)
+        assert float(got.group(1)) == pytest.approx(ps["psi_center_fine"], rel=1e-7)
     recs = _read_records(tmp_path / "backupFile3D-100.bin")
     s = o.state()
     qf, qt = (19, 7) if dims == 3 else (9, 5)

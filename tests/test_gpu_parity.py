@@ -296,3 +296,16 @@ def test_midplane_slices_match_oracle(axis):
     for k in ("psi", "vort"):
         scale = np.abs(b[k]).max()
         np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-13 * scale)
+
+
+@pytest.mark.parametrize("case", ["rb_periodic", "side_heated"])
+def test_psi_vorticity_2d_match_oracle(case):
+    kw = dict(rayleigh=1e5, thot=1.0, tcold=0.0)
+    kw.update(dict(vel_x=1, temp_x=2) if case == "rb_periodic" else
+              dict(side_heated=1, vel_x=0, temp_x=0, temp_y=1))
+    sol, ora = _run_pair(2, 48, 32, 1, (300,), **kw)
+    a, b = sol.psi_vorticity_2d(), ora.psi_vorticity_2d()
+    np.testing.assert_array_equal(a["psi"], b["psi"])
+    np.testing.assert_array_equal(a["vort"], b["vort"])
+    for k in ("psi_center_coarse", "psi_center_fine", "psi_abs_max", "x_at_max", "y_at_max"):
+        assert a[k] == pytest.approx(b[k], rel=1e-12, abs=1e-15), (k, a[k], b[k])

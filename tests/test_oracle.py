@@ -457,3 +457,44 @@ def test_midplane_slice_stream_function_and_vorticity_closed_form(axis):
     comp = {0: "v", 1: "u", 2: "u"}[axis]
     np.testing.assert_allclose(sl[comp], np.broadcast_to(coef * a * s[:, None] ** 2, sl[comp].shape),
                                rtol=1e-12)
+
+
+def test_spline_and_lagrange_against_scipy_numpy():
+    # the natural cubic spline and the 4-point Lagrange interpolant of the 2D |psi| statistics
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(8)
+    x = np.concatenate([[0.0], np.sort(rng.uniform(0.02, 0.98, 9)), [1.0]])
+    y = rng.normal(size=x.size)
+    xq = np.linspace(0.0, 1.0, 57)
+    ref = CubicSpline(x, y, bc_type="natural")(xq)
+    np.testing.assert_allclose(orc.spline_eval(x, y, xq), ref, rtol=1e-12, atol=1e-12)
+    xk = np.sort(rng.uniform(0, 1, 4))
+    fk = rng.normal(size=4)
+    coeff = np.polyfit(xk, fk, 3)
+    for xq1 in (0.3, 0.55, xk[1]):
+        assert orc.lagrange4(xq1, xk, fk) == pytest.approx(np.polyval(coeff, xq1), rel=1e-10, abs=1e-12)
+
+
+def test_psi_vorticity_2d_closed_form():
+    # u = a y (Y - y) (zero on both walls y = 0, Y), v = 0: the 2D stream function
+    # psi = (L/kappa) a (Y y^2 / 2 - y^3 / 3) and the vorticity -(L/kappa) a (Y - 2 y) are exact
+    # for the first-cell rule, Simpson's rule with quadratic midpoints and the one-sided wall
+    # differences through the wall value 0 (2DRB.F90:2657-2770).
+    cfg = orc.Config(dims=2, nx=12, ny=10, rayleigh=1e5, mach=0.1, thot=1.0, tcold=0.0,
+                     vel_x=1, temp_x=2)
+    p = orc.params(cfg)
+    L, coef = p["L"], p["L"] / p["diffusivity"]
+    a = 0.02
+    yp = (np.arange(cfg.ny) + 0.5) / L
+    Y = cfg.ny / L
+    u = np.broadcast_to((a * yp * (Y - yp))[None, :, None], cfg.shape)
+    zero = np.zeros(cfg.shape)
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    assert o.set_fg(*_velocity_state(cfg, u, zero, zero)) == 0
+    r = o.psi_vorticity_2d(stats=False)
+    psi = coef * a * (Y * yp ** 2 / 2 - yp ** 3 / 3)
+    np.testing.assert_allclose(r["psi"], np.broadcast_to(psi[:, None], r["psi"].shape), rtol=1e-10)
+    vort = -coef * a * (Y - 2 * yp)
+    np.testing.assert_allclose(r["vort"], np.broadcast_to(vort[:, None], r["vort"].shape),
+                               rtol=1e-9, atol=1e-9 * coef * a)
