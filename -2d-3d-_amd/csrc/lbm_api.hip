@@ -74,7 +74,9 @@ static int fail(lbm_handle *h, int code, const char *fmt, ...)
             return fail((h), LBM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
     } while (0)
 
-static void compute_params(const lbm_config *c, lbm_params *p)
+// Returns false when the legacy thermal scheme's paraA is outside its range (the reference
+// stops there: 2DRB/3DRB.F90:378-390, 2DRB/2DRB.F90:404-413).
+static bool compute_params(const lbm_config *c, lbm_params *p)
 {
     // module commondata3d, 2DRB/3DRB.F90:73-123 (2D: 2DRB/2DRB.F90:61-115)
     memset(p, 0, sizeof(*p));
@@ -92,7 +94,21 @@ static void compute_params(const lbm_config *c, lbm_params *p)
     p->taug = c->dims == 3 ? 0.5 + p->diffusivity / 0.25 : 0.5 + (p->tauf - 0.5) / c->prandtl;
     p->q_nu = 1.0;
     p->q_k = 1.0 / p->taug;
+    bool ok = true;
+    if (c->legacy_thermal) {
+        if (c->dims == 3) {
+            p->para_a = 42.0 * sqrt(3.0) * p->diffusivity - 6.0;
+            ok = !(p->para_a >= 1.0 || p->para_a <= -6.0);
+        } else {
+            p->para_a = 20.0 * sqrt(3.0) * p->diffusivity - 4.0;
+            ok = !(p->para_a >= 1.0 || p->para_a <= -4.0);
+        }
+        p->taug = 0.0;
+        p->q_k = 3.0 - sqrt(3.0);
+        p->q_nu = 4.0 * sqrt(3.0) - 6.0;
+    }
     p->sg = 1.0 - 0.5 * p->q_k;
+    return ok;
 }
 
 extern "C" const char *lbm_error_string(int code)
@@ -226,11 +242,16 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     if (c.dims == 3 && c.temp_z == LBM_T_CONST) return LBM_ERR_UNSUPPORTED;
     if (c.precision < LBM_FP64 || c.precision > LBM_FP16_SHIFT) return LBM_ERR_ARG;
     if (c.precision != LBM_FP64 && c.dims != 3) return LBM_ERR_UNSUPPORTED;
+    if (c.legacy_thermal < 0 || c.legacy_thermal > 1) return LBM_ERR_ARG;
+    if (c.legacy_thermal && c.precision != LBM_FP64) return LBM_ERR_UNSUPPORTED;
 
     lbm_handle *h = (lbm_handle *)calloc(1, sizeof(lbm_handle));
     if (!h) return LBM_ERR_NOMEM;
     h->cfg = c;
-    compute_params(&c, &h->p);
+    if (!compute_params(&c, &h->p)) {   // legacy thermal scheme outside its paraA range
+        free(h);
+        return LBM_ERR_ARG;
+    }
     h->qf = c.dims == 3 ? 19 : 9;
     h->qt = c.dims == 3 ? 7 : 5;
     h->N = (uint64_t)c.nx * c.ny * c.nz;
@@ -246,6 +267,19 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     k.gbeta = h->p.gbeta; k.tref = h->p.tref; k.thot = c.thot; k.tcold = c.tcold;
     k.sv = 1.0 / h->p.tauf; k.sq = h->p.s_q;
     k.qk = h->p.q_k; k.qnu = h->p.q_nu; k.sg = h->p.sg;
+    // thermal scheme constants (2DRB/3DRB.F90:116-122, :671-676, :1153-1157, :1267-1316;
+    // 2DRB/2DRB.F90:108-114, :514-522, :949-953, :1048-1098)
+    if (c.dims == 3) {
+        k.neqc = c.legacy_thermal ? h->p.para_a : -0.75;
+        k.abb = c.legacy_thermal ? (6.0 + h->p.para_a) / 21.0 : 2.0 * (1.0 / 8.0);
+        k.wt0 = c.legacy_thermal ? (1.0 - h->p.para_a) / 7.0 : 1.0 / 4.0;
+        k.wt1 = c.legacy_thermal ? (6.0 + h->p.para_a) / 42.0 : 1.0 / 8.0;
+    } else {
+        k.neqc = c.legacy_thermal ? h->p.para_a : -2.0 / 3.0;
+        k.abb = c.legacy_thermal ? (4.0 + h->p.para_a) / 10.0 : 2.0 * (1.0 / 6.0);
+        k.wt0 = c.legacy_thermal ? (1.0 - h->p.para_a) / 5.0 : 1.0 / 3.0;
+        k.wt1 = c.legacy_thermal ? (h->p.para_a + 4.0) / 20.0 : 1.0 / 6.0;
+    }
     h->bx = c.nx >= 256 ? 256 : ((c.nx + 63) / 64) * 64;
     k.nxb = (c.nx + h->bx - 1) / h->bx;
     const uint64_t nb = (uint64_t)k.nxb * c.ny * c.nz;

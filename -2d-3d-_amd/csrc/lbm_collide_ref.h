@@ -179,7 +179,7 @@ __device__ __forceinline__ void ref_collide_d3q7(const double *G, double *P, dou
     neq[1] = Bx;
     neq[2] = By;
     neq[3] = Bz;
-    neq[4] = -0.75 * Tl;
+    neq[4] = k.neqc * Tl;   // -0.75 T (legacy scheme: paraA T, 3DRB.F90:1153-1157)
     neq[5] = 0.0;
     neq[6] = 0.0;
     q[0] = 0.0; q[1] = k.qk; q[2] = k.qk; q[3] = k.qk; q[4] = k.qnu; q[5] = k.qnu; q[6] = k.qnu;
@@ -276,7 +276,7 @@ __device__ __forceinline__ void ref_collide_d2q5(const double *G, double *P, dou
     neq[0] = Tl;
     neq[1] = Tl * u;
     neq[2] = Tl * v;
-    neq[3] = Tl * (-2.0 / 3.0);
+    neq[3] = Tl * k.neqc;   // T (-2/3) (legacy scheme: T paraA, 2DRB.F90:949-953)
     neq[4] = 0.0;
     q[0] = 0.0; q[1] = k.qk; q[2] = k.qk; q[3] = k.qnu; q[4] = k.qnu;
     np[0] = nm[0] - q[0] * (nm[0] - neq[0]);

@@ -107,10 +107,11 @@ __device__ __forceinline__ double heat_in(const double *__restrict__ P, uint64_t
     if (kind == T_PERIODIC) return P[A * N + c + wrap];
     const double back = P[L::opp[A] * N + c];
     if (kind == T_ADIABATIC) return back;
-    constexpr double wa = (L::Q == 7) ? w7(A) : w5(A);
+    // kp.abb = 2 wT_A (every moving direction has the same weight); legacy scheme
+    // (6 + paraA) / 21 (3D) or (4 + paraA) / 10 (2D)
     constexpr bool low_wall = (ex + ey + ez) > 0;   // entered through the low-index wall
     const double tw = low_wall ? kp.thot : kp.tcold;
-    return -back + 2.0 * wa * tw;
+    return -back + kp.abb * tw;
 }
 
 template <class L, int... A>
@@ -471,7 +472,7 @@ __global__ void __launch_bounds__(256) k_init(KParams kp, InitParams ip, double 
         for (int a = 0; a < qf; ++a)
             Fout[a * N + c] = qf == 19 ? w19(a) : w9(a);
         for (int a = 0; a < qt; ++a)
-            Gout[a * N + c] = (qt == 7 ? w7(a) : w5(a)) * T;
+            Gout[a * N + c] = (a == 0 ? kp.wt0 : kp.wt1) * T;
         Bp[c] = 0.0;
         Bp[N + c] = 0.0;
         Bp[2 * N + c] = 0.0;
@@ -894,7 +895,7 @@ struct FormF64 {
     using S = double;
     using B = double;
     struct P {
-        double thot, tcold;
+        double thot, tcold, abb;
     };
     struct Node {};
     static __device__ __forceinline__ Node node(const KParams &, const P &, int, int) { return Node{}; }
@@ -955,7 +956,7 @@ struct FormF64 {
         const double back = gp[D3Q7::opp[a]];
         if (kind == T_ADIABATIC) return back;
         const double tw = (a & 1) ? p.thot : p.tcold;
-        return -back + 2.0 * w7(a) * tw;
+        return -back + p.abb * tw;
     }
 };
 
@@ -972,7 +973,7 @@ void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, 
 {
     FusedInstance<FormF64> d;
     d.kp = kp;
-    d.fp = FormF64::P{kp.thot, kp.tcold};
+    d.fp = FormF64::P{kp.thot, kp.tcold, kp.abb};
     d.fg = fg;
     d.Fcur = (const double *)p.Fcur;
     d.Fnext = (double *)p.Fnext;

@@ -61,6 +61,10 @@ struct KParams {
     double gbeta, tref, thot, tcold;
     double sv, sq;    // 1/tau_f (s_e = s_eps = s_nu = s_pi) and s_q = s_m
     double qk, qnu, sg;
+    // thermal scheme constants (default or EnableLegacyThermalScheme): the energy-moment
+    // equilibrium factor (neq = neqc T), the anti-bounce-back wall factor (g_in = -g_out(opp) +
+    // abb Tw) and the rest / moving thermal weights (initial state)
+    double neqc, abb, wt0, wt1;
 };
 
 // Launch geometry of the one-pass kernel (lbm_fused.h): ntx x nty tiles of (TX-2) x (TY-2)

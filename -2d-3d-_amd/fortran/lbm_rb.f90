@@ -20,12 +20,14 @@ module lbm_rb
     real(c_double) :: rayleigh, prandtl, mach, thot, tcold
     integer(c_int32_t) :: side_heated, vel_x, vel_z, temp_x, temp_y, temp_z, use_g
     integer(c_int32_t) :: precision, device
+    integer(c_int32_t) :: legacy_thermal = 0
   end type lbm_config
 
   type, bind(C) :: lbm_params
     real(c_double) :: L, tref, tauf, viscosity, diffusivity, gbeta, time_unit, velocity_unit
     real(c_double) :: s_e, s_eps, s_nu, s_pi, s_q, s_m
     real(c_double) :: taug, q_k, q_nu, sg
+    real(c_double) :: para_a
   end type lbm_params
 
   type, bind(C) :: lbm_diag

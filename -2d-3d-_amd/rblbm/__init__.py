@@ -43,13 +43,13 @@ class _Config(C.Structure):
                 ("thot", C.c_double), ("tcold", C.c_double), ("side_heated", C.c_int32),
                 ("vel_x", C.c_int32), ("vel_z", C.c_int32), ("temp_x", C.c_int32),
                 ("temp_y", C.c_int32), ("temp_z", C.c_int32), ("use_g", C.c_int32),
-                ("precision", C.c_int32), ("device", C.c_int32)]
+                ("precision", C.c_int32), ("device", C.c_int32), ("legacy_thermal", C.c_int32)]
 
 
 class _Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "L", "tref", "tauf", "viscosity", "diffusivity", "gbeta", "time_unit", "velocity_unit",
-        "s_e", "s_eps", "s_nu", "s_pi", "s_q", "s_m", "taug", "q_k", "q_nu", "sg")]
+        "s_e", "s_eps", "s_nu", "s_pi", "s_q", "s_m", "taug", "q_k", "q_nu", "sg", "para_a")]
 
 
 class _Diag(C.Structure):
@@ -135,6 +135,7 @@ class Config:
     use_g: int = 1
     precision: int = FP64
     device: int = 0
+    legacy: int = 0           # EnableLegacyThermalScheme (FP64 form only)
 
     @property
     def shape(self):
@@ -150,6 +151,7 @@ class Config:
 
     def c(self) -> _Config:
         d = asdict(self)
+        d["legacy_thermal"] = d.pop("legacy")
         return _Config(**d)
 
 

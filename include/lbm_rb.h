@@ -56,6 +56,8 @@ typedef struct lbm_config {
     int32_t use_g;                /* EnableUseG: M1G heat-flux correction */
     int32_t precision;            /* LBM_FP64 | LBM_FP32_SHIFT | LBM_FP16_SHIFT */
     int32_t device;               /* HIP device ordinal */
+    int32_t legacy_thermal;       /* EnableLegacyThermalScheme (FP64 form only): fixed Qk, Qnu,
+                                     rest weight from paraA (2DRB/3DRB.F90:93, :116-118) */
 } lbm_config;
 
 /* Derived parameters: module commondata3d, 2DRB/3DRB.F90:73-123 (2D: 2DRB/2DRB.F90:61-115). */
@@ -63,6 +65,7 @@ typedef struct lbm_params {
     double L, tref, tauf, viscosity, diffusivity, gbeta, time_unit, velocity_unit;
     double s_e, s_eps, s_nu, s_pi, s_q, s_m;
     double taug, q_k, q_nu, sg;
+    double para_a;                /* legacy thermal scheme's paraA (0 otherwise) */
 } lbm_params;
 
 /* Diagnostics, FP64 with compensated sums. */

@@ -31,13 +31,15 @@ class _Cfg(C.Structure):
         ("thot", C.c_double), ("tcold", C.c_double),
         ("side_heated", C.c_int), ("vel_x", C.c_int), ("vel_z", C.c_int),
         ("temp_x", C.c_int), ("temp_y", C.c_int), ("temp_z", C.c_int), ("use_g", C.c_int),
+        ("legacy", C.c_int),
     ]
 
 
 class _Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "L", "Tref", "tauf", "viscosity", "diffusivity", "gbeta", "time_unit", "velocity_unit",
-        "Se", "Seps", "Snu", "Spi", "Sq", "Sm", "taug", "Qk", "Qnu", "SG", "heat_flux_scale")]
+        "Se", "Seps", "Snu", "Spi", "Sq", "Sm", "taug", "Qk", "Qnu", "SG", "heat_flux_scale",
+        "paraA")]
 
 
 class _Diag(C.Structure):
@@ -121,12 +123,13 @@ class Config:
     temp_y: int = T_CONST
     temp_z: int = T_ADIABATIC
     use_g: int = 1
+    legacy: int = 0
 
     def c(self) -> _Cfg:
         nz = self.nz if self.dims == 3 else 1
         return _Cfg(self.dims, self.nx, self.ny, nz, self.rayleigh, self.prandtl, self.mach,
                     self.thot, self.tcold, self.side_heated, self.vel_x, self.vel_z,
-                    self.temp_x, self.temp_y, self.temp_z, self.use_g)
+                    self.temp_x, self.temp_y, self.temp_z, self.use_g, self.legacy)
 
     @property
     def qf(self):

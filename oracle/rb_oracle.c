@@ -81,6 +81,20 @@ int orc_params_compute(const orc_cfg *c, orc_params *p)
     }
     p->Qnu = 1.0;
     p->Qk = 1.0 / p->taug;
+    if (c->legacy) {
+        /* EnableLegacyThermalScheme: 3DRB.F90:93, :116-118, :378-390; 2DRB.F90:95, :108-110,
+         * :404-413.  The diffusivity enters through paraA (the rest weight) instead of Qk. */
+        if (c->dims == 3) {
+            p->paraA = 42.0 * sqrt(3.0) * p->diffusivity - 6.0;
+            if (p->paraA >= 1.0 || p->paraA <= -6.0) return -1;
+        } else {
+            p->paraA = 20.0 * sqrt(3.0) * p->diffusivity - 4.0;
+            if (p->paraA >= 1.0 || p->paraA <= -4.0) return -1;
+        }
+        p->taug = 0.0;
+        p->Qk = 3.0 - sqrt(3.0);
+        p->Qnu = 4.0 * sqrt(3.0) - 6.0;
+    }
     p->SG = 1.0 - 0.5 * p->Qk;                           /* 3DRB.F90:1113, 2DRB.F90:915 */
     return 0;
 }
@@ -124,6 +138,17 @@ static void *xcalloc(size_t n, size_t sz)
     return p;
 }
 
+/* Anti-bounce-back wall term: 2 wT Tw, legacy (6 + paraA) / 21 Tw (3D, 3DRB.F90:1267-1272,
+ * :1311-1316) and (4 + paraA) / 10 Tw (2D, 2DRB.F90:1048-1055, :1091-1098). */
+static double abb3(const orc_state *s, double w, double Tw)
+{
+    return s->c.legacy ? (6.0 + s->p.paraA) / 21.0 * Tw : 2.0 * w * Tw;
+}
+static double abb2(const orc_state *s, double w, double Tw)
+{
+    return s->c.legacy ? (4.0 + s->p.paraA) / 10.0 * Tw : 2.0 * w * Tw;
+}
+
 orc_state *orc_create(const orc_cfg *cfg)
 {
     if (cfg->dims != 2 && cfg->dims != 3) return NULL;
@@ -132,7 +157,10 @@ orc_state *orc_create(const orc_cfg *cfg)
     orc_state *s = (orc_state *)xcalloc(1, sizeof(orc_state));
     s->c = *cfg;
     if (s->c.dims == 2) s->c.nz = 1;
-    orc_params_compute(&s->c, &s->p);
+    if (orc_params_compute(&s->c, &s->p) != 0) {   /* legacy scheme outside its paraA range */
+        free(s);
+        return NULL;
+    }
     s->qf = (s->c.dims == 3) ? 19 : 9;
     s->qt = (s->c.dims == 3) ? 7 : 5;
     s->n = (long)s->c.nx * s->c.ny * s->c.nz;
@@ -159,6 +187,10 @@ orc_state *orc_create(const orc_cfg *cfg)
         for (int a = 7; a <= 18; ++a) s->omega[a] = 1.0 / 36.0;
         s->omegaT[0] = 1.0 / 4.0;
         for (int a = 1; a <= 6; ++a) s->omegaT[a] = 1.0 / 8.0;
+        if (s->c.legacy) {   /* 3DRB.F90:671-673 */
+            s->omegaT[0] = (1.0 - s->p.paraA) / 7.0;
+            for (int a = 1; a <= 6; ++a) s->omegaT[a] = (6.0 + s->p.paraA) / 42.0;
+        }
     } else {
         /* 2DRB.F90:506-524 */
         s->omega[0] = 4.0 / 9.0;
@@ -166,6 +198,10 @@ orc_state *orc_create(const orc_cfg *cfg)
         for (int a = 5; a <= 8; ++a) s->omega[a] = 1.0 / 36.0;
         s->omegaT[0] = 1.0 / 3.0;
         for (int a = 1; a <= 4; ++a) s->omegaT[a] = 1.0 / 6.0;
+        if (s->c.legacy) {   /* 2DRB.F90:514-518 */
+            s->omegaT[0] = (1.0 - s->p.paraA) / 5.0;
+            for (int a = 1; a <= 4; ++a) s->omegaT[a] = (s->p.paraA + 4.0) / 20.0;
+        }
     }
     return s;
 }
@@ -285,7 +321,9 @@ void orc_init_exact(orc_state *s, const double *T0)
             }
             for (int a = 0; a < 7; ++a) {
                 double eu = (double)EX3[a] * u + (double)EY3[a] * v + (double)EZ3[a] * w;
-                s->g[n * 7 + a] = s->omegaT[a] * T * (1.0 + 4.0 * eu);
+                /* thermalGeqCoeff: 1/cs2T, legacy 7/((6 + paraA) cs2T) (3DRB.F90:118, :122) */
+                const double gc = c->legacy ? 7.0 / ((6.0 + s->p.paraA) * 0.25) : 4.0;
+                s->g[n * 7 + a] = s->omegaT[a] * T * (1.0 + gc * eu);
             }
         } else {
             double us2 = u * u + v * v;
@@ -296,7 +334,9 @@ void orc_init_exact(orc_state *s, const double *T0)
             }
             for (int a = 0; a < 5; ++a) {
                 double un = u * EX2[a] + v * EY2[a];
-                s->g[n * 5 + a] = s->omegaT[a] * T * (1.0 + 3.0 * un);
+                /* thermalGeqCoeff: 3, legacy 10/(4 + paraA) (2DRB.F90:110, :114) */
+                const double gc = c->legacy ? 10.0 / (4.0 + s->p.paraA) : 3.0;
+                s->g[n * 5 + a] = s->omegaT[a] * T * (1.0 + gc * un);
             }
         }
         if (c->use_g) {
@@ -601,7 +641,7 @@ static void collisionT3d(orc_state *s)
                 neq[1] = Bx;
                 neq[2] = By;
                 neq[3] = Bz;
-                neq[4] = -0.75 * Tl;
+                neq[4] = s->c.legacy ? s->p.paraA * Tl : -0.75 * Tl;   /* 3DRB.F90:1153-1157 */
                 neq[5] = 0.0;
                 neq[6] = 0.0;
                 q[0] = 0.0; q[1] = Qk; q[2] = Qk; q[3] = Qk; q[4] = Qnu; q[5] = Qnu; q[6] = Qnu;
@@ -639,8 +679,8 @@ static void bouncebackT3d(orc_state *s)
                     if (EX3[a] == 1) g[lo] = P[HDX(s, NX - 1, j, k) * 7 + a];
                     if (EX3[a] == -1) g[hi] = P[HDX(s, 0, j, k) * 7 + a];
                 } else if (s->c.temp_x == ORC_T_CONST) {
-                    if (EX3[a] == 1) g[lo] = -P[HDX(s, 0, j, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Th;
-                    if (EX3[a] == -1) g[hi] = -P[HDX(s, NX - 1, j, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Tc;
+                    if (EX3[a] == 1) g[lo] = -P[HDX(s, 0, j, k) * 7 + OPP7[a]] + abb3(s, wT[a], Th);
+                    if (EX3[a] == -1) g[hi] = -P[HDX(s, NX - 1, j, k) * 7 + OPP7[a]] + abb3(s, wT[a], Tc);
                 } else {
                     if (EX3[a] == 1) g[lo] = P[HDX(s, 0, j, k) * 7 + OPP7[a]];
                     if (EX3[a] == -1) g[hi] = P[HDX(s, NX - 1, j, k) * 7 + OPP7[a]];
@@ -654,8 +694,8 @@ static void bouncebackT3d(orc_state *s)
                     if (EY3[a] == 1) g[lo] = P[HDX(s, i, 0, k) * 7 + OPP7[a]];
                     if (EY3[a] == -1) g[hi] = P[HDX(s, i, NY - 1, k) * 7 + OPP7[a]];
                 } else if (s->c.temp_y == ORC_T_CONST) {
-                    if (EY3[a] == 1) g[lo] = -P[HDX(s, i, 0, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Th;
-                    if (EY3[a] == -1) g[hi] = -P[HDX(s, i, NY - 1, k) * 7 + OPP7[a]] + 2.0 * wT[a] * Tc;
+                    if (EY3[a] == 1) g[lo] = -P[HDX(s, i, 0, k) * 7 + OPP7[a]] + abb3(s, wT[a], Th);
+                    if (EY3[a] == -1) g[hi] = -P[HDX(s, i, NY - 1, k) * 7 + OPP7[a]] + abb3(s, wT[a], Tc);
                 }
             }
     for (int j = 0; j < NY; ++j)
@@ -842,7 +882,7 @@ static void collisionT2d(orc_state *s)
             neq[0] = Tl;
             neq[1] = Tl * s->u[n];
             neq[2] = Tl * s->v[n];
-            neq[3] = Tl * (-2.0 / 3.0);
+            neq[3] = s->c.legacy ? Tl * s->p.paraA : Tl * (-2.0 / 3.0);   /* 2DRB.F90:949-953 */
             neq[4] = 0.0;
             q[0] = 0.0; q[1] = Qk; q[2] = Qk; q[3] = Qnu; q[4] = Qnu;
             np[0] = nm[0] - q[0] * (nm[0] - neq[0]);
@@ -873,8 +913,8 @@ static void bouncebackT2d(orc_state *s)
             G2(0, j, 1) = Q2(NX - 1, j, 1);
             G2(NX - 1, j, 3) = Q2(0, j, 3);
         } else if (s->c.temp_x == ORC_T_CONST) {
-            G2(0, j, 1) = -Q2(0, j, 3) + 2.0 * wT[3] * Th;
-            G2(NX - 1, j, 3) = -Q2(NX - 1, j, 1) + 2.0 * wT[1] * Tc;
+            G2(0, j, 1) = -Q2(0, j, 3) + abb2(s, wT[3], Th);
+            G2(NX - 1, j, 3) = -Q2(NX - 1, j, 1) + abb2(s, wT[1], Tc);
         } else {
             G2(0, j, 1) = Q2(0, j, 3);
             G2(NX - 1, j, 3) = Q2(NX - 1, j, 1);
@@ -885,8 +925,8 @@ static void bouncebackT2d(orc_state *s)
             G2(i, 0, 2) = Q2(i, 0, 4);
             G2(i, NY - 1, 4) = Q2(i, NY - 1, 2);
         } else if (s->c.temp_y == ORC_T_CONST) {
-            G2(i, 0, 2) = -Q2(i, 0, 4) + 2.0 * wT[4] * Th;
-            G2(i, NY - 1, 4) = -Q2(i, NY - 1, 2) + 2.0 * wT[2] * Tc;
+            G2(i, 0, 2) = -Q2(i, 0, 4) + abb2(s, wT[4], Th);
+            G2(i, NY - 1, 4) = -Q2(i, NY - 1, 2) + abb2(s, wT[2], Tc);
         }
     }
 #undef G2

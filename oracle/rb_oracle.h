@@ -35,12 +35,14 @@ typedef struct orc_cfg {
     int vel_x, vel_z;         /* ORC_VEL_*; y walls are always no-slip */
     int temp_x, temp_y, temp_z; /* ORC_T_* */
     int use_g;                /* EnableUseG: M1G flux correction */
+    int legacy;               /* EnableLegacyThermalScheme: fixed Qk, Qnu, weights from paraA */
 } orc_cfg;
 
 typedef struct orc_params {
     double L, Tref, tauf, viscosity, diffusivity, gbeta, time_unit, velocity_unit;
     double Se, Seps, Snu, Spi, Sq, Sm;      /* D3Q19 rates (2D uses Snu, Sq) */
     double taug, Qk, Qnu, SG, heat_flux_scale;
+    double paraA;             /* legacy thermal scheme: 3D 42 sqrt3 kappa - 6, 2D 20 sqrt3 kappa - 4 */
 } orc_params;
 
 typedef struct orc_diag {

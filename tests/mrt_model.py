@@ -74,9 +74,12 @@ def basis_d2q5(E):
     return np.array([np.ones_like(ex), ex, ey, 5 * e2 - 4, ex ** 2 - ey ** 2])
 
 
-def params(dims, nx, ny, Ra, Pr, Ma, thot, tcold, side_heated=False):
+def params(dims, nx, ny, Ra, Pr, Ma, thot, tcold, side_heated=False, legacy=False):
     """Dimensionless groups of the reference, rebuilt from their physical meaning:
-    the free-fall velocity sqrt(g beta dT L) equals Ma times the lattice sound speed."""
+    the free-fall velocity sqrt(g beta dT L) equals Ma times the lattice sound speed.
+    Legacy thermal scheme: fixed rates Qk = 3 - sqrt3, Qnu = 4 sqrt3 - 6; the diffusivity
+    kappa = cT^2 (1/Qk - 1/2) then fixes the moving-direction weight, with the thermal sound
+    speed cT^2 = sum_a wT_a e_ax^2 = 2 wT_1 and wT_0 = 1 - (q - 1) wT_1."""
     L = float(nx if side_heated else ny)
     cs = 1 / np.sqrt(3.0)
     U = Ma * cs
@@ -86,6 +89,13 @@ def params(dims, nx, ny, Ra, Pr, Ma, thot, tcold, side_heated=False):
              gbeta=U * U / L, tauf=0.5 + 3 * nu)
     p["taug"] = 0.5 + 4 * kappa if dims == 3 else 0.5 + 3 * kappa
     p["Qk"] = 1 / p["taug"]
+    p["Qnu"] = 1.0
+    if legacy:
+        p["Qk"] = 3 - np.sqrt(3.0)
+        p["Qnu"] = 4 * np.sqrt(3.0) - 6
+        cT2 = kappa / (1 / p["Qk"] - 0.5)
+        p["wT1"] = cT2 / 2
+        p["wT0"] = 1 - (6 if dims == 3 else 4) * p["wT1"]
     # "magic" combination (1/s_q - 1/2)(1/s_nu - 1/2) = 3/16 fixes the energy-flux rate.
     p["Sq"] = 1.0 / (0.5 + (3.0 / 16.0) / (p["tauf"] - 0.5))
     return p
@@ -96,13 +106,13 @@ class Model:
 
     def __init__(self, dims, nx, ny, nz=1, Ra=1e6, Pr=0.71, Ma=0.1, thot=0.5, tcold=-0.5,
                  vel_x=0, vel_z=0, temp_x=1, temp_y=0, temp_z=1, use_g=True,
-                 side_heated=False):
+                 side_heated=False, legacy=False):
         self.dims, self.nx, self.ny, self.nz = dims, nx, ny, (nz if dims == 3 else 1)
         self.thot, self.tcold = thot, tcold
         self.vel_x, self.vel_z = vel_x, vel_z
         self.temp = (temp_x, temp_y, temp_z)
         self.use_g = use_g
-        self.p = params(dims, nx, ny, Ra, Pr, Ma, thot, tcold, side_heated)
+        self.p = params(dims, nx, ny, Ra, Pr, Ma, thot, tcold, side_heated, legacy)
         if dims == 3:
             self.Ef, self.Eg = E3, E3[:7]
             self.M, self.N = basis_d3q19(self.Ef), basis_d3q7(self.Eg)
@@ -110,15 +120,17 @@ class Model:
             sq = self.p["Sq"]
             self.S = np.array([0, s_e, s_e, 0, sq, 0, sq, 0, sq, s_nu, s_nu, s_nu, s_nu,
                                s_nu, s_nu, s_nu, sq, sq, sq])
-            self.Q = np.array([0] + [self.p["Qk"]] * 3 + [1.0] * 3)
+            self.Q = np.array([0] + [self.p["Qk"]] * 3 + [self.p["Qnu"]] * 3)
         else:
             self.Ef, self.Eg = E2, E2[:5]
             self.M, self.N = basis_d2q9(self.Ef), basis_d2q5(self.Eg)
             s_nu, sq = 1 / self.p["tauf"], self.p["Sq"]
             self.S = np.array([0, s_nu, s_nu, 0, sq, 0, sq, s_nu, s_nu])
-            self.Q = np.array([0, self.p["Qk"], self.p["Qk"], 1.0, 1.0])
+            self.Q = np.array([0, self.p["Qk"], self.p["Qk"], self.p["Qnu"], self.p["Qnu"]])
         self.Minv, self.Ninv = np.linalg.inv(self.M), np.linalg.inv(self.N)
         self.wf, self.wg = weights(self.Ef, dims), weights(self.Eg, dims)
+        if legacy:
+            self.wg = np.array([self.p["wT0"]] + [self.p["wT1"]] * (len(self.Eg) - 1))
         self.opp_f, self.opp_g = opposite(self.Ef), opposite(self.Eg)
         self.SG = 1 - 0.5 * self.p["Qk"]
 
@@ -145,7 +157,8 @@ class Model:
         return m
 
     def geq(self, T, u):
-        c = 4.0 if self.dims == 3 else 3.0
+        # first moment of g_eq = u T: c sum_a wT_a e_ax^2 = 1
+        c = 1.0 / (2.0 * self.wg[1])
         eu = np.einsum("...d,qd->...q", u, self.Eg.astype(float))
         return self.wg * T[..., None] * (1 + c * eu)
 

@@ -309,3 +309,25 @@ def test_psi_vorticity_2d_match_oracle(case):
     np.testing.assert_array_equal(a["vort"], b["vort"])
     for k in ("psi_center_coarse", "psi_center_fine", "psi_abs_max", "x_at_max", "y_at_max"):
         assert a[k] == pytest.approx(b[k], rel=1e-12, abs=1e-15), (k, a[k], b[k])
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_legacy_thermal_scheme_bitwise(dims):
+    """EnableLegacyThermalScheme: fixed Qk, Qnu, rest weight from paraA, legacy wall factor;
+    bitwise against the oracle (3D in both kernel forms)."""
+    if dims == 3:
+        kw = dict(rayleigh=1e5, mach=0.1, legacy=1)
+        sol, ora = _run_pair(3, 20, 16, 12, (1, 30), init="exact",
+                             T0=np.random.default_rng(4).uniform(-0.3, 0.3, (12, 16, 20)), **kw)
+        sol.set_kernel_variant(rblbm.KERNEL_ONE_PASS, 5)
+        sol.step(20)
+        ora.step(20)
+        np.testing.assert_array_equal(sol.gather()["g"], ora.state()["g"])
+        _compare_diag(sol, ora)
+    else:
+        sol, ora = _run_pair(2, 40, 21, 1, (1, 200), rayleigh=1e5, mach=0.1, thot=1.0, tcold=0.0,
+                             vel_x=0, temp_x=1, legacy=1)
+        np.testing.assert_array_equal(sol.gather()["f"], ora.state()["f"])
+    with pytest.raises(rblbm.LbmError):
+        rblbm.Solver(rblbm.Config(dims=3, nx=8, ny=8, nz=8, legacy=1,
+                                  precision=rblbm.FP32_SHIFT))

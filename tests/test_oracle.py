@@ -26,7 +26,7 @@ def _pair(cfg, T0):
     o.init_exact(T0)
     m = Model(cfg.dims, cfg.nx, cfg.ny, cfg.nz, cfg.rayleigh, cfg.prandtl, cfg.mach, cfg.thot,
               cfg.tcold, cfg.vel_x, cfg.vel_z, cfg.temp_x, cfg.temp_y, cfg.temp_z,
-              bool(cfg.use_g), bool(cfg.side_heated))
+              bool(cfg.use_g), bool(cfg.side_heated), bool(cfg.legacy))
     m.init(T0)
     return o, m
 
@@ -59,6 +59,22 @@ def test_params_match_physical_definitions(dims, n, ra):
     assert p["velocity_unit"] == pytest.approx(0.1 / np.sqrt(3), rel=1e-12)
     assert p["gbeta"] * p["L"] ** 3 / (p["viscosity"] * p["diffusivity"]) == pytest.approx(ra, rel=1e-12)
     assert p["viscosity"] / p["diffusivity"] == pytest.approx(0.71, rel=1e-14)
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_legacy_thermal_params(dims):
+    # paraA is the rest-weight parameter that makes kappa = cT^2 (1/Qk - 1/2) with the fixed
+    # Qk = 3 - sqrt3 (the model derives the weights from that relation); outside the reference's
+    # range (-6 or -4 < paraA < 1) the oracle refuses the configuration, as the reference stops.
+    cfg = orc.Config(dims=dims, nx=12, ny=10, nz=8, rayleigh=1e5, mach=0.1, legacy=1)
+    p = orc.params(cfg)
+    q = Model(dims, cfg.nx, cfg.ny, cfg.nz, 1e5, legacy=True).p
+    assert p["Qk"] == pytest.approx(q["Qk"], rel=1e-15) and p["Qnu"] == pytest.approx(q["Qnu"])
+    qn = 6 if dims == 3 else 4
+    wT1 = (qn + p["paraA"]) / (7 * qn if dims == 3 else 5 * qn)
+    assert wT1 == pytest.approx(q["wT1"], rel=1e-12)
+    with pytest.raises(ValueError):
+        orc.Oracle(orc.Config(dims=dims, nx=12, ny=10, nz=8, rayleigh=1e5, mach=30.0, legacy=1))
 
 
 def test_params_published_values():
@@ -122,6 +138,8 @@ CASES_3D = {
     "side_heated": dict(side_heated=1, temp_x=0, temp_y=1, temp_z=1),
     "periodic_x_walls_z": dict(vel_x=1, temp_x=2),
     "no_m1g": dict(use_g=0),
+    "legacy_thermal": dict(legacy=1),
+    "legacy_side_heated": dict(legacy=1, side_heated=1, temp_x=0, temp_y=1, temp_z=1),
 }
 
 
@@ -140,6 +158,7 @@ CASES_2D = {
     "rb_walls": dict(vel_x=0, temp_x=1),
     "side_heated": dict(side_heated=1, vel_x=0, temp_x=0, temp_y=1),
     "no_m1g": dict(vel_x=1, temp_x=2, use_g=0),
+    "legacy_thermal": dict(vel_x=0, temp_x=1, legacy=1),
 }
 
 
