@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--forms", default="fp64,fp32",
                     help="storage forms to time, comma separated; the first one is `value`")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=48,
+                    help="timed oracle steps of the CPU baseline (about 10 s on 16 host cores)")
     ap.add_argument("--cpu-nz", type=int, default=64)
     ap.add_argument("--variant", choices=["auto", "two-pass", "one-pass"], default="auto",
                     help="kernel form of the step (auto: the two-pass form)")
