@@ -31,8 +31,12 @@ FORMS = {"fp64": rblbm.FP64, "fp32": rblbm.FP32_SHIFT, "fp16": rblbm.FP16_SHIFT}
 KEYS = ("nu_vol", "nu_global", "nu_hot", "nu_cold", "re_vol")
 
 
-def run(form, n, ra, steps, every, seed, keep_fields, mach):
-    cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=ra, mach=mach, precision=FORMS[form])
+LAYOUTS = {"two-array": rblbm.LAYOUT_TWO_ARRAY, "aa": rblbm.LAYOUT_AA, "auto": rblbm.LAYOUT_AUTO}
+
+
+def run(form, n, ra, steps, every, seed, keep_fields, mach, layout="auto"):
+    cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=ra, mach=mach, precision=FORMS[form],
+                       layout=LAYOUTS[layout])
     sol = rblbm.Solver(cfg)
     sol.initial_seeded(seed, 1e-3)
     series = []
@@ -67,6 +71,11 @@ def main():
     ap.add_argument("--forms", default="fp32")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--out", default="")
+    ap.add_argument("--layout", choices=sorted(LAYOUTS), default="auto",
+                    help="population layout of every run (auto: two arrays when they fit)")
+    ap.add_argument("--ref-form", choices=sorted(FORMS), default="fp64",
+                    help="form of the reference run (fp32 at the same edge isolates the "
+                         "storage precision where FP64 does not fit)")
     ap.add_argument("--ref-n", type=int, default=0,
                     help="run the FP64 reference at this (coarser) edge; fields are compared "
                          "after averaging the fine run over blocks of (n/ref_n)^3 nodes")
@@ -74,19 +83,21 @@ def main():
 
     if a.avg_from < 0:
         a.avg_from = a.steps // 2
+    rk = a.ref_form   # key of the reference run in the result
+    assert rk not in a.forms.split(","), "the reference form is not compared with itself"
     ref_n = a.ref_n or a.n
     ratio = a.n // ref_n
     assert ref_n * ratio == a.n, "--n must be a multiple of --ref-n"
     # the same physical time: the coarse run takes n/ref_n fewer steps per free-fall time
     ref_steps, ref_every = a.steps // ratio, max(1, a.every // ratio)
-    ref, ref_fields, ref_glups = run("fp64", ref_n, a.ra, ref_steps, ref_every, a.seed, True,
-                                     a.mach)
+    ref, ref_fields, ref_glups = run(a.ref_form, ref_n, a.ra, ref_steps, ref_every, a.seed, True,
+                                     a.mach, a.layout)
     for s in ref:
         s["itc"] *= ratio
     result = {"workload": f"3D RB cavity {a.n}^3 Ra={a.ra:g} Pr=0.71 Ma={a.mach}",
-              "reference": f"FP64 at {ref_n}^3, {ref_steps} steps",
+              "reference": f"{a.ref_form.upper()} at {ref_n}^3, {ref_steps} steps",
               "steps": a.steps, "sample_every": a.every, "avg_from": a.avg_from,
-              "fp64": {"glups_incl_diag": ref_glups, "series": ref}}
+              rk: {"glups_incl_diag": ref_glups, "series": ref}}
 
     def coarsen(x):
         if ratio == 1:
@@ -100,10 +111,11 @@ def main():
         return float(np.mean(v))
 
     for k in KEYS:
-        result["fp64"][f"tavg_{k}"] = tavg(ref, k)
-        result["fp64"][f"tavg2_{k}"] = tavg(ref, k, a.avg_from)
+        result[rk][f"tavg_{k}"] = tavg(ref, k)
+        result[rk][f"tavg2_{k}"] = tavg(ref, k, a.avg_from)
     for form in a.forms.split(","):
-        series, fields, glups = run(form, a.n, a.ra, a.steps, a.every, a.seed, True, a.mach)
+        series, fields, glups = run(form, a.n, a.ra, a.steps, a.every, a.seed, True, a.mach,
+                                    a.layout)
         fields = {k: coarsen(v) for k, v in fields.items()}
         if ratio > 1:   # samples at the coarse run's times only
             times = {s["itc"] for s in ref}
@@ -112,8 +124,8 @@ def main():
         for k in KEYS:
             for tag, start in (("tavg", 0), ("tavg2", a.avg_from)):
                 r[f"{tag}_{k}"] = tavg(series, k, start)
-                r[f"{tag}_{k}_relerr"] = abs(r[f"{tag}_{k}"] - result["fp64"][f"{tag}_{k}"]) / \
-                    abs(result["fp64"][f"{tag}_{k}"])
+                r[f"{tag}_{k}_relerr"] = abs(r[f"{tag}_{k}"] - result[rk][f"{tag}_{k}"]) / \
+                    abs(result[rk][f"{tag}_{k}"])
         num = sum(np.sum((fields[k] - ref_fields[k]) ** 2) for k in ("u", "v", "w"))
         den = sum(np.sum(ref_fields[k] ** 2) for k in ("u", "v", "w"))
         r["l2_u_relerr"] = float(np.sqrt(num / den))
@@ -125,7 +137,7 @@ def main():
         with open(a.out, "w") as fh:
             json.dump(result, fh, indent=1)
     summary = {f: {k: v for k, v in result[f].items() if k != "series"}
-               for f in ["fp64"] + a.forms.split(",")}
+               for f in [rk] + a.forms.split(",")}
     print(json.dumps(summary), flush=True)
 
 

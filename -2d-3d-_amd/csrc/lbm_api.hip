@@ -41,6 +41,9 @@ struct lbm_handle {
     int aux_valid;          // aux holds T^n (last step ran the two-pass form)
     int variant;            // LBM_KERNEL_* requested
     bool fused_ok;          // the configuration admits the one-pass form
+    bool aa_ok;             // the configuration admits the in-place layout
+    bool aa;                // in-place layout: F[0], G[0] only, states fpar / gpar
+    int fpar, gpar;         // LAY_ODD / LAY_EVEN of F[0] and G[0] (in-place layout, phase 1)
     FusedGeom fg;
     int64_t itc;
     bool owns_stream;       // false for the domains of a batch (they share the batch's stream)
@@ -143,6 +146,16 @@ static void release(lbm_handle *h)
 }
 
 static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **out);
+
+// Device bytes per node of a domain: populations (one or two arrays per lattice), B_prev (and
+// the one-pass form's second copy), aux, the five FP64 macro planes.
+static double node_bytes(const lbm_config &c, bool aa, bool fused)
+{
+    const int q = c.dims == 3 ? 26 : 14;
+    const double es = c.precision == LBM_FP64 ? 8 : (c.precision == LBM_FP32_SHIFT ? 4 : 2);
+    const double ea = c.precision == LBM_FP64 ? 8 : 4;
+    return (aa ? 1 : 2) * q * es + (fused ? 6 : 3) * ea + (c.dims == 2 ? 2 : 1) * ea + 40.0;
+}
 
 // One-pass launch geometry for z-chunks of `kchunk` planes and thread tile shape `tile`.
 static void set_fused_geom(lbm_handle *h, int kchunk, int tile, int prefetch)
@@ -285,8 +298,24 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     const uint64_t nb = (uint64_t)k.nxb * c.ny * c.nz;
     if (nb >= (1ULL << 31)) { free(h); return LBM_ERR_ARG; }
     k.nblocks = (unsigned)nb;
-    // one-pass form: 3D without periodic thermal walls (see lbm_fused.h)
-    h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC;
+    // one-pass form: 3D without periodic thermal walls (see lbm_fused.h); in-place layout: 3D
+    // with a wall on every face (a same-row periodic copy would be read by two nodes)
+    h->aa_ok = c.dims == 3 && c.vel_x != LBM_VEL_PERIODIC && c.vel_z != LBM_VEL_PERIODIC &&
+               c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC;
+    if (c.layout == LBM_LAYOUT_AA) {
+        if (!h->aa_ok) { free(h); return LBM_ERR_UNSUPPORTED; }
+        h->aa = true;
+    } else if (c.layout == LBM_LAYOUT_AUTO && h->aa_ok) {
+        // two arrays when they fit in free memory (with 2% to spare), else in place
+        size_t fr = 0, tot = 0;
+        const double two = (double)h->N * node_bytes(c, false, true);
+        if (hipSetDevice(c.device) == hipSuccess && hipMemGetInfo(&fr, &tot) == hipSuccess)
+            h->aa = two > 0.98 * (double)fr;
+    } else if (c.layout != LBM_LAYOUT_TWO_ARRAY && c.layout != LBM_LAYOUT_AUTO) {
+        free(h);
+        return LBM_ERR_ARG;
+    }
+    h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC && !h->aa;
     h->variant = LBM_KERNEL_AUTO;
     set_fused_geom(h, c.nz >= 256 ? 64 : (c.nz >= 64 ? 32 : c.nz), 0, 0);
 
@@ -305,9 +334,10 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     h->lp = c.precision != LBM_FP64;
     h->es = c.precision == LBM_FP64 ? 8 : (c.precision == LBM_FP32_SHIFT ? 4 : 2);
     const size_t ea = h->lp ? sizeof(float) : sizeof(double);   // B_prev / aux element
+    const size_t two = h->aa ? 0 : 1;   // second population arrays
     struct { double **p; size_t bytes; } plan[] = {
-        {&h->F[0], N * h->qf * h->es}, {&h->F[1], N * h->qf * h->es},
-        {&h->G[0], N * h->qt * h->es}, {&h->G[1], N * h->qt * h->es},
+        {&h->F[0], N * h->qf * h->es}, {&h->F[1], two * N * h->qf * h->es},
+        {&h->G[0], N * h->qt * h->es}, {&h->G[1], two * N * h->qt * h->es},
         {&h->Bp, N * 3 * ea},          {&h->aux[0], N * ea},
         {&h->Bp_alt, (h->fused_ok ? N * 3 : 1) * ea},
         {&h->aux[1], (c.dims == 2 ? N : 1) * ea},
@@ -315,6 +345,7 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
         {&h->wall, (size_t)2 * (c.nx > c.ny ? c.nx : c.ny) * sizeof(double)},
     };
     for (auto &b : plan) {
+        if (b.bytes == 0) continue;
         if (hipMalloc((void **)b.p, b.bytes) != hipSuccess) { rc = LBM_ERR_NOMEM; break; }
         h->dev_bytes += b.bytes;
     }
@@ -396,8 +427,67 @@ extern "C" int lbm_init_seeded(lbm_handle *h, uint64_t seed, double amp)
     return do_init(h, nullptr, seed, amp, true);
 }
 
+// In-place layout.  A step is the flow collision (pass B) of the step before, then pass A, so
+// that between steps F[0] and G[0] hold f_post^n and g_post^n (both in state fpar == gpar) and
+// the reference state f^{n+1}, g^{n+1} can still be read from them.  The first step has no
+// pass B: the prologue did that collision.
+static int enqueue_steps_aa(lbm_handle *h, int32_t nsteps)
+{
+    if (h->phase == 0) {
+        if (h->lp) {
+            LPArgs a;
+            a.Fn = h->F[0];
+            a.mac = h->mac;
+            a.lay = LAY_ODD;
+            HIPCHK(h, launch_lp(h->cfg.precision, LP_PROLOGUE, h->kp, h->bx, h->stream, a));
+        } else {
+            HIPCHK(h, launch_prologue(h->kp, h->bx, h->stream, nullptr, h->F[0], h->mac, nullptr,
+                                      true, true));
+        }
+        h->fpar = LAY_EVEN;   // the plain state is LAY_ODD; the prologue turned F over
+        h->gpar = LAY_ODD;
+        h->first_thermal = 1;
+        h->phase = 1;
+    }
+    for (int32_t n = 0; n < nsteps; ++n) {
+        if (!h->first_thermal) {
+            if (h->lp) {
+                LPArgs a;
+                a.Fn = h->F[0];
+                a.Gi = h->G[0];
+                a.aux = (float *)h->aux[0];
+                a.lay = h->fpar;
+                HIPCHK(h, launch_lp(h->cfg.precision, LP_FLOW_AA, h->kp, h->bx, h->stream, a));
+            } else {
+                HIPCHK(h, launch_flow3_aa(h->kp, h->bx, h->stream, h->fpar, h->F[0], h->G[0], h->aux[0]));
+            }
+            h->fpar = 3 - h->fpar;
+        }
+        if (h->lp) {
+            LPArgs a;
+            a.Fc = h->F[0];
+            a.Go = h->G[0];
+            a.Bp = (float *)h->Bp;
+            a.aux = (float *)h->aux[0];
+            a.mac = h->mac;
+            a.first = h->first_thermal;
+            a.lay = h->gpar;
+            HIPCHK(h, launch_lp(h->cfg.precision, LP_THERMAL_AA, h->kp, h->bx, h->stream, a));
+        } else {
+            HIPCHK(h, launch_thermal3_aa(h->kp, h->bx, h->stream, h->gpar, h->first_thermal != 0,
+                                         h->F[0], h->G[0], h->Bp, h->aux[0], h->mac));
+        }
+        h->gpar = 3 - h->gpar;
+        h->aux_valid = 1;
+        h->first_thermal = 0;
+        h->itc += 1;
+    }
+    return LBM_OK;
+}
+
 static int enqueue_steps(lbm_handle *h, int32_t nsteps)
 {
+    if (h->aa) return enqueue_steps_aa(h, nsteps);
     const bool d3 = h->cfg.dims == 3;
     if (h->phase == 0) {
         // flow collision of step 1 from the plain state (f^1 in F[0], macros in mac)
@@ -526,7 +616,8 @@ static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefet
     if (tile == LBM_TILE_32X32 && !h->lp)
         return fail(h, LBM_ERR_UNSUPPORTED, "the 32x32 tile needs a reduced-precision form");
     if (variant == LBM_KERNEL_ONE_PASS && !h->fused_ok)
-        return fail(h, LBM_ERR_UNSUPPORTED, "one-pass form needs 3D and non-periodic thermal walls");
+        return fail(h, LBM_ERR_UNSUPPORTED,
+                    "one-pass form needs 3D, non-periodic thermal walls and two population arrays");
     h->variant = variant;
     set_fused_geom(h, kchunk > 0 ? kchunk : h->fg.kchunk, tile, prefetch);
     return LBM_OK;
@@ -548,6 +639,13 @@ extern "C" int lbm_set_fused_tile(lbm_handle *h, int32_t tile, int32_t prefetch)
     return set_form(h, h->variant, 0, tile, prefetch);
 }
 
+extern "C" int lbm_get_layout(lbm_handle *h, int32_t *layout)
+{
+    if (!h || !layout) return LBM_ERR_ARG;
+    *layout = h->aa ? LBM_LAYOUT_AA : LBM_LAYOUT_TWO_ARRAY;
+    return LBM_OK;
+}
+
 extern "C" int lbm_get_kernel_form(lbm_handle *h, int32_t *variant, int32_t *kchunk, int32_t *tile,
                                    int32_t *prefetch)
 {
@@ -560,16 +658,60 @@ extern "C" int lbm_get_kernel_form(lbm_handle *h, int32_t *variant, int32_t *kch
 }
 
 // ---- autotuner ---------------------------------------------------------------------------
+// An AUTO-layout domain that has not stepped may still change its layout.
+static bool layout_open(const lbm_handle *h)
+{
+    return h->cfg.layout == LBM_LAYOUT_AUTO && h->aa_ok && h->phase <= 0;
+}
+
 static void tune_key(const lbm_handle *h, char *key, size_t n)
 {
     const lbm_config &c = h->cfg;
-    snprintf(key, n, "d%d_%dx%dx%d_p%d_w%d%d%d%d%d", c.dims, c.nx, c.ny, c.nz, c.precision,
-             c.vel_x, c.vel_z, c.temp_x, c.temp_y, c.temp_z);
+    snprintf(key, n, "d%d_%dx%dx%d_p%d_w%d%d%d%d%d_l%d", c.dims, c.nx, c.ny, c.nz, c.precision,
+             c.vel_x, c.vel_z, c.temp_x, c.temp_y, c.temp_z,
+             layout_open(h) ? LBM_LAYOUT_AUTO : (h->aa ? LBM_LAYOUT_AA : LBM_LAYOUT_TWO_ARRAY));
 }
 
-// Cache lines: "<key> <variant> <kchunk> <tile> <prefetch> <ms per step>"; the last line for a
-// key wins.
-struct TuneChoice { int variant, kchunk, tile, pf; };
+// Cache lines: "<key> <variant> <kchunk> <tile> <prefetch> <ms per step> <layout>"; the last
+// line for a key wins.
+struct TuneChoice { int variant, kchunk, tile, pf, aa; };
+
+// Switch the layout of a domain in its plain state (F[0], G[0] hold the same data in both).
+static int convert_layout(lbm_handle *h, bool aa)
+{
+    if (aa == h->aa) return LBM_OK;
+    if (h->phase > 0 || !h->aa_ok) return fail(h, LBM_ERR_STATE, "layout change after stepping");
+    const size_t N = h->N, ea = h->lp ? sizeof(float) : sizeof(double);
+    const size_t fb = N * h->qf * h->es, gb = N * h->qt * h->es;
+    if (aa) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipFree(h->F[1]);
+        (void)hipFree(h->G[1]);
+        (void)hipFree(h->Bp_alt);
+        h->F[1] = h->G[1] = h->Bp_alt = nullptr;
+        h->dev_bytes -= fb + gb + N * 3 * ea;
+        h->aa = true;
+        h->fused_ok = false;
+        h->ftab_ok = false;
+        if (h->variant == LBM_KERNEL_ONE_PASS) h->variant = LBM_KERNEL_AUTO;
+        return LBM_OK;
+    }
+    double *f1 = nullptr, *g1 = nullptr, *b1 = nullptr;
+    if (hipMalloc((void **)&f1, fb) != hipSuccess || hipMalloc((void **)&g1, gb) != hipSuccess ||
+        hipMalloc((void **)&b1, N * 3 * ea) != hipSuccess) {
+        (void)hipFree(f1);
+        (void)hipFree(g1);
+        (void)hipFree(b1);
+        return fail(h, LBM_ERR_NOMEM, "second population arrays");
+    }
+    h->F[1] = f1;
+    h->G[1] = g1;
+    h->Bp_alt = b1;
+    h->dev_bytes += fb + gb + N * 3 * ea;
+    h->aa = false;
+    h->fused_ok = true;   // aa_ok implies the one-pass conditions
+    return LBM_OK;
+}
 
 static bool tune_lookup(const char *path, const char *key, TuneChoice *out)
 {
@@ -581,8 +723,11 @@ static bool tune_lookup(const char *path, const char *key, TuneChoice *out)
     while (fgets(line, sizeof(line), fp)) {
         TuneChoice c;
         double ms;
-        if (sscanf(line, "%127s %d %d %d %d %lf", k, &c.variant, &c.kchunk, &c.tile, &c.pf, &ms) == 6 &&
-            strcmp(k, key) == 0) {
+        int lay = LBM_LAYOUT_TWO_ARRAY;
+        const int got = sscanf(line, "%127s %d %d %d %d %lf %d", k, &c.variant, &c.kchunk, &c.tile,
+                               &c.pf, &ms, &lay);
+        if (got >= 6 && strcmp(k, key) == 0) {
+            c.aa = lay == LBM_LAYOUT_AA;
             *out = c;
             found = true;
         }
@@ -599,55 +744,70 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
     tune_key(h, key, sizeof(key));
     if (ms_per_step) *ms_per_step = 0.0;
     TuneChoice win{};
-    if (tune_lookup(cache_path, key, &win))
-        return set_form(h, win.variant, win.kchunk, win.tile, win.pf);
+    const bool open = layout_open(h);
+    if (tune_lookup(cache_path, key, &win)) {
+        int rc = open ? convert_layout(h, win.aa != 0) : LBM_OK;
+        return rc ? rc : set_form(h, win.variant, win.kchunk, win.tile, win.pf);
+    }
 
-    lbm_config sc = h->cfg;
     size_t free_b = 0, total_b = 0;
     HIPCHK(h, hipSetDevice(h->cfg.device));
     HIPCHK(h, hipMemGetInfo(&free_b, &total_b));
-    const double per_plane = (double)h->dev_bytes / (sc.dims == 3 ? sc.nz : 1);
-    const int fit = (int)(0.8 * (double)free_b / per_plane);
-    if (sc.dims == 3 && fit < sc.nz) {
-        if (fit < 16) return fail(h, LBM_ERR_NOMEM, "autotune: no room for a scratch domain");
-        sc.nz = fit;
-    }
-    lbm_handle *t = nullptr;
-    int rc = create_impl(&sc, nullptr, &t);
-    if (rc) return fail(h, rc, "autotune: scratch domain: %s", lbm_error_string(rc));
-    rc = lbm_init_seeded(t, 1, 1e-3);
-    if (!rc) rc = lbm_step(t, 1);   // leave the plain state: the candidates time steady steps
-    std::vector<TuneChoice> cands;
-    cands.push_back({LBM_KERNEL_TWO_PASS, h->fg.kchunk, h->fg.tile, h->fg.pf});
-    if (t->fused_ok)
-        for (int tile = 0; tile < FUSED_TILES; ++tile)
-            for (int pf = 0; pf < 2; ++pf)
-                if (tile != LBM_TILE_32X32 || h->lp)
-                for (int kc : {8, 16, 32, 64, 128})
-                    if (kc <= sc.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc, tile, pf});
+    // time per full-size step of the fastest candidate over the layouts open to the domain
     double best = 1e300;
-    win = cands[0];
-    for (const TuneChoice &cd : cands) {
+    int rc = LBM_OK;
+    bool timed = false;
+    const bool lays[2] = {false, true};
+    for (bool aa : lays) {
         if (rc) break;
-        rc = set_form(t, cd.variant, cd.kchunk, cd.tile, cd.pf);
-        double ms = 0.0;
-        if (!rc) rc = lbm_step(t, 2);                   // warm-up with this form
-        if (!rc) rc = lbm_step_timed(t, steps, &ms);
-        if (!rc && ms / steps < best) {
-            best = ms / steps;
-            win = cd;
+        if (open ? false : aa != h->aa) continue;
+        lbm_config sc = h->cfg;
+        sc.layout = aa ? LBM_LAYOUT_AA : LBM_LAYOUT_TWO_ARRAY;
+        const double per_plane = node_bytes(sc, aa, !aa && h->aa_ok) * sc.nx * (sc.dims == 3 ? sc.ny : 1);
+        const int fit = (int)(0.8 * (double)free_b / per_plane);
+        if (sc.dims == 3 && fit < sc.nz) {
+            if (fit < 16) continue;   // no room for a scratch domain of this layout
+            sc.nz = fit;
         }
+        lbm_handle *t = nullptr;
+        rc = create_impl(&sc, nullptr, &t);
+        if (rc) return fail(h, rc, "autotune: scratch domain: %s", lbm_error_string(rc));
+        rc = lbm_init_seeded(t, 1, 1e-3);
+        if (!rc) rc = lbm_step(t, 1);   // leave the plain state: the candidates time steady steps
+        std::vector<TuneChoice> cands;
+        cands.push_back({LBM_KERNEL_TWO_PASS, h->fg.kchunk, h->fg.tile, h->fg.pf, aa ? 1 : 0});
+        if (t->fused_ok)
+            for (int tile = 0; tile < FUSED_TILES; ++tile)
+                for (int pf = 0; pf < 2; ++pf)
+                    if (tile != LBM_TILE_32X32 || h->lp)
+                    for (int kc : {8, 16, 32, 64, 128})
+                        if (kc <= sc.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc, tile, pf, 0});
+        const double scale = (double)h->cfg.nz / (double)sc.nz;
+        for (const TuneChoice &cd : cands) {
+            if (rc) break;
+            rc = set_form(t, cd.variant, cd.kchunk, cd.tile, cd.pf);
+            double ms = 0.0;
+            if (!rc) rc = lbm_step(t, 2);                   // warm-up with this form
+            if (!rc) rc = lbm_step_timed(t, steps, &ms);
+            if (!rc && ms / steps * scale < best) {
+                best = ms / steps * scale;
+                win = cd;
+                timed = true;
+            }
+        }
+        lbm_destroy(t);
     }
-    lbm_destroy(t);
     if (rc) return fail(h, rc, "autotune: timing failed");
-    rc = set_form(h, win.variant, win.kchunk, win.tile, win.pf);
+    if (!timed) return fail(h, LBM_ERR_NOMEM, "autotune: no room for a scratch domain");
+    if (open) rc = convert_layout(h, win.aa != 0);
+    if (!rc) rc = set_form(h, win.variant, win.kchunk, win.tile, win.pf);
     if (rc) return rc;
-    if (ms_per_step) *ms_per_step = best * (double)h->cfg.nz / (double)sc.nz;
+    if (ms_per_step) *ms_per_step = best;
     if (cache_path) {
         FILE *fp = fopen(cache_path, "a");
         if (fp) {
-            fprintf(fp, "%s %d %d %d %d %.6f\n", key, win.variant, win.kchunk, win.tile, win.pf,
-                    best * (double)h->cfg.nz / (double)sc.nz);
+            fprintf(fp, "%s %d %d %d %d %.6f %d\n", key, win.variant, win.kchunk, win.tile, win.pf,
+                    best, win.aa ? LBM_LAYOUT_AA : LBM_LAYOUT_TWO_ARRAY);
             fclose(fp);
         }
     }
@@ -666,7 +826,24 @@ extern "C" int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps
 static int refresh_macros(lbm_handle *h)
 {
     if (h->phase < 0) return fail(h, LBM_ERR_STATE, "no state: call lbm_init_* first");
-    if (h->phase == 1 && h->lp) {
+    if (h->phase == 1 && h->first_thermal) {
+        // after the prologue alone: the macro planes still hold the plain state; in place, the
+        // prologue has overwritten the plain f
+        if (h->aa) return fail(h, LBM_ERR_STATE, "in-place layout between the prologue and the first step");
+        return LBM_OK;
+    }
+    if (h->phase == 1 && h->aa && h->lp) {
+        LPArgs a;
+        a.Fc = h->F[0];
+        a.Gi = h->G[0];
+        a.aux = (float *)h->aux[0];
+        a.mac = h->mac;
+        a.lay = h->fpar;
+        HIPCHK(h, launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a));
+    } else if (h->phase == 1 && h->aa) {
+        HIPCHK(h, launch_state(h->kp, h->bx, h->stream, true, h->F[0], h->G[0], h->aux[0], nullptr,
+                               h->mac, nullptr, nullptr, 0, 0, h->fpar));
+    } else if (h->phase == 1 && h->lp) {
         LPArgs a;
         a.Fc = h->F[h->fcur ^ 1];
         a.Gi = h->G[h->gcur];
@@ -1311,6 +1488,11 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
     const bool d3 = h->cfg.dims == 3;
     const int nk = chunk_planes(h);
     const size_t plane = (size_t)h->cfg.nx * h->cfg.ny;
+    // after the prologue alone the plain arrays still hold the state (two arrays) or the
+    // prologue has overwritten the plain f (in place)
+    const bool plain = h->phase == 0 || (h->phase == 1 && h->first_thermal);
+    if (plain && h->phase == 1 && h->aa && (f || g))
+        return fail(h, LBM_ERR_STATE, "in-place layout between the prologue and the first step");
     double *stage = nullptr;
     if (f || g) {
         if (hipMalloc((void **)&stage, (size_t)nk * plane * (h->qf + h->qt) * sizeof(double)) != hipSuccess)
@@ -1326,12 +1508,20 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
             LPArgs a;
             a.fo = fo;
             a.go = go;
-            if (h->phase == 0) {
+            if (plain) {
                 a.Fc = h->F[0];
                 a.Gi = h->G[0];
                 a.c0 = c0;
                 a.n = cnt;
                 e = launch_lp(h->cfg.precision, LP_TO_ABS, h->kp, h->bx, h->stream, a);
+            } else if (h->aa) {
+                a.Fc = h->F[0];
+                a.Gi = h->G[0];
+                a.aux = (float *)h->aux[0];
+                a.lay = h->fpar;
+                a.k0 = k0;
+                a.nk = n;
+                e = launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a);
             } else {
                 a.Fc = h->F[h->fcur ^ 1];
                 a.Gi = h->G[h->gcur];
@@ -1341,9 +1531,12 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
                 a.nk = n;
                 e = launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a);
             }
-        } else if (h->phase == 0) {
+        } else if (plain) {
             e = launch_transpose(false, h->stream, h->F[0], fo, h->qf, h->N, c0, cnt);
             if (e == hipSuccess) e = launch_transpose(false, h->stream, h->G[0], go, h->qt, h->N, c0, cnt);
+        } else if (h->aa) {
+            e = launch_state(h->kp, h->bx, h->stream, true, h->F[0], h->G[0], h->aux[0], nullptr,
+                             nullptr, fo, go, k0, n, h->fpar);
         } else {
             const double *aux = d3 ? (h->aux_valid ? h->aux[0] : nullptr) : h->aux[h->fy ^ 1];
             e = launch_state(h->kp, h->bx, h->stream, d3, h->F[h->fcur ^ 1], h->G[h->gcur], aux,
@@ -1444,7 +1637,9 @@ struct lbm_batch {
     hipStream_t stream;
     hipEvent_t ev0, ev1;
     int mode;
-    // one-launch tables, one per buffer parity, built at the first batched step
+    // one-launch tables, one per buffer parity, built at the first batched step; two-pass form
+    // (two launches a step, 256-node blocks) unless every domain runs the one-pass form
+    bool twopass;
     void *table[2];
     uint16_t *owner;
     unsigned nblocks;
@@ -1484,8 +1679,8 @@ extern "C" int lbm_batch_create(const lbm_config *cfgs, int32_t n, lbm_batch **o
     for (int i = 0; i < n; ++i)
         if (cfgs[i].dims != 3 || cfgs[i].precision != cfgs[0].precision ||
             cfgs[i].device != cfgs[0].device || cfgs[i].temp_x == LBM_T_PERIODIC ||
-            cfgs[i].temp_z == LBM_T_PERIODIC)
-            return LBM_ERR_UNSUPPORTED;   // one-pass batched form: 3D walls, one precision
+            cfgs[i].temp_z == LBM_T_PERIODIC || cfgs[i].layout == LBM_LAYOUT_AA)
+            return LBM_ERR_UNSUPPORTED;   // table forms: 3D walls, one precision, two arrays
     lbm_batch *b = (lbm_batch *)calloc(1, sizeof(lbm_batch));
     if (!b) return LBM_ERR_NOMEM;
     b->n = n;
@@ -1500,7 +1695,9 @@ extern "C" int lbm_batch_create(const lbm_config *cfgs, int32_t n, lbm_batch **o
     b->dom = (lbm_handle **)calloc(n, sizeof(lbm_handle *));
     if (!b->dom) { batch_release(b); return LBM_ERR_NOMEM; }
     for (int i = 0; i < n; ++i) {
-        int rc = create_impl(&cfgs[i], b->stream, &b->dom[i]);
+        lbm_config ci = cfgs[i];
+        ci.layout = LBM_LAYOUT_TWO_ARRAY;
+        int rc = create_impl(&ci, b->stream, &b->dom[i]);
         if (rc) { batch_release(b); return rc; }
         b->dom[i]->in_batch = true;
         b->dom[i]->instance = (uint64_t)i;
@@ -1583,6 +1780,19 @@ extern "C" int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t 
     return LBM_OK;
 }
 
+extern "C" int lbm_batch_set_kernel_variant(lbm_batch *b, int32_t variant)
+{
+    if (!b || variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS) return LBM_ERR_ARG;
+    HIPCHK(b->dom[0], hipStreamSynchronize(b->stream));
+    for (int i = 0; i < b->n; ++i) {
+        lbm_handle *h = b->dom[i];
+        int rc = set_form(h, variant, 0, h->fg.tile, h->fg.pf);
+        if (rc) return rc;
+    }
+    batch_invalidate(b);
+    return LBM_OK;
+}
+
 extern "C" int lbm_batch_set_mode(lbm_batch *b, int32_t mode)
 {
     if (!b || mode < LBM_BATCH_PER_DOMAIN || mode > LBM_BATCH_GRAPH) return LBM_ERR_ARG;
@@ -1590,14 +1800,26 @@ extern "C" int lbm_batch_set_mode(lbm_batch *b, int32_t mode)
     return LBM_OK;
 }
 
+// Blocks of one domain in the batch's form.
+static unsigned batch_blocks(const lbm_batch *b, const lbm_handle *h)
+{
+    return b->twopass ? (unsigned)((h->N + 255) / 256) : h->fg.nblocks;
+}
+
 // Tables for both buffer parities from the domains' current pointers: parity p is the current
-// state, parity p^1 the state after one one-pass step (F, G, and B_prev when use_g, swapped).
+// state, parity p^1 the state after one step (F and G swapped; in the one-pass form also
+// B_prev when use_g, which the two-pass form updates in place).
 static int batch_tables(lbm_batch *b)
 {
     lbm_handle *h0 = b->dom[0];
-    const size_t isz = fused_entry_bytes(h0);
+    b->twopass = false;
+    for (int i = 0; i < b->n; ++i)
+        if (b->dom[i]->variant != LBM_KERNEL_ONE_PASS) b->twopass = true;
+    const size_t isz = !b->twopass ? fused_entry_bytes(h0)
+                       : h0->lp    ? twopass_entry_bytes_lp(b->precision)
+                                   : twopass_entry_bytes_f64();
     unsigned total = 0;
-    for (int i = 0; i < b->n; ++i) total += b->dom[i]->fg.nblocks;
+    for (int i = 0; i < b->n; ++i) total += batch_blocks(b, b->dom[i]);
     if (!b->owner) {
         if (hipMalloc((void **)&b->owner, total * sizeof(uint16_t)) != hipSuccess ||
             hipMalloc(&b->table[0], isz * b->n) != hipSuccess ||
@@ -1606,7 +1828,7 @@ static int batch_tables(lbm_batch *b)
         std::vector<uint16_t> own(total);
         unsigned at = 0;
         for (int i = 0; i < b->n; ++i)
-            for (unsigned k = 0; k < b->dom[i]->fg.nblocks; ++k) own[at++] = (uint16_t)i;
+            for (unsigned k = 0; k < batch_blocks(b, b->dom[i]); ++k) own[at++] = (uint16_t)i;
         HIPCHK(h0, hipMemcpy(b->owner, own.data(), total * sizeof(uint16_t), hipMemcpyHostToDevice));
         b->nblocks = total;
     }
@@ -1615,9 +1837,17 @@ static int batch_tables(lbm_batch *b)
         unsigned first = 0;
         for (int i = 0; i < b->n; ++i) {
             lbm_handle *h = b->dom[i];
-            fused_fill(h, host.data() + isz * i, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, par),
-                       first);
-            first += h->fg.nblocks;
+            void *dst = host.data() + isz * i;
+            if (!b->twopass)
+                fused_fill(h, dst, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, par), first);
+            else if (h->lp)
+                twopass_entry_fill_lp(b->precision, dst, h->kp,
+                                      fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp, par),
+                                      (float *)h->aux[0], first);
+            else
+                twopass_entry_fill_f64(dst, h->kp, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp, par),
+                                       h->aux[0], first);
+            first += batch_blocks(b, h);
         }
         HIPCHK(h0, hipMemcpy(b->table[par], host.data(), host.size(), hipMemcpyHostToDevice));
     }
@@ -1628,22 +1858,26 @@ static int batch_tables(lbm_batch *b)
 
 static hipError_t batch_launch(lbm_batch *b, int par)
 {
+    if (b->twopass)
+        return b->dom[0]->lp
+                   ? launch_twopass_batch_lp(b->precision, b->table[par], b->owner, b->nblocks, b->stream)
+                   : launch_twopass_batch_f64(b->table[par], b->owner, b->nblocks, b->stream);
     const FusedGeom &fg0 = b->dom[0]->fg;
     if (b->dom[0]->lp)
         return launch_fused_batch_lp(b->precision, fg0, b->table[par], b->owner, b->nblocks, b->stream);
     return launch_fused_batch_f64(fg0, b->table[par], b->owner, b->nblocks, b->stream);
 }
 
-// Host bookkeeping of one one-pass step on every domain (buffers swap like enqueue_steps).
+// Host bookkeeping of one batched step on every domain (buffers swap like enqueue_steps).
 static void batch_advance(lbm_batch *b)
 {
     for (int i = 0; i < b->n; ++i) {
         lbm_handle *h = b->dom[i];
         h->fcur ^= 1;
         h->gcur ^= 1;
-        h->aux_valid = 0;
+        h->aux_valid = b->twopass ? 1 : 0;
         h->itc += 1;
-        if (h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
+        if (!b->twopass && h->cfg.use_g) std::swap(h->Bp, h->Bp_alt);
     }
     b->table_parity ^= 1;
 }

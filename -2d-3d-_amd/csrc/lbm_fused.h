@@ -198,8 +198,8 @@ k3_fused(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restric
 {
     const unsigned b = xcd_remap(blockIdx.x, nblocks);
     const FusedInstance<Form> &d = inst[owner ? owner[b] : 0];
-    fused_tile<Form, TX, TY, PF>(d.kp, d.fp, d.fg, b - d.first_block, d.Fcur, d.Fnext, d.Gin,
-                                 d.Gout, d.Bp, d.Bpout);
+    fused_tile<Form, TX, TY, PF>(d.kp, d.fp, d.fg, b - d.first_block, gptr(d.Fcur), gptr(d.Fnext),
+                                 gptr(d.Gin), gptr(d.Gout), gptr(d.Bp), gptr(d.Bpout));
 }
 
 // Run-time tile shapes: index -> (TX, TY).

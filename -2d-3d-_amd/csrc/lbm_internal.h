@@ -18,17 +18,29 @@ struct InitParams {
     double noise_scale;   // amp * (thot - tcold); 0 disables the noise
 };
 
+// aa: 3D in-place layout (Fin == Fout; the plain state LAY_ODD becomes LAY_EVEN).
 hipError_t launch_prologue(const KParams &kp, int bx, hipStream_t s, const double *Fin,
-                           double *Fout, const double *mac, double *fy_out, bool three_d);
+                           double *Fout, const double *mac, double *fy_out, bool three_d,
+                           bool aa = false);
+// In-place (A-A) 3D passes, each a separate launch: pass B (flow collision; F and G in state
+// lay, F updated) and pass A (thermal collision; G in state lay and updated, F read in the
+// other state).  first: pass A of the first step (plain g, T from mac; lay = LAY_ODD).
+hipError_t launch_flow3_aa(const KParams &kp, int bx, hipStream_t s, int lay, double *F,
+                           const double *G, const double *aux);
+hipError_t launch_thermal3_aa(const KParams &kp, int bx, hipStream_t s, int lay, bool first,
+                              const double *F, double *G, double *Bp, double *aux,
+                              const double *mac);
 hipError_t launch_step3(const KParams &kp, int bx, hipStream_t s, bool first, const double *Fcur,
                         double *Fnext, const double *Gin, double *Gout, double *Bp, double *aux,
                         const double *mac);
 hipError_t launch_step2(const KParams &kp, int bx, hipStream_t s, bool first, const double *Fcur,
                         double *Fnext, const double *Gin, double *Gout, double *Bp,
                         const double *fy_cur, double *fy_next, const double *mac);
+// Reference state from the pipelined buffers; lay: layout of Fprev / Gpost (3D: LAY_PULL, or
+// an in-place state with F and G in the same state).
 hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, const double *Fprev,
                         const double *Gpost, const double *aux_or_fy, const double *Gprev,
-                        double *mac, double *fo, double *go, int k0, int nk);
+                        double *mac, double *fo, double *go, int k0, int nk, int lay = 0);
 
 // One-pass form: tile of (TX - 2) x (TY - 2) nodes plus a one-node ring per block; the thread
 // tile TX x TY is one of four shapes (FusedGeom::tile): 32x16, 16x16, 32x8, and 32x32 for the
@@ -69,6 +81,8 @@ hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *m
 enum {
     LP_PROLOGUE,      // flow collision of step 1: Fc (plain) -> Fn, macros from mac
     LP_STEP,          // passes A and B: Fc, Gi -> Go, Fn; Bp, aux updated
+    LP_FLOW_AA,       // in place, state lay: flow collision, Fn updated, Gi read
+    LP_THERMAL_AA,    // in place, state lay: thermal collision, Go updated, Fc read (first: plain g)
     LP_STATE,         // reference state from Fc (= f_post^n), Gi (= g_post^n), aux -> mac, fo/go
     LP_INIT,          // plain state from mac: Fn = 0, Go = wT (T - T0), Bp = u T
     LP_FROM_ABS,      // plain state from an absolute AoS chunk fo/go [c0, c0 + n) -> Fn, Go
@@ -88,6 +102,7 @@ struct LPArgs {
     uint64_t c0 = 0, n = 0;
     int *flag = nullptr;
     const void *Gprev = nullptr;   // LP_STATE: g_post^{n-1} when aux is not valid
+    int lay = 0;                   // population layout (LAY_PULL, or an in-place state)
 };
 hipError_t launch_lp(int precision, int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a);
 
@@ -107,6 +122,18 @@ void fused_instance_fill_f64(void *dst, const KParams &kp, const FusedGeom &fg, 
                              unsigned first_block);
 hipError_t launch_fused_batch_f64(const FusedGeom &fg0, const void *inst, const uint16_t *owner,
                                   unsigned nblocks, hipStream_t s);
+// Batched two-pass step (3D): per-domain entries of 256-node blocks, pass A then pass B.
+// p.Bp is the history updated in place; aux holds T^n between the passes.
+size_t twopass_entry_bytes_f64();
+void twopass_entry_fill_f64(void *dst, const KParams &kp, const FusedPtrs &p, double *aux,
+                            unsigned first_block);
+hipError_t launch_twopass_batch_f64(const void *tab, const uint16_t *owner, unsigned nblocks,
+                                    hipStream_t s);
+size_t twopass_entry_bytes_lp(int precision);
+void twopass_entry_fill_lp(int precision, void *dst, const KParams &kp, const FusedPtrs &p,
+                           float *aux, unsigned first_block);
+hipError_t launch_twopass_batch_lp(int precision, const void *tab, const uint16_t *owner,
+                                   unsigned nblocks, hipStream_t s);
 size_t fused_instance_bytes_lp(int precision);
 void fused_instance_fill_lp(int precision, void *dst, const KParams &kp, const FusedGeom &fg,
                             const FusedPtrs &p, unsigned first_block);

@@ -137,6 +137,101 @@ __device__ __forceinline__ void pull_heat(const double *__restrict__ P, uint64_t
     heat_in_all<L>(P, N, i, j, k, c, kp, g, std::make_integer_sequence<int, L::Q>{});
 }
 
+// In-place (A-A) layout (see LAY_ODD / LAY_EVEN): the incoming populations, their wall values
+// and the store of the post-collision populations.  Walls only, as in pull_flow / heat_in
+// without their periodic branches.
+__device__ __forceinline__ bool inside3(int i, int j, int k, const KParams &kp)
+{
+    return (unsigned)i < (unsigned)kp.nx && (unsigned)j < (unsigned)kp.ny && (unsigned)k < (unsigned)kp.nz;
+}
+
+template <class L, int LAY>
+__device__ __forceinline__ void in_flow_aa(const double *P, uint64_t N, int i, int j, int k,
+                                           uint64_t c, bool border, const KParams &kp, double *f)
+{
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+#pragma unroll
+    for (int a = 0; a < L::Q; ++a) {
+        if (LAY == LAY_ODD) {
+            f[a] = P[a * N + c];
+        } else {
+            const int64_t off = L::ex[a] + L::ey[a] * sy + L::ez[a] * sz;
+            const bool in = !border || inside3(i - L::ex[a], j - L::ey[a], k - L::ez[a], kp);
+            f[a] = P[in ? L::opp[a] * N + c - off : a * N + c];
+        }
+    }
+}
+
+template <class L, int LAY, int A>
+__device__ __forceinline__ double heat_in_aa(const double *P, uint64_t N, int i, int j, int k,
+                                             uint64_t c, bool border, const KParams &kp)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+    int kind = -1;
+    if (border) {
+        if (ex != 0) {
+            if ((unsigned)(i - ex) >= (unsigned)kp.nx) kind = kp.temp_x;
+        } else if (ey != 0) {
+            if ((unsigned)(j - ey) >= (unsigned)kp.ny) kind = kp.temp_y;
+        } else if (ez != 0) {
+            if ((unsigned)(k - ez) >= (unsigned)kp.nz) kind = kp.temp_z;
+        }
+    }
+    if (kind < 0)
+        return LAY == LAY_ODD ? P[A * N + c]
+                              : P[L::opp[A] * N + c - (int64_t)(ex + ey * sy + ez * sz)];
+    const double back = P[A * N + c];   // g_post of the opposite direction, in both states
+    if (kind == T_ADIABATIC) return back;
+    constexpr bool low_wall = (ex + ey + ez) > 0;
+    const double tw = low_wall ? kp.thot : kp.tcold;
+    return -back + kp.abb * tw;
+}
+
+template <class L, int LAY, int... A>
+__device__ __forceinline__ void in_heat_aa(const double *P, uint64_t N, int i, int j, int k,
+                                           uint64_t c, bool border, const KParams &kp, double *g,
+                                           std::integer_sequence<int, A...>)
+{
+    ((g[A] = heat_in_aa<L, LAY, A>(P, N, i, j, k, c, border, kp)), ...);
+}
+
+template <class L, int LAY>
+__device__ __forceinline__ void out_aa(double *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                       bool border, const KParams &kp, const double *v)
+{
+    const int sy = kp.nx, sz = kp.nx * kp.ny;
+#pragma unroll
+    for (int a = 0; a < L::Q; ++a) {
+        if (LAY == LAY_ODD) {
+            P[L::opp[a] * N + c] = v[a];
+        } else {
+            const int64_t off = L::ex[a] + L::ey[a] * sy + L::ez[a] * sz;
+            const bool in = !border || inside3(i + L::ex[a], j + L::ey[a], k + L::ez[a], kp);
+            P[in ? a * N + c + off : L::opp[a] * N + c] = v[a];
+        }
+    }
+}
+
+// Incoming populations of a node in layout LAY (pull from the other array, or A-A in place).
+template <int LAY>
+__device__ __forceinline__ void in_flow(const double *P, uint64_t N, int i, int j, int k,
+                                        uint64_t c, bool border, const KParams &kp, double *f)
+{
+    if (LAY == LAY_PULL) pull_flow<D3Q19>(P, N, i, j, k, c, border, kp, f);
+    else in_flow_aa<D3Q19, LAY>(P, N, i, j, k, c, border, kp, f);
+}
+template <int LAY>
+__device__ __forceinline__ void in_heat(const double *P, uint64_t N, int i, int j, int k,
+                                        uint64_t c, bool border, const KParams &kp, double *g)
+{
+    if (LAY == LAY_PULL) pull_heat<D3Q7>(P, N, i, j, k, c, border, kp, g);
+    else in_heat_aa<D3Q7, LAY>(P, N, i, j, k, c, border, kp, g, std::make_integer_sequence<int, 7>{});
+}
+// the state an in-place pass leaves behind, and the layout the flow lattice is read with while
+// the thermal lattice is in state LAY (pass A runs between the two flow states)
+constexpr int lay_flip(int lay) { return lay == LAY_PULL ? LAY_PULL : 3 - lay; }
+
 __device__ __forceinline__ bool on_border(int i, int j, int k, const KParams &kp)
 {
     return i == 0 || j == 0 || k == 0 || i == kp.nx - 1 || j == kp.ny - 1 || k == kp.nz - 1;
@@ -178,6 +273,8 @@ __device__ __forceinline__ double jz19(const double *f)
 // 3D step kernels
 // ---------------------------------------------------------------------------------------
 // Flow collision of step 1 from the plain initial state (f^1 unpulled in Fin, macros in mac).
+// In place (AA): Fin is Fout, the plain state is LAY_ODD and the result LAY_EVEN.
+template <bool AA>
 __global__ void __launch_bounds__(256) k3_prologue(KParams kp, const double *__restrict__ Fin,
                                                    double *__restrict__ Fout,
                                                    const double *__restrict__ mac)
@@ -186,16 +283,62 @@ __global__ void __launch_bounds__(256) k3_prologue(KParams kp, const double *__r
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
     const uint64_t N = kp.N;
+    const double *src = AA ? Fout : Fin;
     double f[19];
 #pragma unroll
-    for (int a = 0; a < 19; ++a) f[a] = Fin[a * N + c];
+    for (int a = 0; a < 19; ++a) f[a] = src[a * N + c];
     ref_collide_d3q19(f, f, mac[c], mac[N + c], mac[2 * N + c], mac[3 * N + c], mac[4 * N + c], kp);
 #pragma unroll
-    for (int a = 0; a < 19; ++a) Fout[a * N + c] = f[a];
+    for (int a = 0; a < 19; ++a) Fout[(AA ? D3Q19::opp[a] : a) * N + c] = f[a];
 }
 
-// Pass A of step n.  FIRST: g^n and T^n come unpulled from the initial state.
-template <bool FIRST>
+// Pass A of step n.  FIRST: g^n and T^n come unpulled from the initial state.  LAY: layout of
+// the thermal lattice (LAY_PULL: Gin -> Gout; in place: Gout holds state LAY and is updated).
+template <bool FIRST, int LAY>
+__device__ __forceinline__ void thermal3_node(const KParams &kp, int i, int j, int k, uint64_t c,
+                                              const double *__restrict__ Fcur,
+                                              const double *__restrict__ Gin,
+                                              double *__restrict__ Gout, double *__restrict__ Bp,
+                                              double *__restrict__ aux,
+                                              const double *__restrict__ mac)
+{
+    const uint64_t N = kp.N;
+    const bool border = on_border(i, j, k, kp);
+    const double *G = LAY == LAY_PULL ? Gin : Gout;
+    double g[7], Tn;
+    if (FIRST) {   // the plain state; in place it is LAY_ODD, read the same way
+#pragma unroll
+        for (int a = 0; a < 7; ++a) g[a] = G[a * N + c];
+        Tn = mac[4 * N + c];
+    } else {
+        in_heat<LAY>(G, N, i, j, k, c, border, kp, g);
+        Tn = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
+    }
+    double f[19];
+    in_flow<lay_flip(LAY)>(Fcur, N, i, j, k, c, border, kp, f);
+    const double rho = sum19(f);
+    const double Fy = rho * kp.gbeta * (Tn - kp.tref);
+    const double u = jx19(f) / rho;
+    const double v = (jy19(f) + 0.5 * Fy) / rho;
+    const double w = jz19(f) / rho;
+    double bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
+    double p[7];
+    ref_collide_d3q7(g, p, Tn, u, v, w, bp, kp);
+    if (LAY == LAY_PULL) {
+#pragma unroll
+        for (int a = 0; a < 7; ++a) Gout[a * N + c] = p[a];
+    } else {
+        out_aa<D3Q7, LAY>(Gout, N, i, j, k, c, border, kp, p);
+    }
+    if (kp.use_g) {
+        Bp[c] = bp[0];
+        Bp[N + c] = bp[1];
+        Bp[2 * N + c] = bp[2];
+    }
+    aux[c] = Tn;
+}
+
+template <bool FIRST, int LAY>
 __global__ void __launch_bounds__(256) k3_thermal(KParams kp, const double *__restrict__ Fcur,
                                                   const double *__restrict__ Gin,
                                                   double *__restrict__ Gout,
@@ -206,38 +349,40 @@ __global__ void __launch_bounds__(256) k3_thermal(KParams kp, const double *__re
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
+    thermal3_node<FIRST, LAY>(kp, i, j, k, c, Fcur, Gin, Gout, Bp, aux, mac);
+}
+
+// Pass B of step n: flow collision of step n+1.  LAY: layout of both lattices (LAY_PULL:
+// Fcur -> Fnext; in place: Fnext holds state LAY and is updated, Gpost is read in state LAY).
+template <int LAY>
+__device__ __forceinline__ void flow3_node(const KParams &kp, int i, int j, int k, uint64_t c,
+                                           const double *__restrict__ Fcur,
+                                           double *__restrict__ Fnext,
+                                           const double *__restrict__ Gpost,
+                                           const double *__restrict__ aux)
+{
     const uint64_t N = kp.N;
     const bool border = on_border(i, j, k, kp);
-    double g[7], Tn;
-    if (FIRST) {
-#pragma unroll
-        for (int a = 0; a < 7; ++a) g[a] = Gin[a * N + c];
-        Tn = mac[4 * N + c];
-    } else {
-        pull_heat<D3Q7>(Gin, N, i, j, k, c, border, kp, g);
-        Tn = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
-    }
+    double g[7];
+    in_heat<LAY>(Gpost, N, i, j, k, c, border, kp, g);
+    const double T1 = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
     double f[19];
-    pull_flow<D3Q19>(Fcur, N, i, j, k, c, border, kp, f);
+    in_flow<LAY>(LAY == LAY_PULL ? Fcur : Fnext, N, i, j, k, c, border, kp, f);
     const double rho = sum19(f);
-    const double Fy = rho * kp.gbeta * (Tn - kp.tref);
+    const double Fy = rho * kp.gbeta * (aux[c] - kp.tref);
     const double u = jx19(f) / rho;
     const double v = (jy19(f) + 0.5 * Fy) / rho;
     const double w = jz19(f) / rho;
-    double bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
-    double p[7];
-    ref_collide_d3q7(g, p, Tn, u, v, w, bp, kp);
+    ref_collide_d3q19(f, f, rho, u, v, w, T1, kp);
+    if (LAY == LAY_PULL) {
 #pragma unroll
-    for (int a = 0; a < 7; ++a) Gout[a * N + c] = p[a];
-    if (kp.use_g) {
-        Bp[c] = bp[0];
-        Bp[N + c] = bp[1];
-        Bp[2 * N + c] = bp[2];
+        for (int a = 0; a < 19; ++a) Fnext[a * N + c] = f[a];
+    } else {
+        out_aa<D3Q19, LAY>(Fnext, N, i, j, k, c, border, kp, f);
     }
-    aux[c] = Tn;
 }
 
-// Pass B of step n: flow collision of step n+1.
+template <int LAY>
 __global__ void __launch_bounds__(256) k3_flow(KParams kp, const double *__restrict__ Fcur,
                                                double *__restrict__ Fnext,
                                                const double *__restrict__ Gpost,
@@ -246,26 +391,88 @@ __global__ void __launch_bounds__(256) k3_flow(KParams kp, const double *__restr
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
-    const bool border = on_border(i, j, k, kp);
-    double g[7];
-    pull_heat<D3Q7>(Gpost, N, i, j, k, c, border, kp, g);
-    const double T1 = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
-    double f[19];
-    pull_flow<D3Q19>(Fcur, N, i, j, k, c, border, kp, f);
-    const double rho = sum19(f);
-    const double Fy = rho * kp.gbeta * (aux[c] - kp.tref);
-    const double u = jx19(f) / rho;
-    const double v = (jy19(f) + 0.5 * Fy) / rho;
-    const double w = jz19(f) / rho;
-    ref_collide_d3q19(f, f, rho, u, v, w, T1, kp);
-#pragma unroll
-    for (int a = 0; a < 19; ++a) Fnext[a * N + c] = f[a];
+    flow3_node<LAY>(kp, i, j, k, c, Fcur, Fnext, Gpost, aux);
+}
+
+// Batched two-pass step: many domains per launch.  Block b (after the XCD remap) belongs to
+// domain owner[b] and covers its nodes [(b - first_block) * 256, + 256) in linear order.
+struct TwoPassEntryF64 {
+    KParams kp;
+    const double *Fcur;
+    double *Fnext;
+    const double *Gin;
+    double *Gout;
+    double *Bp, *aux;
+    unsigned first_block;
+};
+
+__device__ __forceinline__ bool batch_node(const KParams &kp, unsigned lb, int &i, int &j, int &k,
+                                           uint64_t &c)
+{
+    c = (uint64_t)lb * 256u + threadIdx.x;
+    if (c >= kp.N) return false;
+    const uint64_t nx = (uint64_t)kp.nx, row = c / nx;
+    i = (int)(c - row * nx);
+    j = (int)(row % (uint64_t)kp.ny);
+    k = (int)(row / (uint64_t)kp.ny);
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k3_thermal_batch(const TwoPassEntryF64 *__restrict__ tab,
+                                                        const uint16_t *__restrict__ owner,
+                                                        unsigned nblocks)
+{
+    const unsigned b = xcd_remap(blockIdx.x, nblocks);
+    const TwoPassEntryF64 &d = tab[owner[b]];
+    int i, j, k;
+    uint64_t c;
+    if (!batch_node(d.kp, b - d.first_block, i, j, k, c)) return;
+    thermal3_node<false, LAY_PULL>(d.kp, i, j, k, c, gptr(d.Fcur), gptr(d.Gin), gptr(d.Gout),
+                                   gptr(d.Bp), gptr(d.aux), nullptr);
+}
+
+__global__ void __launch_bounds__(256) k3_flow_batch(const TwoPassEntryF64 *__restrict__ tab,
+                                                     const uint16_t *__restrict__ owner,
+                                                     unsigned nblocks)
+{
+    const unsigned b = xcd_remap(blockIdx.x, nblocks);
+    const TwoPassEntryF64 &d = tab[owner[b]];
+    int i, j, k;
+    uint64_t c;
+    if (!batch_node(d.kp, b - d.first_block, i, j, k, c)) return;
+    flow3_node<LAY_PULL>(d.kp, i, j, k, c, gptr(d.Fcur), gptr(d.Fnext), gptr(d.Gout), gptr(d.aux));
+}
+
+size_t twopass_entry_bytes_f64() { return sizeof(TwoPassEntryF64); }
+
+void twopass_entry_fill_f64(void *dst, const KParams &kp, const FusedPtrs &p, double *aux,
+                            unsigned first_block)
+{
+    TwoPassEntryF64 d;
+    d.kp = kp;
+    d.Fcur = (const double *)p.Fcur;
+    d.Fnext = (double *)p.Fnext;
+    d.Gin = (const double *)p.Gin;
+    d.Gout = (double *)p.Gout;
+    d.Bp = (double *)p.Bp;
+    d.aux = aux;
+    d.first_block = first_block;
+    memcpy(dst, &d, sizeof(d));
+}
+
+hipError_t launch_twopass_batch_f64(const void *tab, const uint16_t *owner, unsigned nblocks,
+                                    hipStream_t s)
+{
+    const TwoPassEntryF64 *d = (const TwoPassEntryF64 *)tab;
+    k3_thermal_batch<<<nblocks, 256, 0, s>>>(d, owner, nblocks);
+    k3_flow_batch<<<nblocks, 256, 0, s>>>(d, owner, nblocks);
+    return hipGetLastError();
 }
 
 // Reference state after the last step (f^{n+1}, g^{n+1}, macros) from the pipelined buffers.
 // Writes macros to mac (5 planes) and, if fo/go are non-null, AoS populations for the nodes
 // of planes [k0, k0 + nk) into fo/go (node index relative to plane k0).
+template <int LAY>
 __global__ void __launch_bounds__(256) k3_state(KParams kp, const double *__restrict__ Fprev,
                                                 const double *__restrict__ Gpost,
                                                 const double *__restrict__ aux,
@@ -279,8 +486,8 @@ __global__ void __launch_bounds__(256) k3_state(KParams kp, const double *__rest
     const uint64_t N = kp.N;
     const bool border = on_border(i, j, k, kp);
     double g[7], f[19];
-    pull_heat<D3Q7>(Gpost, N, i, j, k, c, border, kp, g);
-    pull_flow<D3Q19>(Fprev, N, i, j, k, c, border, kp, f);
+    in_heat<LAY>(Gpost, N, i, j, k, c, border, kp, g);
+    in_flow<LAY>(Fprev, N, i, j, k, c, border, kp, f);
     const double T = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
     // T^n of the half force: stored by the two-pass form, else the pull-sum of g_post^{n-1}
     double Tn;
@@ -997,10 +1204,37 @@ hipError_t launch_fused_batch_f64(const FusedGeom &fg0, const void *inst, const 
 static dim3 step_grid(const KParams &kp) { return dim3(kp.nblocks); }
 
 hipError_t launch_prologue(const KParams &kp, int bx, hipStream_t s, const double *Fin,
-                           double *Fout, const double *mac, double *fy_out, bool three_d)
+                           double *Fout, const double *mac, double *fy_out, bool three_d, bool aa)
 {
-    if (three_d) k3_prologue<<<step_grid(kp), bx, 0, s>>>(kp, Fin, Fout, mac);
+    if (three_d && aa) k3_prologue<true><<<step_grid(kp), bx, 0, s>>>(kp, nullptr, Fout, mac);
+    else if (three_d) k3_prologue<false><<<step_grid(kp), bx, 0, s>>>(kp, Fin, Fout, mac);
     else k2_prologue<<<step_grid(kp), bx, 0, s>>>(kp, Fin, Fout, mac, fy_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_flow3_aa(const KParams &kp, int bx, hipStream_t s, int lay, double *F,
+                           const double *G, const double *aux)
+{
+    if (lay == LAY_ODD) k3_flow<LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, nullptr, F, G, aux);
+    else if (lay == LAY_EVEN) k3_flow<LAY_EVEN><<<step_grid(kp), bx, 0, s>>>(kp, nullptr, F, G, aux);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_thermal3_aa(const KParams &kp, int bx, hipStream_t s, int lay, bool first,
+                              const double *F, double *G, double *Bp, double *aux,
+                              const double *mac)
+{
+    if (first && lay == LAY_ODD)
+        k3_thermal<true, LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac);
+    else if (first)
+        return hipErrorInvalidValue;   // the plain state is LAY_ODD
+    else if (lay == LAY_ODD)
+        k3_thermal<false, LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac);
+    else if (lay == LAY_EVEN)
+        k3_thermal<false, LAY_EVEN><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -1008,9 +1242,9 @@ hipError_t launch_step3(const KParams &kp, int bx, hipStream_t s, bool first, co
                         double *Fnext, const double *Gin, double *Gout, double *Bp, double *aux,
                         const double *mac)
 {
-    if (first) k3_thermal<true><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
-    else k3_thermal<false><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
-    k3_flow<<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Fnext, Gout, aux);
+    if (first) k3_thermal<true, LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
+    else k3_thermal<false, LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
+    k3_flow<LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Fnext, Gout, aux);
     return hipGetLastError();
 }
 
@@ -1026,9 +1260,14 @@ hipError_t launch_step2(const KParams &kp, int bx, hipStream_t s, bool first, co
 
 hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, const double *Fprev,
                         const double *Gpost, const double *aux_or_fy, const double *Gprev,
-                        double *mac, double *fo, double *go, int k0, int nk)
+                        double *mac, double *fo, double *go, int k0, int nk, int lay)
 {
-    if (three_d) k3_state<<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, Gprev, mac, fo, go, k0, nk);
+    if (three_d && lay == LAY_ODD)
+        k3_state<LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, nullptr, mac, fo, go, k0, nk);
+    else if (three_d && lay == LAY_EVEN)
+        k3_state<LAY_EVEN><<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, nullptr, mac, fo, go, k0, nk);
+    else if (three_d)
+        k3_state<LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, Gprev, mac, fo, go, k0, nk);
     else k2_state<<<step_grid(kp), bx, 0, s>>>(kp, Fprev, Gpost, aux_or_fy, mac, fo, go, k0, nk);
     return hipGetLastError();
 }

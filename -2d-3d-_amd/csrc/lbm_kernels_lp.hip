@@ -130,6 +130,100 @@ __device__ __forceinline__ float heat_in(const S *__restrict__ P, uint64_t N, in
     return -back + 2.0f * (float)wq<L::Q>(A) * tw;
 }
 
+// In-place (A-A) layout (LAY_ODD / LAY_EVEN, lbm_lattice.h): the same incoming values as the
+// pulls above, with every face a wall.
+__device__ __forceinline__ bool inside3_lp(int i, int j, int k, const KParams &kp)
+{
+    return (unsigned)i < (unsigned)kp.nx && (unsigned)j < (unsigned)kp.ny && (unsigned)k < (unsigned)kp.nz;
+}
+
+template <class L, typename S, int LAY, int A>
+__device__ __forceinline__ float flow_in_aa(const S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                            bool border, const KParams &kp)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    if (LAY == LAY_ODD) return Pop<S>::template ld<L::Q, A>(P + A * N + c);
+    const int64_t off = ex + ey * kp.nx + ez * (int64_t)kp.nx * kp.ny;
+    const bool in = !border || inside3_lp(i - ex, j - ey, k - ez, kp);
+    return Pop<S>::template ld<L::Q, A>(P + (in ? L::opp[A] * N + c - off : A * N + c));
+}
+
+template <class L, typename S, int LAY, int A>
+__device__ __forceinline__ float heat_in_aa(const S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                            bool border, const KParams &kp, const LPParams &lp)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    int kind = -1;
+    if (border) {
+        if (ex != 0) {
+            if ((unsigned)(i - ex) >= (unsigned)kp.nx) kind = kp.temp_x;
+        } else if (ey != 0) {
+            if ((unsigned)(j - ey) >= (unsigned)kp.ny) kind = kp.temp_y;
+        } else if (ez != 0) {
+            if ((unsigned)(k - ez) >= (unsigned)kp.nz) kind = kp.temp_z;
+        }
+    }
+    if (kind < 0) {
+        const int64_t off = ex + ey * kp.nx + ez * (int64_t)kp.nx * kp.ny;
+        const float v = LAY == LAY_ODD ? Pop<S>::template ld<L::Q, A>(P + A * N + c)
+                                       : Pop<S>::template ld<L::Q, A>(P + L::opp[A] * N + c - off);
+        return v + t0_pull<A>(lp);
+    }
+    // g_post of the opposite direction, in both states
+    const float back = Pop<S>::template ld<L::Q, L::opp[A]>(P + A * N + c);
+    if (kind == T_ADIABATIC) return back;
+    constexpr bool low_wall = (ex + ey + ez) > 0;
+    const float tw = (low_wall ? lp.thot : lp.tcold) - t0_of(lp, i, j);
+    return -back + 2.0f * (float)wq<L::Q>(A) * tw;
+}
+
+template <class L, typename S, int LAY, int A>
+__device__ __forceinline__ void store_aa(S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                         bool border, const KParams &kp, float v)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    if (LAY == LAY_ODD) {
+        Pop<S>::template st<L::Q, A>(P + L::opp[A] * N + c, v);
+        return;
+    }
+    const int64_t off = ex + ey * kp.nx + ez * (int64_t)kp.nx * kp.ny;
+    const bool in = !border || inside3_lp(i + ex, j + ey, k + ez, kp);
+    Pop<S>::template st<L::Q, A>(P + (in ? A * N + c + off : L::opp[A] * N + c), v);
+}
+
+// Incoming populations in layout LAY and the store of the post-collision ones.
+template <class L, typename S, int LAY, int... A>
+__device__ __forceinline__ void in_flow_lp(const S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                           bool border, const KParams &kp, float *f,
+                                           std::integer_sequence<int, A...>)
+{
+    if constexpr (LAY == LAY_PULL)
+        ((f[A] = flow_in<L, S, A>(P, N, i, j, k, c, border, kp)), ...);
+    else
+        ((f[A] = flow_in_aa<L, S, LAY, A>(P, N, i, j, k, c, border, kp)), ...);
+}
+template <class L, typename S, int LAY, int... A>
+__device__ __forceinline__ void in_heat_lp(const S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                           bool border, const KParams &kp, const LPParams &lp,
+                                           float *g, std::integer_sequence<int, A...>)
+{
+    if constexpr (LAY == LAY_PULL)
+        ((g[A] = heat_in<L, S, A>(P, N, i, j, k, c, border, kp, lp)), ...);
+    else
+        ((g[A] = heat_in_aa<L, S, LAY, A>(P, N, i, j, k, c, border, kp, lp)), ...);
+}
+template <class L, typename S, int LAY, int... A>
+__device__ __forceinline__ void out_lp(S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                       bool border, const KParams &kp, const float *v,
+                                       std::integer_sequence<int, A...>)
+{
+    if constexpr (LAY == LAY_PULL)
+        (Pop<S>::template st<L::Q, A>(P + A * N + c, v[A]), ...);
+    else
+        (store_aa<L, S, LAY, A>(P, N, i, j, k, c, border, kp, v[A]), ...);
+}
+constexpr int lay_flip_lp(int lay) { return lay == LAY_PULL ? LAY_PULL : 3 - lay; }
+
 template <class L, typename S, int... A>
 __device__ __forceinline__ void pull_flow_lp(const S *__restrict__ P, uint64_t N, int i, int j, int k,
                                              uint64_t c, bool border, const KParams &kp, float *f,
@@ -323,7 +417,8 @@ __device__ __forceinline__ void moments19(const float *f, float &drho, float &jx
 // -------------------------------------------------------------------------------------------
 // kernels
 // -------------------------------------------------------------------------------------------
-template <typename S>
+// In place (AA): Fin is Fout; the plain state (LAY_ODD) becomes LAY_EVEN.
+template <typename S, bool AA>
 __global__ void __launch_bounds__(256) lp3_prologue(KParams kp, LPParams lp, const S *__restrict__ Fin,
                                                     S *__restrict__ Fout, const double *__restrict__ mac)
 {
@@ -332,36 +427,37 @@ __global__ void __launch_bounds__(256) lp3_prologue(KParams kp, LPParams lp, con
     if (!node_of_lp(kp, i, j, k, c)) return;
     const uint64_t N = kp.N;
     float f[19];
-    load_all<D3Q19, S>(Fin, N, c, f, SEQ(D3Q19));
+    load_all<D3Q19, S>(AA ? Fout : Fin, N, c, f, SEQ(D3Q19));
     const float drho = (float)(mac[c] - 1.0);
     const float dT = (float)(mac[4 * N + c] - lp.tref);
     collide19_dev(f, f, drho, (float)mac[N + c], (float)mac[2 * N + c], (float)mac[3 * N + c], dT, lp);
-    store_all<D3Q19, S>(Fout, N, c, f, SEQ(D3Q19));
+    out_lp<D3Q19, S, AA ? LAY_ODD : LAY_PULL>(Fout, N, i, j, k, c, false, kp, f, SEQ(D3Q19));
 }
 
 // Pass A of step n (see lbm_kernels.hip).  aux holds dT^n as float.
-template <typename S, bool FIRST>
-__global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, const S *__restrict__ Fcur,
-                                                   const S *__restrict__ Gin, S *__restrict__ Gout,
-                                                   float *__restrict__ Bp, float *__restrict__ aux,
-                                                   const double *__restrict__ mac)
+// LAY: layout of the thermal lattice (LAY_PULL: Gin -> Gout; in place: Gout holds state LAY
+// and is updated, F is read in the other state).
+template <typename S, bool FIRST, int LAY>
+__device__ __forceinline__ void lp3_thermal_node(const KParams &kp, const LPParams &lp, int i, int j,
+                                                 int k, uint64_t c, const S *__restrict__ Fcur,
+                                                 const S *__restrict__ Gin, S *__restrict__ Gout,
+                                                 float *__restrict__ Bp, float *__restrict__ aux,
+                                                 const double *__restrict__ mac)
 {
-    int i, j, k;
-    uint64_t c;
-    if (!node_of_lp(kp, i, j, k, c)) return;
     const uint64_t N = kp.N;
     const bool border = on_border_lp(i, j, k, kp);
     const float t0 = t0_of(lp, i, j), b0 = (float)((double)t0 - lp.tref);
+    const S *G = LAY == LAY_PULL ? Gin : Gout;
     float g[7], dT;   // dT = T - T0(x)
-    if (FIRST) {
-        load_all<D3Q7, S>(Gin, N, c, g, SEQ(D3Q7));
+    if (FIRST) {   // the plain state; in place it is LAY_ODD, read the same way
+        load_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
         dT = (float)(mac[4 * N + c] - (double)t0);
     } else {
-        pull_heat_lp<D3Q7, S>(Gin, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+        in_heat_lp<D3Q7, S, LAY>(G, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
         dT = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
     }
     float f[19];
-    pull_flow_lp<D3Q19, S>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    in_flow_lp<D3Q19, S, lay_flip_lp(LAY)>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
     float drho, jx, jy, jz;
     moments19(f, drho, jx, jy, jz);
     const float rho = 1.0f + drho;
@@ -372,7 +468,7 @@ __global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, cons
     float bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
     float p[7];
     collide7_dev(g, p, dT, t0 + dT, u, v, w, bp, kp.use_g, lp);
-    store_all<D3Q7, S>(Gout, N, c, p, SEQ(D3Q7));
+    out_lp<D3Q7, S, LAY>(Gout, N, i, j, k, c, border, kp, p, SEQ(D3Q7));
     if (kp.use_g) {
         Bp[c] = bp[0];
         Bp[N + c] = bp[1];
@@ -381,22 +477,35 @@ __global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, cons
     aux[c] = bT;
 }
 
-template <typename S>
-__global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S *__restrict__ Fcur,
-                                                S *__restrict__ Fnext, const S *__restrict__ Gpost,
-                                                const float *__restrict__ aux)
+template <typename S, bool FIRST, int LAY>
+__global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, const S *__restrict__ Fcur,
+                                                   const S *__restrict__ Gin, S *__restrict__ Gout,
+                                                   float *__restrict__ Bp, float *__restrict__ aux,
+                                                   const double *__restrict__ mac)
 {
     int i, j, k;
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
+    lp3_thermal_node<S, FIRST, LAY>(kp, lp, i, j, k, c, Fcur, Gin, Gout, Bp, aux, mac);
+}
+
+// LAY: layout of both lattices (LAY_PULL: Fcur -> Fnext; in place: Fnext holds state LAY and
+// is updated, Gpost is read in state LAY).
+template <typename S, int LAY>
+__device__ __forceinline__ void lp3_flow_node(const KParams &kp, const LPParams &lp, int i, int j,
+                                              int k, uint64_t c, const S *__restrict__ Fcur,
+                                              S *__restrict__ Fnext, const S *__restrict__ Gpost,
+                                              const float *__restrict__ aux)
+{
     const uint64_t N = kp.N;
     const bool border = on_border_lp(i, j, k, kp);
     float g[7];
-    pull_heat_lp<D3Q7, S>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+    in_heat_lp<D3Q7, S, LAY>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
     const float dT1 = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
     const float b0 = (float)((double)t0_of(lp, i, j) - lp.tref);
     float f[19];
-    pull_flow_lp<D3Q19, S>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    in_flow_lp<D3Q19, S, LAY>(LAY == LAY_PULL ? Fcur : Fnext, N, i, j, k, c, border, kp, f,
+                              SEQ(D3Q19));
     float drho, jx, jy, jz;
     moments19(f, drho, jx, jy, jz);
     const float rho = 1.0f + drho;
@@ -404,11 +513,74 @@ __global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S
     const float Fy = rho * lp.gbeta * aux[c];
     const float u = jx * ir, v = (jy + 0.5f * Fy) * ir, w = jz * ir;
     collide19_dev(f, f, drho, u, v, w, b0 + dT1, lp);
-    store_all<D3Q19, S>(Fnext, N, c, f, SEQ(D3Q19));
+    out_lp<D3Q19, S, LAY>(Fnext, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+}
+
+template <typename S, int LAY>
+__global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S *__restrict__ Fcur,
+                                                S *__restrict__ Fnext, const S *__restrict__ Gpost,
+                                                const float *__restrict__ aux)
+{
+    int i, j, k;
+    uint64_t c;
+    if (!node_of_lp(kp, i, j, k, c)) return;
+    lp3_flow_node<S, LAY>(kp, lp, i, j, k, c, Fcur, Fnext, Gpost, aux);
+}
+
+// Batched two-pass step over a per-domain table (see k3_thermal_batch).
+template <typename S> struct TwoPassEntryLP {
+    KParams kp;
+    LPParams lp;
+    const S *Fcur;
+    S *Fnext;
+    const S *Gin;
+    S *Gout;
+    float *Bp, *aux;
+    unsigned first_block;
+};
+
+__device__ __forceinline__ bool batch_node_lp(const KParams &kp, unsigned lb, int &i, int &j,
+                                              int &k, uint64_t &c)
+{
+    c = (uint64_t)lb * 256u + threadIdx.x;
+    if (c >= kp.N) return false;
+    const uint64_t nx = (uint64_t)kp.nx, row = c / nx;
+    i = (int)(c - row * nx);
+    j = (int)(row % (uint64_t)kp.ny);
+    k = (int)(row / (uint64_t)kp.ny);
+    return true;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) lp3_thermal_batch(const TwoPassEntryLP<S> *__restrict__ tab,
+                                                         const uint16_t *__restrict__ owner,
+                                                         unsigned nblocks)
+{
+    const unsigned b = xcd_remap(blockIdx.x, nblocks);
+    const TwoPassEntryLP<S> &d = tab[owner[b]];
+    int i, j, k;
+    uint64_t c;
+    if (!batch_node_lp(d.kp, b - d.first_block, i, j, k, c)) return;
+    lp3_thermal_node<S, false, LAY_PULL>(d.kp, d.lp, i, j, k, c, gptr(d.Fcur), gptr(d.Gin), gptr(d.Gout),
+                               gptr(d.Bp), gptr(d.aux), nullptr);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) lp3_flow_batch(const TwoPassEntryLP<S> *__restrict__ tab,
+                                                      const uint16_t *__restrict__ owner,
+                                                      unsigned nblocks)
+{
+    const unsigned b = xcd_remap(blockIdx.x, nblocks);
+    const TwoPassEntryLP<S> &d = tab[owner[b]];
+    int i, j, k;
+    uint64_t c;
+    if (!batch_node_lp(d.kp, b - d.first_block, i, j, k, c)) return;
+    lp3_flow_node<S, LAY_PULL>(d.kp, d.lp, i, j, k, c, gptr(d.Fcur), gptr(d.Fnext), gptr(d.Gout),
+                     gptr(d.aux));
 }
 
 // Reference state (absolute FP64) from the pipelined buffers; see k3_state.
-template <typename S>
+template <typename S, int LAY>
 __global__ void __launch_bounds__(256) lp3_state(KParams kp, LPParams lp, const S *__restrict__ Fprev,
                                                  const S *__restrict__ Gpost, const float *__restrict__ aux,
                                                  const S *__restrict__ Gprev,
@@ -421,8 +593,8 @@ __global__ void __launch_bounds__(256) lp3_state(KParams kp, LPParams lp, const 
     const uint64_t N = kp.N;
     const bool border = on_border_lp(i, j, k, kp);
     float g[7], f[19];
-    pull_heat_lp<D3Q7, S>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
-    pull_flow_lp<D3Q19, S>(Fprev, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    in_heat_lp<D3Q7, S, LAY>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+    in_flow_lp<D3Q19, S, LAY>(Fprev, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
     double drho = 0.0, jx = 0.0, jy = 0.0, jz = 0.0, dT = 0.0;
 #pragma unroll
     for (int a = 0; a < 19; ++a) {
@@ -706,18 +878,46 @@ static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, 
     const S *Fc = (const S *)a.Fc;
     switch (op) {
     case LP_PROLOGUE:
-        lp3_prologue<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, a.mac);
+        if (a.lay == LAY_PULL) lp3_prologue<S, false><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, a.mac);
+        else lp3_prologue<S, true><<<grid, bx, 0, s>>>(kp, lp, nullptr, (S *)a.Fn, a.mac);
         break;
     case LP_STEP:
         if (a.first)
-            lp3_thermal<S, true><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
+            lp3_thermal<S, true, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
         else
-            lp3_thermal<S, false><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
-        lp3_flow<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, (const S *)a.Go, a.aux);
+            lp3_thermal<S, false, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
+        lp3_flow<S, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, (const S *)a.Go, a.aux);
+        break;
+    case LP_FLOW_AA:
+        if (a.lay == LAY_ODD)
+            lp3_flow<S, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, nullptr, (S *)a.Fn, (const S *)a.Gi, a.aux);
+        else if (a.lay == LAY_EVEN)
+            lp3_flow<S, LAY_EVEN><<<grid, bx, 0, s>>>(kp, lp, nullptr, (S *)a.Fn, (const S *)a.Gi, a.aux);
+        else
+            return hipErrorInvalidValue;
+        break;
+    case LP_THERMAL_AA:
+        if (a.first && a.lay == LAY_ODD)
+            lp3_thermal<S, true, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac);
+        else if (a.first)
+            return hipErrorInvalidValue;   // the plain state is LAY_ODD
+        else if (a.lay == LAY_ODD)
+            lp3_thermal<S, false, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac);
+        else if (a.lay == LAY_EVEN)
+            lp3_thermal<S, false, LAY_EVEN><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac);
+        else
+            return hipErrorInvalidValue;
         break;
     case LP_STATE:
-        lp3_state<S><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux, (const S *)a.Gprev,
-                                         a.mac, a.fo, a.go, a.k0, a.nk);
+        if (a.lay == LAY_ODD)
+            lp3_state<S, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux, nullptr,
+                                                      a.mac, a.fo, a.go, a.k0, a.nk);
+        else if (a.lay == LAY_EVEN)
+            lp3_state<S, LAY_EVEN><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux, nullptr,
+                                                       a.mac, a.fo, a.go, a.k0, a.nk);
+        else
+            lp3_state<S, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, a.aux,
+                                                       (const S *)a.Gprev, a.mac, a.fo, a.go, a.k0, a.nk);
         break;
     case LP_INIT:
         lp_init<S><<<grid, bx, 0, s>>>(kp, lp, (S *)a.Fn, (S *)a.Go, a.Bp, a.mac);
@@ -769,6 +969,52 @@ static void fill_lp(void *dst, const KParams &kp, const FusedGeom &fg, const Fus
     d.Bpout = (float *)p.Bpout;
     d.first_block = first_block;
     memcpy(dst, &d, sizeof(d));
+}
+
+template <typename S>
+static void fill_twopass_lp(void *dst, const KParams &kp, const FusedPtrs &p, float *aux,
+                            unsigned first_block)
+{
+    TwoPassEntryLP<S> d;
+    d.kp = kp;
+    d.lp = lp_params(kp);
+    d.Fcur = (const S *)p.Fcur;
+    d.Fnext = (S *)p.Fnext;
+    d.Gin = (const S *)p.Gin;
+    d.Gout = (S *)p.Gout;
+    d.Bp = (float *)p.Bp;
+    d.aux = aux;
+    d.first_block = first_block;
+    memcpy(dst, &d, sizeof(d));
+}
+
+size_t twopass_entry_bytes_lp(int precision)
+{
+    return precision == 1 ? sizeof(TwoPassEntryLP<float>) : sizeof(TwoPassEntryLP<_Float16>);
+}
+
+void twopass_entry_fill_lp(int precision, void *dst, const KParams &kp, const FusedPtrs &p,
+                           float *aux, unsigned first_block)
+{
+    if (precision == 1) fill_twopass_lp<float>(dst, kp, p, aux, first_block);
+    else fill_twopass_lp<_Float16>(dst, kp, p, aux, first_block);
+}
+
+template <typename S>
+static hipError_t twopass_batch(const void *tab, const uint16_t *owner, unsigned nblocks,
+                                hipStream_t s)
+{
+    const TwoPassEntryLP<S> *d = (const TwoPassEntryLP<S> *)tab;
+    lp3_thermal_batch<S><<<nblocks, 256, 0, s>>>(d, owner, nblocks);
+    lp3_flow_batch<S><<<nblocks, 256, 0, s>>>(d, owner, nblocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_twopass_batch_lp(int precision, const void *tab, const uint16_t *owner,
+                                   unsigned nblocks, hipStream_t s)
+{
+    if (precision == 1) return twopass_batch<float>(tab, owner, nblocks, s);
+    return twopass_batch<_Float16>(tab, owner, nblocks, s);
 }
 
 size_t fused_instance_bytes_lp(int precision)

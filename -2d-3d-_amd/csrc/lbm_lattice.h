@@ -49,6 +49,20 @@ __host__ __device__ constexpr double w5(int a) { return a == 0 ? 1.0 / 3.0 : 1.0
 enum { VEL_NOSLIP = 0, VEL_PERIODIC = 1 };
 enum { T_CONST = 0, T_ADIABATIC = 1, T_PERIODIC = 2 };
 
+// Population layouts of the two-pass step.
+//   LAY_PULL: two arrays per lattice; a node stores its post-collision populations in its own
+//     slots of the other array, and the next pass pulls population a of node x from slot
+//     (a, x - e_a) (or, at a wall, from its own slot of the opposite direction).
+//   In-place (A-A) layout: one array per lattice, every face a wall (no periodic axis).  In
+//     state LAY_ODD the incoming population a of node x (what the pull returns) sits in slot
+//     (a, x).  In state LAY_EVEN it sits in slot (opp a, x - e_a), or in (a, x) when x - e_a
+//     is outside (the value the wall sends back).  A node that stores its post-collision
+//     populations into the slots it read turns one state into the other:
+//       from LAY_ODD:  f*_a -> (opp a, x)
+//       from LAY_EVEN: f*_a -> (a, x + e_a), or (opp a, x) when x + e_a is outside.
+//     Each slot is read and written by one node only, so a pass updates the array in place.
+enum { LAY_PULL = 0, LAY_ODD = 1, LAY_EVEN = 2 };
+
 // Everything a step kernel needs, passed by value (lives in SGPRs / kernarg segment).
 struct KParams {
     int nx, ny, nz;
@@ -88,6 +102,19 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks)
     unsigned base = x * per + (x < rem ? x : rem);
     unsigned cnt = per + (x < rem ? 1u : 0u);
     return r < cnt ? base + r : b;  // b beyond the even split keeps identity (not reached)
+}
+
+// A pointer read from device memory (a parameter table) is generic to the compiler, which then
+// emits flat loads and stores; flat operations share the counter of the scalar loads that
+// re-read table parameters, so each such reload would wait for every load in flight.  The
+// pointer is cast to the global address space and made opaque (an empty asm on a uniform
+// value), so that the cast back to a generic pointer is not folded away and the compiler
+// infers global operations for every access derived from it.  p must be wave-uniform.
+template <typename T> __device__ __forceinline__ T *gptr(T *p)
+{
+    auto q = (__attribute__((address_space(1))) T *)p;
+    asm("" : "+s"(q));
+    return (T *)q;
 }
 
 } // namespace rblbm

@@ -4,9 +4,9 @@
 Ensemble: 64 cubic cavities, edge drawn uniformly (seeded) from {32, 48, 64, 96, 128},
 Ra log-uniform in [1e4, 1e7], Pr in {0.71, 7.0}, FP32-shifted storage, seeded initial noise
 (instance index in the noise key).  Aggregate GLUPS = sum of nodes * steps / device seconds,
-timed for each stepping mode of the batch:
-  per_domain   one launch per cavity per step
-  one_launch   one launch per step for all cavities (per-cavity parameter/offset table)
+timed for each kernel form (two_pass, one_pass) and stepping mode of the batch:
+  per_domain   one launch per cavity per pass
+  one_launch   one launch per pass for all cavities (per-cavity parameter/offset table)
   graph        one_launch, two steps captured in a HIP graph and replayed
 Prints one JSON line.
 """
@@ -49,18 +49,23 @@ def main():
     out = {"workload": f"{a.n} cavities, edges {sorted(set(c.nx for c in cfgs))}, "
                        f"Ra log-uniform [1e4,1e7], Pr in {{0.71,7}}, {a.precision}",
            "nodes": nodes, "steps": a.steps, "modes": {}}
-    for name, mode in (("per_domain", rblbm.BATCH_PER_DOMAIN),
-                       ("one_launch", rblbm.BATCH_ONE_LAUNCH),
-                       ("graph", rblbm.BATCH_GRAPH)):
-        b.set_mode(mode)
-        b.step(a.warmup)
-        ms = b.step_timed(a.steps)
-        out["modes"][name] = {"ms_per_step": ms / a.steps,
-                              "glups": nodes * a.steps / (ms / 1e3) / 1e9}
-        print(f"{name}: {out['modes'][name]}", file=sys.stderr, flush=True)
+    for form, variant in (("two_pass", rblbm.KERNEL_TWO_PASS), ("one_pass", rblbm.KERNEL_ONE_PASS)):
+        b.set_kernel_variant(variant)
+        for name, mode in (("per_domain", rblbm.BATCH_PER_DOMAIN),
+                           ("one_launch", rblbm.BATCH_ONE_LAUNCH),
+                           ("graph", rblbm.BATCH_GRAPH)):
+            b.set_mode(mode)
+            b.step(a.warmup)
+            ms = b.step_timed(a.steps)
+            key = f"{form}/{name}"
+            out["modes"][key] = {"ms_per_step": ms / a.steps,
+                                 "glups": nodes * a.steps / (ms / 1e3) / 1e9}
+            print(f"{key}: {out['modes'][key]}", file=sys.stderr, flush=True)
     d = b.domains[0].diagnostics()
     out["domain0_nu_vol"] = d["nu_vol"]
-    out["graph_vs_per_domain"] = out["modes"]["graph"]["glups"] / out["modes"]["per_domain"]["glups"]
+    out["graph_vs_per_domain"] = {
+        f: out["modes"][f + "/graph"]["glups"] / out["modes"][f + "/per_domain"]["glups"]
+        for f in ("two_pass", "one_pass")}
     b.close()
     print(json.dumps(out), flush=True)
 

@@ -14,6 +14,7 @@ module lbm_rb
   integer(c_int), parameter :: LBM_VEL_NOSLIP = 0, LBM_VEL_PERIODIC = 1
   integer(c_int), parameter :: LBM_T_CONST = 0, LBM_T_ADIABATIC = 1, LBM_T_PERIODIC = 2
   integer(c_int), parameter :: LBM_FP64 = 0, LBM_FP32_SHIFT = 1, LBM_FP16_SHIFT = 2
+  integer(c_int), parameter :: LBM_LAYOUT_TWO_ARRAY = 0, LBM_LAYOUT_AA = 1, LBM_LAYOUT_AUTO = 2
 
   type, bind(C) :: lbm_config
     integer(c_int32_t) :: dims, nx, ny, nz
@@ -21,6 +22,7 @@ module lbm_rb
     integer(c_int32_t) :: side_heated, vel_x, vel_z, temp_x, temp_y, temp_z, use_g
     integer(c_int32_t) :: precision, device
     integer(c_int32_t) :: legacy_thermal = 0
+    integer(c_int32_t) :: layout = 0          ! LBM_LAYOUT_*
   end type lbm_config
 
   type, bind(C) :: lbm_params

@@ -33,6 +33,7 @@ program rb_main
   integer :: tempX = LBM_T_ADIABATIC, tempY = LBM_T_CONST, tempZ = LBM_T_ADIABATIC
   integer :: useG = 1
   integer :: legacyThermal = 0       ! EnableLegacyThermalScheme (FP64 form only)
+  integer :: layout = 0              ! population layout: 0 two arrays, 1 in place (A-A), 2 auto
   integer :: precision = LBM_FP64
   integer :: device = 0
   integer :: seeded = 1
@@ -48,7 +49,7 @@ program rb_main
   namelist /rb/ dims, nx, ny, nz, Rayleigh, Prandtl, Mach, Thot, Tcold, itc_max, checkInterval, &
        epsU, epsT, sideHeated, velX, velZ, tempX, tempY, tempZ, useG, precision, device, &
        seeded, seed, noiseAmp, loadInitField, reloadFile, outputBinFile, backupAtEnd, settingsFile, &
-       outputPltFile, pltFolderPrefix, legacyThermal
+       outputPltFile, pltFolderPrefix, legacyThermal, layout
 
   type(lbm_config) :: cfg
   type(lbm_params) :: prm
@@ -172,6 +173,7 @@ contains
     cfg%temp_x = tempX; cfg%temp_y = tempY; cfg%temp_z = tempZ
     cfg%use_g = useG; cfg%precision = precision; cfg%device = device
     cfg%legacy_thermal = legacyThermal
+    cfg%layout = layout
     call lbm_require(lbm_create(cfg, h), 'lbm_create', h, 6)
     call lbm_require(lbm_get_params(h, prm), 'lbm_get_params', h, 6)
 

@@ -22,12 +22,13 @@ VEL_NOSLIP, VEL_PERIODIC = 0, 1
 T_CONST, T_ADIABATIC, T_PERIODIC = 0, 1, 2
 FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_ONE_PASS = 0, 1, 2
+LAYOUT_TWO_ARRAY, LAYOUT_AA, LAYOUT_AUTO = 0, 1, 2
 TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32 = 0, 1, 2, 3
 
 EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
            "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",
            "lbm_diagnostics", "lbm_midplane_extrema", "lbm_gather", "lbm_scatter", "lbm_bytes_per_update",
-           "lbm_set_kernel_variant", "lbm_set_fused_tile", "lbm_get_kernel_form",
+           "lbm_set_kernel_variant", "lbm_set_fused_tile", "lbm_get_kernel_form", "lbm_get_layout",
            "lbm_last_error", "lbm_error_string")
 
 
@@ -43,7 +44,8 @@ class _Config(C.Structure):
                 ("thot", C.c_double), ("tcold", C.c_double), ("side_heated", C.c_int32),
                 ("vel_x", C.c_int32), ("vel_z", C.c_int32), ("temp_x", C.c_int32),
                 ("temp_y", C.c_int32), ("temp_z", C.c_int32), ("use_g", C.c_int32),
-                ("precision", C.c_int32), ("device", C.c_int32), ("legacy_thermal", C.c_int32)]
+                ("precision", C.c_int32), ("device", C.c_int32), ("legacy_thermal", C.c_int32),
+                ("layout", C.c_int32)]
 
 
 class _Params(C.Structure):
@@ -99,6 +101,7 @@ def library(path: str = LIB_PATH):
     L.lbm_set_fused_tile.argtypes = [h, C.c_int32, C.c_int32]
     ip = C.POINTER(C.c_int32)
     L.lbm_get_kernel_form.argtypes = [h, ip, ip, ip, ip]
+    L.lbm_get_layout.argtypes = [h, ip]
     L.lbm_last_error.argtypes = [h]
     L.lbm_last_error.restype = C.c_char_p
     L.lbm_error_string.argtypes = [C.c_int]
@@ -136,6 +139,7 @@ class Config:
     precision: int = FP64
     device: int = 0
     legacy: int = 0           # EnableLegacyThermalScheme (FP64 form only)
+    layout: int = 0           # LAYOUT_TWO_ARRAY / LAYOUT_AA / LAYOUT_AUTO
 
     @property
     def shape(self):
@@ -235,6 +239,13 @@ class Solver:
         v = [C.c_int32() for _ in range(4)]
         self._chk(self.lib.lbm_get_kernel_form(self._h, *[C.byref(x) for x in v]))
         return dict(zip(("variant", "kchunk", "tile", "prefetch"), (x.value for x in v)))
+
+    @property
+    def layout(self) -> int:
+        """The population layout in effect: LAYOUT_TWO_ARRAY or LAYOUT_AA."""
+        v = C.c_int32()
+        self._chk(self.lib.lbm_get_layout(self._h, C.byref(v)))
+        return v.value
 
     def autotune(self, cache: str | None = None, steps: int = 10) -> float:
         """Pick the kernel form and z-chunk for this shape (lbm_autotune); returns ms/step."""
@@ -387,6 +398,11 @@ class Batch:
 
     def set_mode(self, mode: int):
         self._chk(self.lib.lbm_batch_set_mode(self._b, int(mode)))
+
+    def set_kernel_variant(self, variant: int):
+        L = self.lib
+        L.lbm_batch_set_kernel_variant.argtypes = [C.c_void_p, C.c_int32]
+        self._chk(L.lbm_batch_set_kernel_variant(self._b, int(variant)))
 
     def autotune(self, cache: str | None = None, steps: int = 10):
         L = self.lib

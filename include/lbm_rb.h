@@ -58,7 +58,17 @@ typedef struct lbm_config {
     int32_t device;               /* HIP device ordinal */
     int32_t legacy_thermal;       /* EnableLegacyThermalScheme (FP64 form only): fixed Qk, Qnu,
                                      rest weight from paraA (2DRB/3DRB.F90:93, :116-118) */
+    int32_t layout;               /* LBM_LAYOUT_*: population arrays of the step (no reference
+                                     counterpart; results do not depend on it) */
 } lbm_config;
+
+/* Population layouts.  TWO_ARRAY: two arrays per lattice, every kernel form.  AA: one array per
+ * lattice updated in place (half the population memory; 3D with a wall on every face, two-pass
+ * form only).  AUTO: TWO_ARRAY when it fits in free device memory, else AA where allowed;
+ * lbm_autotune may switch an AUTO domain that has not stepped yet to the faster of the two. */
+#define LBM_LAYOUT_TWO_ARRAY 0
+#define LBM_LAYOUT_AA 1
+#define LBM_LAYOUT_AUTO 2
 
 /* Derived parameters: module commondata3d, 2DRB/3DRB.F90:73-123 (2D: 2DRB/2DRB.F90:61-115). */
 typedef struct lbm_params {
@@ -184,6 +194,8 @@ int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 #define LBM_TILE_32X8 2
 #define LBM_TILE_32X32 3
 int lbm_set_fused_tile(lbm_handle *h, int32_t tile, int32_t prefetch);
+/* The population layout in effect (LBM_LAYOUT_TWO_ARRAY or LBM_LAYOUT_AA). */
+int lbm_get_layout(lbm_handle *h, int32_t *layout);
 /* The kernel form in effect: variant as requested (LBM_KERNEL_*), z-chunk, tile, prefetch. */
 int lbm_get_kernel_form(lbm_handle *h, int32_t *variant, int32_t *kchunk, int32_t *tile,
                         int32_t *prefetch);
@@ -200,7 +212,11 @@ int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *m
 /* ---- Batched ensembles (no reference counterpart: the reference runs one cavity per
  * process).  A batch owns n independent domains of any shape and parameters; they share one
  * HIP stream and advance together.  Domain i is an ordinary handle for every per-domain call
- * (init, diagnostics, macros, gather); its seeded noise uses i as the instance key. */
+ * (init, diagnostics, macros, gather); its seeded noise uses i as the instance key.
+ * The one-launch modes run the two-pass form (two launches a step over 256-node blocks of all
+ * domains) unless every domain is set to LBM_KERNEL_ONE_PASS (lbm_batch_set_kernel_variant or
+ * lbm_batch_autotune), then the one-pass form (one launch a step; the tile shape and prefetch
+ * of domain 0 for all, each domain's own z-chunk). */
 typedef struct lbm_batch lbm_batch;
 #define LBM_BATCH_PER_DOMAIN 0    /* one launch per domain per step */
 #define LBM_BATCH_ONE_LAUNCH 1    /* one launch per step for all domains (per-domain table) */
@@ -210,6 +226,8 @@ int lbm_batch_destroy(lbm_batch *b);
 int lbm_batch_domain(lbm_batch *b, int32_t i, lbm_handle **h);
 int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp);
 int lbm_batch_set_mode(lbm_batch *b, int32_t mode);
+/* lbm_set_kernel_variant for every domain of the batch (LBM_KERNEL_*; z-chunks kept). */
+int lbm_batch_set_kernel_variant(lbm_batch *b, int32_t variant);
 int lbm_batch_step(lbm_batch *b, int32_t nsteps);
 int lbm_batch_step_timed(lbm_batch *b, int32_t nsteps, double *ms);
 int lbm_batch_nodes(lbm_batch *b, int64_t *nodes);

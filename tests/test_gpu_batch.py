@@ -1,5 +1,6 @@
 """Batched ensembles: independent cavities of different shapes and parameters advancing in one
-launch per step (plain or HIP-graph replay) or one launch per domain, against the oracle."""
+table launch per pass (two-pass form) or per step (one-pass form), plain or HIP-graph replay,
+or one launch per domain, against the oracle."""
 import numpy as np
 import pytest
 
@@ -26,10 +27,15 @@ def _oracles(seed, amp):
     return out
 
 
+VARIANTS = [rblbm.KERNEL_TWO_PASS, rblbm.KERNEL_ONE_PASS]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("mode", [rblbm.BATCH_PER_DOMAIN, rblbm.BATCH_ONE_LAUNCH, rblbm.BATCH_GRAPH])
-def test_batch_fp64_bitwise_against_oracle(mode):
+def test_batch_fp64_bitwise_against_oracle(mode, variant):
     b = rblbm.Batch(_cfgs(rblbm.FP64))
     b.set_mode(mode)
+    b.set_kernel_variant(variant)
     b.initial_seeded(31, 0.05)
     ora = _oracles(31, 0.05)
     done = 0
@@ -49,9 +55,11 @@ def test_batch_fp64_bitwise_against_oracle(mode):
     b.close()
 
 
-def test_batch_fp32_graph_tracks_oracle():
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_batch_fp32_graph_tracks_oracle(variant):
     b = rblbm.Batch(_cfgs(rblbm.FP32_SHIFT))
     b.set_mode(rblbm.BATCH_GRAPH)
+    b.set_kernel_variant(variant)
     b.initial_seeded(5, 0.05)
     ora = _oracles(5, 0.05)
     b.step(101)                     # first step two-pass, then 50 graph replays
@@ -74,4 +82,23 @@ def test_batch_rejects_mixed_precision_and_owned_domain_destroy():
     b = rblbm.Batch(_cfgs(rblbm.FP64)[:2])
     lib = b.lib
     assert lib.lbm_destroy(b.domains[0]._h) == -4
+    with pytest.raises(rblbm.LbmError):
+        b.set_kernel_variant(7)
     b.close()
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_batch_fp16_table_launch_equals_per_domain(variant):
+    # the table launches run the same node arithmetic as the per-domain kernels: bitwise equal
+    def run(mode):
+        b = rblbm.Batch(_cfgs(rblbm.FP16_SHIFT))
+        b.set_mode(mode)
+        b.set_kernel_variant(variant)
+        b.initial_seeded(9, 0.05)
+        b.step(23)
+        out = [d.macros() for d in b.domains]
+        b.close()
+        return out
+    for m, r in zip(run(rblbm.BATCH_GRAPH), run(rblbm.BATCH_PER_DOMAIN)):
+        for k in ("rho", "u", "v", "w", "T"):
+            np.testing.assert_array_equal(m[k], r[k])
