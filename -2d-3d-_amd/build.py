@@ -16,8 +16,10 @@ LIB = os.path.join(LIBDIR, "librblbm.so")
 # FP64 kernels and host code: no contraction, so FP64 results round exactly like the
 # reference's CPU build.
 SOURCES = {"lbm_kernels.hip": ["-ffp-contract=off"], "lbm_api.hip": ["-ffp-contract=off"],
-           # reduced-precision forms: FP32 arithmetic with fused multiply-add
-           "lbm_kernels_lp.hip": ["-ffp-contract=fast"]}
+           # reduced-precision forms: FP32 arithmetic with fused multiply-add inside each source
+           # expression only, so that every kernel instance (layout, kernel form, tile) fuses the
+           # same products and the forms give bitwise the same results
+           "lbm_kernels_lp.hip": ["-ffp-contract=on"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RBLBM_ARCH", "gfx950")
 

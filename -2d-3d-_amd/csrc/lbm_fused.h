@@ -116,6 +116,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
             const R Tn = Form::tsum(lg);
             Form::macro(fnew, Tn, kp, fp, nd, rho_n, u_n, v_n, w_n);
             Form::thermal(lg, gnew, Tn, u_n, v_n, w_n, bp, kp, fp, nd);
+            Form::round_g(gnew);   // exchanged and stored as the storage rounds them
             if (tile && kk >= k0 && kk < k1) {
                 Form::store_g(Gout, N, c, gnew);
                 if (kp.use_g) {

@@ -1108,6 +1108,7 @@ struct FormF64 {
     static __device__ __forceinline__ Node node(const KParams &, const P &, int, int) { return Node{}; }
     // a neighbour's post-collision population of direction a pulled into this node: unchanged
     static __device__ __forceinline__ double pulled(int, double v, const P &) { return v; }
+    static __device__ __forceinline__ void round_g(double *) {}
     static __device__ __forceinline__ void pull_g(const double *G, uint64_t N, int i, int j, int k,
                                                   uint64_t c, bool border, const KParams &kp,
                                                   const P &, double *g)

@@ -758,6 +758,19 @@ template <typename SS> struct FormLP {
         const float t0 = t0_of(lp, i, j);
         return Node{t0, (float)((double)t0 - lp.tref)};
     }
+    // post-collision thermal deviations as the storage holds them (FP16: rounded; the weights
+    // of D3Q7 are powers of two, so a second rounding leaves them unchanged): the values the
+    // one-pass kernel exchanges are then the ones the two-pass form reads back
+    static __device__ __forceinline__ void round_g(float *g)
+    {
+        if constexpr (sizeof(S) < sizeof(float)) {
+#pragma unroll
+            for (int a = 0; a < 7; ++a) {
+                const float w = (float)w7(a);
+                g[a] = (float)(S)(g[a] * (1.0f / w)) * w;
+            }
+        }
+    }
     // a neighbour's post-collision deviation of direction a pulled into this node
     static __device__ __forceinline__ float pulled(int a, float v, const P &lp)
     {

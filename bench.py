@@ -52,8 +52,12 @@ def parse():
     ap.add_argument("--tile", type=int, default=0, help="one-pass thread tile (0: 32x16, 1: 16x16, 2: 32x8)")
     ap.add_argument("--prefetch", type=int, default=0, help="one-pass prefetching (0/1)")
     ap.add_argument("--kchunk", type=int, default=0, help="z-planes per one-pass block (0: default)")
-    ap.add_argument("--layout", choices=["two-array", "aa", "auto"], default="two-array",
-                    help="population layout (aa: one array per lattice updated in place)")
+    ap.add_argument("--layout", choices=["tuned", "two-array", "aa", "auto"], default="tuned",
+                    help="population layout; tuned (default): an LBM_LAYOUT_AUTO domain whose "
+                         "layout and kernel form lbm_autotune picks before the first step "
+                         "(--variant, --kchunk, --tile, --prefetch are then ignored); aa: one "
+                         "array per lattice updated in place (two-pass form only)")
+    ap.add_argument("--tune-steps", type=int, default=8, help="timed steps per autotune candidate")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run a device-to-device copy of GIB GiB (counter calibration)")
     return ap.parse_args()
@@ -123,13 +127,20 @@ def copy_gbs(torch, gib=2.0, reps=5):
 
 
 def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=0, prefetch=0,
-            layout=0):
-    """Time `steps` steps of one storage form on the n^3 cavity (inputs resident in HBM)."""
+            layout=0, tune_steps=0):
+    """Time `steps` steps of one storage form on the n^3 cavity (inputs resident in HBM).
+    tune_steps > 0: lbm_autotune picks the layout and kernel form first (outside the timing)."""
     cfg = rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=ra, prandtl=0.71, mach=0.1,
                        precision=prec, layout=layout)
     sol = rblbm.Solver(cfg)
-    sol.set_kernel_variant(variant, kchunk)
-    sol.set_fused_tile(tile, prefetch)
+    tune_s = None
+    if tune_steps > 0:
+        t0 = time.perf_counter()
+        sol.autotune(None, steps=tune_steps)
+        tune_s = time.perf_counter() - t0
+    else:
+        sol.set_kernel_variant(variant, kchunk)
+        sol.set_fused_tile(tile, prefetch)
     form = sol.kernel_form()
     aa = sol.layout == rblbm.LAYOUT_AA
     sol.initial_seeded(SEED, 1e-3)
@@ -153,6 +164,8 @@ def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=
                        "k3_thermal + k3_flow (two-pass step" + (", in place)" if aa else ")"))
     form["name"] = "one-pass" if one else ("two-pass-aa" if aa else "two-pass")
     form["layout"] = "aa" if aa else "two-array"
+    if tune_s is not None:
+        form["chosen_by"] = f"lbm_autotune ({tune_s:.1f} s)"
     return {"glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms_step, "form": form,
             "bytes_per_update": bpu, "achieved_gbs": bpu * nodes / (ms_step / 1e3) / 1e9,
             "roofline_frac": bpu * nodes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
@@ -185,10 +198,12 @@ def main():
         prec = {"fp64": rblbm.FP64, "fp32": rblbm.FP32_SHIFT, "fp16": rblbm.FP16_SHIFT}[name]
         variant = {"auto": rblbm.KERNEL_AUTO, "two-pass": rblbm.KERNEL_TWO_PASS,
                    "one-pass": rblbm.KERNEL_ONE_PASS}[a.variant]
+        lname = "two-array" if a.variant == "one-pass" and a.layout != "tuned" else a.layout
         layout = {"two-array": rblbm.LAYOUT_TWO_ARRAY, "aa": rblbm.LAYOUT_AA,
-                  "auto": rblbm.LAYOUT_AUTO}[a.layout]
+                  "auto": rblbm.LAYOUT_AUTO, "tuned": rblbm.LAYOUT_AUTO}[lname]
         forms[name] = measure(rblbm, torch if have_torch else None, n, a.rayleigh, prec,
-                              a.steps, a.warmup, variant, a.kchunk, a.tile, a.prefetch, layout)
+                              a.steps, a.warmup, variant, a.kchunk, a.tile, a.prefetch, layout,
+                              a.tune_steps if lname == "tuned" else 0)
     main_form = forms[a.forms.split(",")[0]]
     ms_step = main_form["ms_per_step"]
     glups = main_form["glups"]
