@@ -44,6 +44,7 @@ struct lbm_handle {
     bool aa_ok;             // the configuration admits the in-place layout
     bool aa;                // in-place layout: F[0], G[0] only, states fpar / gpar
     int fpar, gpar;         // LAY_ODD / LAY_EVEN of F[0] and G[0] (in-place layout, phase 1)
+    bool capturing;         // the handle's stream is being captured into a HIP graph
     FusedGeom fg;
     int64_t itc;
     bool owns_stream;       // false for the domains of a batch (they share the batch's stream)
@@ -213,6 +214,8 @@ static int launch_fused_single(lbm_handle *h)
             par = 1;
     }
     if (par < 0) {
+        if (h->capturing)   // the rebuild below synchronises: not inside a capture
+            return fail(h, LBM_ERR_STATE, "one-pass launch table out of date during graph capture");
         if (!h->ftab && hipMalloc(&h->ftab, 2 * isz) != hipSuccess)
             return fail(h, LBM_ERR_NOMEM, "one-pass launch table");
         std::vector<unsigned char> host(2 * isz);
@@ -351,8 +354,9 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     }
     h->red_blocks = 1024;
     if (rc == LBM_OK) {
+        // volume partials (6 sums) then wall partials (4 sums per wall-profile block)
         const unsigned nw = (unsigned)(c.nx > c.ny ? c.nx : c.ny);   // wall-profile blocks
-        size_t nred = (size_t)2 * 8 * (h->red_blocks > nw ? h->red_blocks : nw);
+        size_t nred = (size_t)2 * 6 * h->red_blocks + (size_t)2 * 4 * nw;
         if (hipMalloc((void **)&h->red, nred * sizeof(double)) != hipSuccess ||
             hipMalloc((void **)&h->flag, sizeof(int)) != hipSuccess)
             rc = LBM_ERR_NOMEM;
@@ -981,27 +985,46 @@ static void fit_parabola5(const double *y, const double *f, int mode, double *fs
     *fs = A + B * yst + C * yst * yst;
 }
 
-extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
+// Device part of the diagnostics: the macro planes brought up to date, then the volume sums and
+// the wall sums and profiles into the partials buffer and h->wall (no host synchronisation; a
+// HIP graph may capture it).
+static int diag_device(lbm_handle *h)
 {
-    if (!h || !d) return LBM_ERR_ARG;
-    HIPCHK(h, hipSetDevice(h->cfg.device));
     int rc = refresh_macros(h);
     if (rc) return rc;
     const lbm_config &c = h->cfg;
     const double L = h->p.L, dy = 1.0 / L, coef = L / h->p.diffusivity;
-    const double dT = c.thot - c.tcold, N = (double)h->N;
     // Rayleigh-Benard cell: hot wall y = 0, wall profile along x.  Side-heated cell: hot wall
     // x = 0, wall profile along y (SideHeatedcalc_Nu_wall_avg, 2DRB/3DRB.F90:1889-2038).
     const int side = c.side_heated ? 1 : 0;
     const int nx = side ? c.ny : c.nx;   // length of the hot-wall profile
     const unsigned nb = h->red_blocks;
-    memset(d, 0, sizeof(*d));
-
     HIPCHK(h, launch_reduce_volume(h->kp, h->stream, h->mac, coef, dy, side, h->red, nb));
-    std::vector<double> part((size_t)nb * 6 * 2);
-    HIPCHK(h, hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, launch_reduce_walls(h->kp, h->stream, h->mac, coef, dy, side, h->wall, h->wall + nx,
+                                  h->red + (size_t)nb * 6 * 2));
+    return LBM_OK;
+}
+
+// Doubles of one diagnostics sample as diag_device leaves them: volume partials, wall
+// partials, the two wall profiles.
+static size_t diag_sample_doubles(const lbm_handle *h)
+{
+    const int nx = h->cfg.side_heated ? h->cfg.ny : h->cfg.nx;
+    return (size_t)h->red_blocks * 6 * 2 + (size_t)nx * 4 * 2 + (size_t)2 * nx;
+}
+
+// Host part: compensated sums of the partials and the wall fits of one sample laid out as
+// diag_sample_doubles describes.
+static void diag_finish(const lbm_handle *h, const double *sample, lbm_diag *d)
+{
+    const lbm_config &c = h->cfg;
+    const double L = h->p.L, dy = 1.0 / L;
+    const double dT = c.thot - c.tcold, N = (double)h->N;
+    const int side = c.side_heated ? 1 : 0;
+    const int nx = side ? c.ny : c.nx;
+    const unsigned nb = h->red_blocks;
+    memset(d, 0, sizeof(*d));
+    std::vector<double> part(sample, sample + (size_t)nb * 6 * 2);
     double s[6];
     combine(part, nb, 6, s);
     d->nu_vol = s[0] / N * L / h->p.diffusivity + 1.0;
@@ -1011,14 +1034,8 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
     d->heat = s[4];
     d->kinetic = s[5];
 
-    HIPCHK(h, launch_reduce_walls(h->kp, h->stream, h->mac, coef, dy, side, h->wall, h->wall + nx,
-                                  h->red));
-    std::vector<double> wpart((size_t)nx * 4 * 2), prof((size_t)2 * nx);
-    HIPCHK(h, hipMemcpyAsync(wpart.data(), h->red, wpart.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(prof.data(), h->wall, prof.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const double *wpart = sample + (size_t)nb * 6 * 2;
+    const double *prof = wpart + (size_t)nx * 4 * 2;
     HostKS hot, cold, mid;
     for (int i = 0; i < nx; ++i) {
         hot.add(prof[nx + i]);
@@ -1038,7 +1055,7 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
     xp[nx + 1] = (double)nx / L;
     for (int i = 1; i <= nx; ++i) xp[i] = ((double)i - 0.5) / L;
     for (int i = 0; i < nx; ++i) ext[i + 1] = prof[nx + i];
-    const double *tb = prof.data();
+    const double *tb = prof;
     ext[0] = (2.0 * (c.thot - fit_wall_T4(0.0, &xp[1], tb)) / dy) / dT;
     ext[nx + 1] = (2.0 * (c.thot - fit_wall_T4(xp[nx + 1], &xp[nx - 3], tb + nx - 4)) / dy) / dT;
     int imax = 0, imin = 0;
@@ -1055,6 +1072,106 @@ extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
         if (pass == 0) { d->nu_hot_max = fs; d->nu_hot_max_pos = ys; }
         else { d->nu_hot_min = fs; d->nu_hot_min_pos = ys; }
     }
+}
+
+extern "C" int lbm_diagnostics(lbm_handle *h, lbm_diag *d)
+{
+    if (!h || !d) return LBM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc = diag_device(h);
+    if (rc) return rc;
+    const int nx = h->cfg.side_heated ? h->cfg.ny : h->cfg.nx;
+    const size_t nred = (size_t)h->red_blocks * 6 * 2 + (size_t)nx * 4 * 2;
+    std::vector<double> sample(diag_sample_doubles(h));
+    HIPCHK(h, hipMemcpyAsync(sample.data(), h->red, nred * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(sample.data() + nred, h->wall, (size_t)2 * nx * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    diag_finish(h, sample.data(), d);
+    return LBM_OK;
+}
+
+// ---- runs with periodic diagnostics in a HIP graph ------------------------------------------
+// A device series of diagnostics samples (the partials and wall profiles of diag_device).
+struct DiagSeries {
+    double *buf = nullptr;
+    int *count = nullptr;
+    ~DiagSeries()
+    {
+        if (buf) (void)hipFree(buf);
+        if (count) (void)hipFree(count);
+    }
+};
+
+// Appends the diagnostics of domain h's current state to slot count[0] of series.
+static int diag_append(lbm_handle *h, double *series, int cap, int *count)
+{
+    int rc = diag_device(h);
+    if (rc) return rc;
+    const int nx = h->cfg.side_heated ? h->cfg.ny : h->cfg.nx;
+    const size_t nred = (size_t)h->red_blocks * 6 * 2 + (size_t)nx * 4 * 2;
+    HIPCHK(h, launch_series_append(h->stream, h->red, nred, h->wall, (size_t)2 * nx, series,
+                                   diag_sample_doubles(h), cap, count));
+    return LBM_OK;
+}
+
+// The steps of one period return every buffer to its place when the period is even (two-array
+// buffers, in-place states, the one-pass history buffers and the 2D force planes all alternate
+// from step to step).
+static int graph_periods(int every) { return every % 2 == 0 ? 1 : 2; }
+
+extern "C" int lbm_run_diagnostics(lbm_handle *h, int32_t nsteps, int32_t every, lbm_diag *out)
+{
+    if (!h || !out || every <= 0 || nsteps < every || nsteps % every) return LBM_ERR_ARG;
+    if (h->phase < 0) return fail(h, LBM_ERR_STATE, "lbm_run_diagnostics before initialisation");
+    if (h->in_batch) return fail(h, LBM_ERR_STATE, "batch domain: use lbm_batch_run_diagnostics");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const int ns = nsteps / every;
+    const size_t stride = diag_sample_doubles(h);
+    DiagSeries ser;
+    if (hipMalloc((void **)&ser.buf, (size_t)ns * stride * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&ser.count, sizeof(int)) != hipSuccess)
+        return fail(h, LBM_ERR_NOMEM, "diagnostics series");
+    HIPCHK(h, launch_series_reset(h->stream, ser.count, 1));
+    auto period = [&]() -> int {
+        int rc = enqueue_steps(h, every);
+        return rc ? rc : diag_append(h, ser.buf, ns, ser.count);
+    };
+    // the first period eagerly: it leaves the plain state and settles the buffer parities
+    int rc = period();
+    if (rc) return rc;
+    int done = 1;
+    const int P = graph_periods(every);
+    const int replays = (ns - done) / P;
+    if (replays > 0) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        const int64_t itc0 = h->itc;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        h->capturing = true;
+        for (int p = 0; p < P && !rc; ++p) rc = period();
+        h->capturing = false;
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        // host bookkeeping now stands one replay ahead; every further replay adds P periods
+        for (int r = 0; r < replays && !rc && e == hipSuccess; ++r) e = hipGraphLaunch(ge, h->stream);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(h, LBM_ERR_HIP, "diagnostics graph: %s", hipGetErrorString(e));
+        h->itc = itc0 + (int64_t)replays * P * every;
+        done += replays * P;
+    }
+    for (; done < ns; ++done) {
+        rc = period();
+        if (rc) return rc;
+    }
+    std::vector<double> host((size_t)ns * stride);
+    HIPCHK(h, hipMemcpyAsync(host.data(), ser.buf, host.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < ns; ++k) diag_finish(h, host.data() + (size_t)k * stride, &out[k]);
     return LBM_OK;
 }
 
@@ -1941,6 +2058,85 @@ static int batch_enqueue(lbm_batch *b, int32_t nsteps)
     for (; done < nsteps; ++done) {
         HIPCHK(h0, batch_launch(b, b->table_parity));
         batch_advance(b);
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_batch_run_diagnostics(lbm_batch *b, int32_t nsteps, int32_t every, lbm_diag *out)
+{
+    if (!b || !out || every <= 0 || nsteps < every || nsteps % every) return LBM_ERR_ARG;
+    lbm_handle *h0 = b->dom[0];
+    HIPCHK(h0, hipSetDevice(h0->cfg.device));
+    for (int i = 0; i < b->n; ++i)
+        if (b->dom[i]->phase < 0) return fail(b->dom[i], LBM_ERR_STATE, "batch domain %d not initialised", i);
+    const int ns = nsteps / every, n = b->n;
+    std::vector<size_t> off(n + 1, 0);
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (size_t)ns * diag_sample_doubles(b->dom[i]);
+    DiagSeries ser;
+    if (hipMalloc((void **)&ser.buf, off[n] * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&ser.count, (size_t)n * sizeof(int)) != hipSuccess)
+        return fail(h0, LBM_ERR_NOMEM, "diagnostics series");
+    HIPCHK(h0, launch_series_reset(b->stream, ser.count, n));
+    auto diag_all = [&]() -> int {
+        for (int i = 0; i < n; ++i) {
+            int rc = diag_append(b->dom[i], ser.buf + off[i], ns, ser.count + i);
+            if (rc) return rc;
+        }
+        return LBM_OK;
+    };
+    // the first period eagerly, in the batch's own mode (it leaves the plain states)
+    int rc = batch_enqueue(b, every);
+    if (!rc) rc = diag_all();
+    if (rc) return rc;
+    int done = 1;
+    const int P = graph_periods(every);
+    const int replays = (ns - done) / P;
+    const bool table = b->mode != LBM_BATCH_PER_DOMAIN;
+    if (replays > 0 && table && !b->tables_ready) rc = batch_tables(b);   // outside the capture
+    if (rc) return rc;
+    if (replays > 0) {
+        std::vector<int64_t> itc0(n);
+        for (int i = 0; i < n; ++i) itc0[i] = b->dom[i]->itc;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(h0, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < n; ++i) b->dom[i]->capturing = true;
+        for (int p = 0; p < P && !rc; ++p) {
+            for (int s = 0; s < every && !rc; ++s) {
+                if (table) {
+                    if (batch_launch(b, b->table_parity) != hipSuccess)
+                        rc = fail(h0, LBM_ERR_HIP, "batch launch during capture");
+                    batch_advance(b);
+                } else {
+                    for (int i = 0; i < n && !rc; ++i) rc = enqueue_steps(b->dom[i], 1);
+                }
+            }
+            if (!rc) rc = diag_all();
+        }
+        for (int i = 0; i < n; ++i) b->dom[i]->capturing = false;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        for (int r = 0; r < replays && !rc && e == hipSuccess; ++r) e = hipGraphLaunch(ge, b->stream);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(h0, LBM_ERR_HIP, "diagnostics graph: %s", hipGetErrorString(e));
+        for (int i = 0; i < n; ++i) b->dom[i]->itc = itc0[i] + (int64_t)replays * P * every;
+        done += replays * P;
+    }
+    for (; done < ns; ++done) {
+        rc = batch_enqueue(b, every);
+        if (!rc) rc = diag_all();
+        if (rc) return rc;
+    }
+    std::vector<double> host(off[n]);
+    HIPCHK(h0, hipMemcpyAsync(host.data(), ser.buf, host.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(h0, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; ++i) {
+        const size_t stride = diag_sample_doubles(b->dom[i]);
+        for (int k = 0; k < ns; ++k)
+            diag_finish(b->dom[i], host.data() + off[i] + (size_t)k * stride, &out[(size_t)k * n + i]);
     }
     return LBM_OK;
 }

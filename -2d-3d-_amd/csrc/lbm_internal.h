@@ -76,6 +76,12 @@ hipError_t launch_psi_absmax(hipStream_t s, int nx, int ny, int nfx, int nfy, do
                              double *bval, unsigned long long *bidx, unsigned blocks);
 hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *mac, double *prev,
                                double *out, unsigned blocks);
+// Diagnostics series: append n1 doubles of a and n2 of b at slot *count of `series` (slots of
+// `stride` doubles, `cap` of them; nothing is written past the last) and advance *count.  The
+// slot is read on the device, so a captured graph appends to the next slot at each replay.
+hipError_t launch_series_append(hipStream_t s, const double *a, size_t n1, const double *b,
+                                size_t n2, double *series, size_t stride, int cap, int *count);
+hipError_t launch_series_reset(hipStream_t s, int *count, int n);
 
 // Reduced-precision (FP32 / FP16 deviation storage) 3D kernels, lbm_kernels_lp.hip.
 enum {

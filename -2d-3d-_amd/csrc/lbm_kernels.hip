@@ -1325,6 +1325,45 @@ hipError_t launch_plane(const KParams &kp, hipStream_t s, const double *f, int a
     return hipGetLastError();
 }
 
+__global__ void k_series_append(const double *__restrict__ a, size_t n1, const double *__restrict__ b,
+                                size_t n2, double *__restrict__ series, size_t stride, int cap,
+                                const int *__restrict__ count)
+{
+    const int slot = *count;
+    if (slot < 0 || slot >= cap) return;
+    double *dst = series + (size_t)slot * stride;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n1 + n2;
+         t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = t < n1 ? a[t] : b[t - n1];
+}
+
+__global__ void k_series_advance(int *count)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count += 1;
+}
+
+__global__ void k_series_reset(int *count, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) count[t] = 0;
+}
+
+hipError_t launch_series_append(hipStream_t s, const double *a, size_t n1, const double *b,
+                                size_t n2, double *series, size_t stride, int cap, int *count)
+{
+    const size_t n = n1 + n2;
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+    k_series_append<<<blocks ? blocks : 1, 256, 0, s>>>(a, n1, b, n2, series, stride, cap, count);
+    k_series_advance<<<1, 64, 0, s>>>(count);
+    return hipGetLastError();
+}
+
+hipError_t launch_series_reset(hipStream_t s, int *count, int n)
+{
+    k_series_reset<<<(n + 255) / 256, 256, 0, s>>>(count, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce_check(const KParams &kp, hipStream_t s, const double *mac, double *prev,
                                double *out, unsigned blocks)
 {

@@ -8,6 +8,11 @@ timed for each kernel form (two_pass, one_pass) and stepping mode of the batch:
   per_domain   one launch per cavity per pass
   one_launch   one launch per pass for all cavities (per-cavity parameter/offset table)
   graph        one_launch, two steps captured in a HIP graph and replayed
+and, with the diagnostics of every cavity every --every steps (two-pass form, wall-clock time
+including the diagnostics' host work):
+  diag_eager   lbm_batch_step + lbm_diagnostics per cavity (one synchronisation per cavity)
+  diag_graph   lbm_batch_run_diagnostics (steps and reductions of a period replayed from a
+               HIP graph; one synchronisation per run)
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -16,6 +21,7 @@ import argparse
 import json
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -40,6 +46,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "fp32", "fp16"], default="fp32")
+    ap.add_argument("--every", type=int, default=10, help="steps between diagnostics samples")
     a = ap.parse_args()
     prec = {"fp64": rblbm.FP64, "fp32": rblbm.FP32_SHIFT, "fp16": rblbm.FP16_SHIFT}[a.precision]
     cfgs = ensemble(a.n, a.seed, prec)
@@ -61,6 +68,24 @@ def main():
             out["modes"][key] = {"ms_per_step": ms / a.steps,
                                  "glups": nodes * a.steps / (ms / 1e3) / 1e9}
             print(f"{key}: {out['modes'][key]}", file=sys.stderr, flush=True)
+    b.set_kernel_variant(rblbm.KERNEL_TWO_PASS)
+    b.set_mode(rblbm.BATCH_GRAPH)
+    b.step(a.warmup)
+    nsamp = max(1, a.steps // a.every)
+    b.domains[0].sync()
+    t0 = time.perf_counter()
+    for _ in range(nsamp):
+        b.step(a.every)
+        eager = [d.diagnostics() for d in b.domains]
+    t_eager = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    series = b.run_diagnostics(nsamp * a.every, a.every)
+    t_graph = time.perf_counter() - t0
+    for name, t in (("diag_eager", t_eager), ("diag_graph", t_graph)):
+        out["modes"][name] = {"ms_per_step": t * 1e3 / (nsamp * a.every),
+                              "glups": nodes * nsamp * a.every / t / 1e9, "every": a.every}
+        print(f"{name}: {out['modes'][name]}", file=sys.stderr, flush=True)
+    assert len(series) == nsamp and len(eager) == len(b.domains)
     d = b.domains[0].diagnostics()
     out["domain0_nu_vol"] = d["nu_vol"]
     out["graph_vs_per_domain"] = {

@@ -96,6 +96,12 @@ module lbm_rb
       type(c_ptr), value :: h
       type(lbm_diag), intent(out) :: d
     end function
+    integer(c_int) function lbm_run_diagnostics(h, nsteps, every, d) bind(C, name="lbm_run_diagnostics")
+      import :: c_int, c_int32_t, c_ptr, lbm_diag
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsteps, every
+      type(lbm_diag), intent(out) :: d(*)
+    end function
     integer(c_int) function lbm_midplane_extrema(h, m) bind(C, name="lbm_midplane_extrema")
       import :: c_int, c_ptr, lbm_midplane
       type(c_ptr), value :: h

@@ -34,6 +34,9 @@ program rb_main
   integer :: useG = 1
   integer :: legacyThermal = 0       ! EnableLegacyThermalScheme (FP64 form only)
   integer :: layout = 0              ! population layout: 0 two arrays, 1 in place (A-A), 2 auto
+  real(8) :: outputFrequency = 100.0d0   ! free-fall times between in-loop calNuRe samples
+  integer :: reloadDimensionlessTime = 0
+  integer :: nuReInLoop = 0          ! 1: the in-loop calNuRe call (commented out in the reference)
   integer :: precision = LBM_FP64
   integer :: device = 0
   integer :: seeded = 1
@@ -49,7 +52,8 @@ program rb_main
   namelist /rb/ dims, nx, ny, nz, Rayleigh, Prandtl, Mach, Thot, Tcold, itc_max, checkInterval, &
        epsU, epsT, sideHeated, velX, velZ, tempX, tempY, tempZ, useG, precision, device, &
        seeded, seed, noiseAmp, loadInitField, reloadFile, outputBinFile, backupAtEnd, settingsFile, &
-       outputPltFile, pltFolderPrefix, legacyThermal, layout
+       outputPltFile, pltFolderPrefix, legacyThermal, layout, outputFrequency, &
+       reloadDimensionlessTime, nuReInLoop
 
   type(lbm_config) :: cfg
   type(lbm_params) :: prm
@@ -58,6 +62,8 @@ program rb_main
   type(c_ptr) :: h = c_null_ptr
   integer :: itc, nchunk, qf, qt, ios
   real(8) :: errorU, errorT
+  integer :: dimensionlessTime = 0, cadence, target, ns, lead, k
+  type(lbm_diag), allocatable :: series(:)
   integer(8) :: t0, t1, rate
   real(8) :: seconds
   character(len=256) :: nmlFile
@@ -88,11 +94,40 @@ program rb_main
   errorT = 100.0d0
   itc = 0
   call system_clock(t0, rate)
+  cadence = max(1, int(outputFrequency * prm%time_unit))   ! 2DRB/3DRB.F90:249
   do while (((errorU > epsU) .or. (errorT > epsT)) .and. (itc <= itc_max))
-    ! advance to the next check point (or to the last step) in one call
+    ! advance to the next check point (or to the last step)
     nchunk = min(checkInterval - mod(itc, checkInterval), itc_max + 1 - itc)
-    call lbm_require(lbm_step(h, int(nchunk, c_int32_t)), 'lbm_step', h, 6)
-    itc = itc + nchunk
+    target = itc + nchunk
+    if (nuReInLoop == 1) then
+      ! calNuRe at every multiple of the cadence: the first sample point eagerly if it is not a
+      ! whole cadence away, the following ones from lbm_run_diagnostics (graph replays)
+      ns = target / cadence - itc / cadence
+      if (ns > 0) then
+        lead = cadence - mod(itc, cadence)
+        if (lead < cadence) then
+          call lbm_require(lbm_step(h, int(lead, c_int32_t)), 'lbm_step', h, 6)
+          itc = itc + lead
+          call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
+          call calNuRe(dg)
+          ns = ns - 1
+        endif
+        if (ns > 0) then
+          allocate(series(ns))
+          call lbm_require(lbm_run_diagnostics(h, int(ns * cadence, c_int32_t), &
+               int(cadence, c_int32_t), series), 'lbm_run_diagnostics', h, 6)
+          itc = itc + ns * cadence
+          do k = 1, ns
+            call calNuRe(series(k))
+          enddo
+          deallocate(series)
+        endif
+      endif
+      if (target > itc) call lbm_require(lbm_step(h, int(target - itc, c_int32_t)), 'lbm_step', h, 6)
+    else
+      call lbm_require(lbm_step(h, int(nchunk, c_int32_t)), 'lbm_step', h, 6)
+    endif
+    itc = target
     if (mod(itc, checkInterval) == 0) call check()
   enddo
   call lbm_require(lbm_sync(h), 'lbm_sync', h, 6)
@@ -100,6 +135,7 @@ program rb_main
   seconds = dble(t1 - t0) / dble(rate)
 
   call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
+  call calNuRe(dg)                                     ! the end-of-run call, 2DRB/3DRB.F90:297
   call lbm_require(lbm_midplane_extrema(h, mp), 'lbm_midplane_extrema', h, 6)
   if (outputBinFile == 1) call output_binary()
   if (outputPltFile == 1) call output_Tecplot()
@@ -222,6 +258,26 @@ contains
     call lbm_require(lbm_check(h, errorU, errorT), 'lbm_check', h, 6)
     write(*,'(I12,1X,ES24.16,1X,ES24.16)') itc, errorU, errorT
   end subroutine check
+
+  !------------------------------------------------------------------------------------------
+  ! calNuRe (2DRB/3DRB.F90:1712-1778; 2DRB/2DRB.F90:1610-1675): one sample of the volume
+  ! Nusselt and Reynolds time series, appended at time reloadDimensionlessTime +
+  ! dimensionlessTime * outputFrequency
+  !------------------------------------------------------------------------------------------
+  subroutine calNuRe(d)
+    type(lbm_diag), intent(in) :: d
+    real(8) :: tnow
+    dimensionlessTime = dimensionlessTime + 1
+    tnow = real(reloadDimensionlessTime + dimensionlessTime * outputFrequency, kind=8)
+    open(unit=01, file=trim(merge('Nu_VolAvg_3D.dat', 'Nu_VolAvg.dat   ', dims == 3)), &
+         status='unknown', position='append')
+    write(01,*) tnow, d%nu_vol
+    close(01)
+    open(unit=02, file=trim(merge('Re_VolAvg_3D.dat', 'Re_VolAvg.dat   ', dims == 3)), &
+         status='unknown', position='append')
+    write(02,*) tnow, d%re_vol
+    close(02)
+  end subroutine calNuRe
 
   !------------------------------------------------------------------------------------------
   ! output_binary (2DRB/3DRB.F90:1599-1627): u, v, w scaled by L/kappa, then T, rho

@@ -277,6 +277,16 @@ class Solver:
         self._chk(self.lib.lbm_diagnostics(self._h, C.byref(d)))
         return {n: getattr(d, n) for n, _ in _Diag._fields_}
 
+    def run_diagnostics(self, nsteps: int, every: int) -> list:
+        """nsteps steps with the diagnostics every `every` steps, the periods replayed from a
+        HIP graph (lbm_run_diagnostics); one dict per sample."""
+        L = self.lib
+        L.lbm_run_diagnostics.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(_Diag)]
+        ns = nsteps // every if every > 0 else 0
+        out = (_Diag * max(ns, 1))()
+        self._chk(L.lbm_run_diagnostics(self._h, int(nsteps), int(every), out))
+        return [{n: getattr(d, n) for n, _ in _Diag._fields_} for d in out[:ns]]
+
     def midplane_slice(self, axis: int) -> dict:
         """3D mid-plane slice across axis 0/1/2 (lbm_midplane_slice): u, v, w scaled by
         L / kappa, T, stream function and vorticity, each as (n2, n1) arrays."""
@@ -403,6 +413,16 @@ class Batch:
         L = self.lib
         L.lbm_batch_set_kernel_variant.argtypes = [C.c_void_p, C.c_int32]
         self._chk(L.lbm_batch_set_kernel_variant(self._b, int(variant)))
+
+    def run_diagnostics(self, nsteps: int, every: int) -> list:
+        """lbm_batch_run_diagnostics: result[k][i] is sample k of domain i."""
+        L = self.lib
+        L.lbm_batch_run_diagnostics.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(_Diag)]
+        ns, n = (nsteps // every if every > 0 else 0), len(self.domains)
+        out = (_Diag * max(ns * n, 1))()
+        self._chk(L.lbm_batch_run_diagnostics(self._b, int(nsteps), int(every), out))
+        fields = [f for f, _ in _Diag._fields_]
+        return [[{f: getattr(out[k * n + i], f) for f in fields} for i in range(n)] for k in range(ns)]
 
     def autotune(self, cache: str | None = None, steps: int = 10):
         L = self.lib

@@ -139,6 +139,14 @@ int lbm_check(lbm_handle *h, double *error_u, double *error_t);
 
 /* Nusselt / Reynolds / conservation diagnostics of the current state. */
 int lbm_diagnostics(lbm_handle *h, lbm_diag *d);
+/* Runs nsteps steps with the diagnostics every `every` steps (nsteps a multiple of every) into
+ * out[nsteps / every] (no reference counterpart: the time series that calNuRe appends to,
+ * 2DRB/3DRB.F90:1712-1778, at the reference's check cadence or any other).  The first period
+ * runs eagerly; then one or two periods of steps and the diagnostics' device reductions are
+ * captured in a HIP graph and replayed, each replay appending its partial sums to a device
+ * series; the host finishes the compensated sums after the run.  Results equal lbm_step +
+ * lbm_diagnostics at the same steps. */
+int lbm_run_diagnostics(lbm_handle *h, int32_t nsteps, int32_t every, lbm_diag *out);
 /* Mid-plane velocity maxima of the current state.  3D: calc_umid_max_common,
  * calc_vmid_max_common, calc_wmid_max_common (2DRB/3DRB.F90:2399-2518; the field interpolated
  * onto the plane at half the domain extent, the first grid maximum).  2D: RBcalc_umid_max /
@@ -231,6 +239,10 @@ int lbm_batch_set_kernel_variant(lbm_batch *b, int32_t variant);
 int lbm_batch_step(lbm_batch *b, int32_t nsteps);
 int lbm_batch_step_timed(lbm_batch *b, int32_t nsteps, double *ms);
 int lbm_batch_nodes(lbm_batch *b, int64_t *nodes);
+/* lbm_run_diagnostics for a batch: out[k * n + i] is sample k of domain i.  The captured
+ * period holds the batch's steps in its mode (table launches, or per-domain launches in
+ * LBM_BATCH_PER_DOMAIN) and every domain's diagnostics reductions. */
+int lbm_batch_run_diagnostics(lbm_batch *b, int32_t nsteps, int32_t every, lbm_diag *out);
 /* lbm_autotune for each distinct domain shape (the one-launch modes use the tuned z-chunks). */
 int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t steps);
 

@@ -171,3 +171,37 @@ def test_driver_tecplot_files_match_oracle(tmp_path):
         assert names == [c1, c2, "PSI_" + c1 + c2, "VORT_" + a]
         np.testing.assert_allclose(vals[:, 2], sl["psi"].reshape(-1), rtol=1e-12, atol=1e-14)
         np.testing.assert_allclose(vals[:, 3], sl["vort"].reshape(-1), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_driver_calnure_series(tmp_path, dims):
+    # nuReInLoop=1: calNuRe every int(outputFrequency * timeUnit) steps (the reference's
+    # commented-out in-loop call, 2DRB/3DRB.F90:249-250), served by lbm_run_diagnostics after
+    # the first sample, plus the end-of-run call (:297); time column as calNuRe writes it
+    nx, ny, nz = (16, 12, 10) if dims == 3 else (40, 21, 1)
+    extra = "" if dims == 3 else ", velX=1, tempX=2, Thot=1.0d0, Tcold=0.0d0"
+    kw = dict(dims=dims, nx=nx, ny=ny, nz=nz, rayleigh=1e5)
+    if dims == 2:
+        kw.update(vel_x=1, temp_x=2, thot=1.0, tcold=0.0)
+    cfg = orc.Config(**kw)
+    cadence = int(0.1 * orc.params(cfg)["time_unit"])
+    itc_max = 5 * cadence + 7          # six samples in the loop span, then the final call
+    _run(tmp_path, f"dims={dims}, nx={nx}, ny={ny}, nz={nz}, Rayleigh=1.0d5, itc_max={itc_max}, "
+                   f"checkInterval=100000, seed=9, nuReInLoop=1, outputFrequency=0.1d0, "
+                   f"reloadDimensionlessTime=3{extra}")
+    suffix = "_3D" if dims == 3 else ""
+    nu = np.loadtxt(tmp_path / f"Nu_VolAvg{suffix}.dat", ndmin=2)
+    re = np.loadtxt(tmp_path / f"Re_VolAvg{suffix}.dat", ndmin=2)
+    steps = itc_max + 1
+    sample_at = [cadence * k for k in range(1, steps // cadence + 1)] + [steps]
+    assert nu.shape == (len(sample_at), 2) and re.shape == nu.shape
+    np.testing.assert_allclose(nu[:, 0], 3 + 0.1 * np.arange(1, len(sample_at) + 1))
+    o = orc.Oracle(cfg)
+    o.init_seeded(9, 1e-3)
+    done = 0
+    for k, n in enumerate(sample_at):
+        o.step(n - done)
+        done = n
+        d = o.diagnostics()
+        assert nu[k, 1] == pytest.approx(d["nu_vol"], rel=1e-12, abs=1e-14), (k, n)
+        assert re[k, 1] == pytest.approx(d["re_vol"], rel=1e-12, abs=1e-14), (k, n)
