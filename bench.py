@@ -58,6 +58,11 @@ def parse():
                          "(--variant, --kchunk, --tile, --prefetch are then ignored); aa: one "
                          "array per lattice updated in place (two-pass form only)")
     ap.add_argument("--tune-steps", type=int, default=8, help="timed steps per autotune candidate")
+    ap.add_argument("--full-config", choices=["c1", "c2"], default="",
+                    help="instead of the benchmark: the whole 2D 257x129 Ra=1e5 run (20,000 "
+                         "steps) or the whole 128^3 Ra=1e6 FP64 run (5,000 steps), timed on the "
+                         "GPU and, as the CPU baseline, the FP64 OpenMP oracle over the same run; "
+                         "reports both and the largest difference of f, g, u, T and Nu")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run a device-to-device copy of GIB GiB (counter calibration)")
     return ap.parse_args()
@@ -104,6 +109,51 @@ def cpu_baseline(size, nz, ra, steps):
     return {"value": n * steps / dt / 1e9, "unit": "GLUPS", "cores": cores, "kind": "port",
             "sample": f"{size}x{size}x{nz} slab of the {size}^3 Ra={ra:g} cavity, FP64 OpenMP "
                       f"oracle, {steps} timed steps after 1 warm-up step ({dt:.1f} s)"}
+
+
+def full_config(name):
+    """One whole benchmark configuration on the GPU and, as the CPU baseline, on the oracle."""
+    import numpy as np
+
+    import rblbm
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc  # the checker / baseline only; never the measured GPU path
+    if name == "c1":
+        kw = dict(dims=2, nx=257, ny=129, nz=1, rayleigh=1e5, prandtl=0.71, thot=1.0, tcold=0.0,
+                  vel_x=rblbm.VEL_PERIODIC, temp_x=rblbm.T_PERIODIC)
+        steps, label = 20000, "2D RB 257x129 Ra=1e5 Pr=0.71, periodic x, FP64, 20,000 steps"
+    else:
+        kw = dict(dims=3, nx=128, ny=128, nz=128, rayleigh=1e6, prandtl=0.71)
+        steps, label = 5000, "3D RB cavity 128^3 Ra=1e6 Pr=0.71, FP64, 5,000 steps"
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = orc.set_threads(int(os.environ.get("OMP_NUM_THREADS", cores)))
+    sol = rblbm.Solver(rblbm.Config(**kw))
+    sol.initial_seeded(SEED, 1e-3)
+    sol.step_timed(0)                       # the prologue outside the timing
+    ms = sol.step_timed(steps)
+    m, st, d = sol.macros(), sol.gather(with_bprev=False), sol.diagnostics()
+    sol.close()
+    o = orc.Oracle(orc.Config(**kw))
+    o.init_seeded(SEED, 1e-3)
+    t0 = time.perf_counter()
+    o.step(steps)
+    cpu_s = time.perf_counter() - t0
+    s, e = o.state(), o.diagnostics()
+    o.close()
+    n = kw["nx"] * kw["ny"] * kw["nz"]
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    return {"config": label, "nodes": n, "steps": steps,
+            "gpu": {"seconds": ms / 1e3, "glups": n * steps / (ms / 1e3) / 1e9},
+            "cpu_baseline": {"seconds": cpu_s, "glups": n * steps / cpu_s / 1e9, "cores": cores,
+                             "kind": "port"},
+            "max_rel_diff": {"f": rel(st["f"], s["f"]), "g": rel(st["g"], s["g"]),
+                             "u": rel(m["u"], s["u"]), "T": rel(m["T"], s["T"]),
+                             "nu_vol": abs(d["nu_vol"] - e["nu_vol"]) / abs(e["nu_vol"]),
+                             "re_vol": abs(d["re_vol"] - e["re_vol"]) / abs(e["re_vol"])},
+            "nu_vol": d["nu_vol"], "re_vol": d["re_vol"], "nu_global": d["nu_global"]}
 
 
 def copy_gbs(torch, gib=2.0, reps=5):
@@ -177,6 +227,9 @@ def main():
     a = parse()
     if a.copy_probe > 0:
         copy_probe(a.copy_probe)
+        return
+    if a.full_config:
+        print(json.dumps(full_config(a.full_config)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); the solver "

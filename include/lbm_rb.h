@@ -184,8 +184,9 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
  * TWO_PASS: a thermal and a flow launch per step.  ONE_PASS: one launch per step (3D without
  * periodic thermal walls only); it moves fewer bytes but recomputes a ring of nodes per tile.
  * AUTO (the default) runs TWO_PASS, the faster form at 512^3 on MI355X; lbm_autotune measures
- * both per shape.  Both give bitwise identical results in the FP64 form.  kchunk > 0 sets the
- * number of z-planes per block of the one-pass form (0 keeps the current value). */
+ * both per shape.  Both give bitwise identical results in every precision form.  ONE_PASS needs
+ * the two-array layout.  kchunk > 0 sets the number of z-planes per block of the one-pass form
+ * (0 keeps the current value). */
 #define LBM_KERNEL_AUTO 0
 #define LBM_KERNEL_TWO_PASS 1
 #define LBM_KERNEL_ONE_PASS 2
@@ -196,7 +197,7 @@ int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
  * FP32 / FP16 forms only, else LBM_ERR_UNSUPPORTED); prefetch = 1 issues the next plane's loads
  * before the current flow collision.  Results do not depend on the choice.
  * Batch domains reject this and lbm_set_kernel_variant (LBM_ERR_STATE): a batch is tuned with
- * lbm_batch_autotune. */
+ * lbm_batch_autotune or lbm_batch_set_kernel_variant. */
 #define LBM_TILE_32X16 0
 #define LBM_TILE_16X16 1
 #define LBM_TILE_32X8 2
