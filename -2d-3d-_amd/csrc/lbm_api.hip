@@ -148,6 +148,20 @@ static void release(lbm_handle *h)
 
 static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **out);
 
+// Plane pitch (elements) of the plane-structured arrays.  A step streams up to 40 planes at
+// once; with a pitch of N nodes at 512^3 every plane is exactly 2^30 bytes from the next, all
+// streams start at the same offset modulo every power of two, and the step's speed then
+// depended on where the allocation landed (17.9 or 16.2 ms for the same FP64 kernels in two
+// allocations of one process).  The pitch adds 1 KiB plus up to 1 MiB (a sixteenth of a plane
+// for small planes) of population elements of size es; with 1 MiB + 1 KiB the FP64 step ran
+// in 15.6-15.8 ms in every allocation measured.
+static uint64_t plane_pitch(uint64_t N, int es)
+{
+    const uint64_t plane = N * (uint64_t)es;
+    uint64_t pad = std::min<uint64_t>(plane / 16, (uint64_t)1 << 20) / 1024 * 1024 + 1024;
+    return N + (pad + (uint64_t)es - 1) / (uint64_t)es;
+}
+
 // Device bytes per node of a domain: populations (one or two arrays per lattice), B_prev (and
 // the one-pass form's second copy), aux, the five FP64 macro planes.
 static double node_bytes(const lbm_config &c, bool aa, bool fused)
@@ -280,6 +294,7 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     k.temp_x = c.temp_x; k.temp_y = c.temp_y; k.temp_z = c.dims == 3 ? c.temp_z : 1;
     k.use_g = c.use_g ? 1 : 0;
     k.N = h->N;
+    k.P = plane_pitch(h->N, c.precision == LBM_FP64 ? 8 : (c.precision == LBM_FP32_SHIFT ? 4 : 2));
     k.gbeta = h->p.gbeta; k.tref = h->p.tref; k.thot = c.thot; k.tcold = c.tcold;
     k.sv = 1.0 / h->p.tauf; k.sq = h->p.s_q;
     k.qk = h->p.q_k; k.qnu = h->p.q_nu; k.sg = h->p.sg;
@@ -333,7 +348,7 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
         free(h);
         return LBM_ERR_HIP;
     }
-    const size_t N = h->N;
+    const size_t N = h->kp.P;   // plane pitch
     h->lp = c.precision != LBM_FP64;
     h->es = c.precision == LBM_FP64 ? 8 : (c.precision == LBM_FP32_SHIFT ? 4 : 2);
     const size_t ea = h->lp ? sizeof(float) : sizeof(double);   // B_prev / aux element
@@ -398,7 +413,7 @@ static int do_init(lbm_handle *h, const double *T0, uint64_t seed, double amp, b
     ip.seed = seed;
     ip.instance = h->instance;
     ip.noise_scale = seeded ? amp * (c.thot - c.tcold) : 0.0;
-    if (T0) HIPCHK(h, hipMemcpyAsync(h->mac + 4 * h->N, T0, h->N * sizeof(double),
+    if (T0) HIPCHK(h, hipMemcpyAsync(h->mac + 4 * h->kp.P, T0, h->N * sizeof(double),
                                      hipMemcpyHostToDevice, h->stream));
     if (!h->lp) {
         HIPCHK(h, launch_init(h->kp, h->bx, h->stream, ip, h->F[0], h->G[0], h->Bp, h->mac, h->qf, h->qt));
@@ -412,7 +427,7 @@ static int do_init(lbm_handle *h, const double *T0, uint64_t seed, double amp, b
         a.mac = h->mac;
         HIPCHK(h, launch_lp(h->cfg.precision, LP_INIT, h->kp, h->bx, h->stream, a));
     }
-    if (h->prev) HIPCHK(h, hipMemsetAsync(h->prev, 0, 4 * h->N * sizeof(double), h->stream));
+    if (h->prev) HIPCHK(h, hipMemsetAsync(h->prev, 0, 4 * h->kp.P * sizeof(double), h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->phase = 0;
     h->itc = 0;
@@ -685,7 +700,7 @@ static int convert_layout(lbm_handle *h, bool aa)
 {
     if (aa == h->aa) return LBM_OK;
     if (h->phase > 0 || !h->aa_ok) return fail(h, LBM_ERR_STATE, "layout change after stepping");
-    const size_t N = h->N, ea = h->lp ? sizeof(float) : sizeof(double);
+    const size_t N = h->kp.P, ea = h->lp ? sizeof(float) : sizeof(double);   // plane pitch
     const size_t fb = N * h->qf * h->es, gb = N * h->qt * h->es;
     if (aa) {
         (void)hipStreamSynchronize(h->stream);
@@ -873,7 +888,7 @@ extern "C" int lbm_macros(lbm_handle *h, double *rho, double *u, double *v, doub
     double *dst[5] = {rho, u, v, w, T};
     for (int q = 0; q < 5; ++q)
         if (dst[q])
-            HIPCHK(h, hipMemcpyAsync(dst[q], h->mac + q * h->N, h->N * sizeof(double),
+            HIPCHK(h, hipMemcpyAsync(dst[q], h->mac + q * h->kp.P, h->N * sizeof(double),
                                      hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return LBM_OK;
@@ -909,9 +924,9 @@ extern "C" int lbm_check(lbm_handle *h, double *error_u, double *error_t)
     if (!h || !error_u || !error_t) return LBM_ERR_ARG;
     HIPCHK(h, hipSetDevice(h->cfg.device));
     if (!h->prev) {
-        if (hipMalloc((void **)&h->prev, 4 * h->N * sizeof(double)) != hipSuccess)
+        if (hipMalloc((void **)&h->prev, 4 * h->kp.P * sizeof(double)) != hipSuccess)
             return fail(h, LBM_ERR_NOMEM, "prev arrays");
-        HIPCHK(h, hipMemsetAsync(h->prev, 0, 4 * h->N * sizeof(double), h->stream));
+        HIPCHK(h, hipMemsetAsync(h->prev, 0, 4 * h->kp.P * sizeof(double), h->stream));
     }
     int rc = refresh_macros(h);
     if (rc) return rc;
@@ -1225,7 +1240,7 @@ extern "C" int lbm_midplane_extrema(lbm_handle *h, lbm_midplane *m)
     std::vector<double> plane(np);
     // one plane of component `comp` (1 u, 2 v, 3 w) across `axis` -> host vector
     auto extract = [&](int comp, int axis, int l, int r, double w, int mode, size_t cnt) -> int {
-        hipError_t e = launch_plane(h->kp, h->stream, h->mac + (size_t)comp * h->N, axis, l - 1,
+        hipError_t e = launch_plane(h->kp, h->stream, h->mac + (size_t)comp * h->kp.P, axis, l - 1,
                                     r - 1, w, mode, dplane);
         if (e == hipSuccess)
             e = hipMemcpyAsync(plane.data(), dplane, cnt * sizeof(double), hipMemcpyDeviceToHost,
@@ -1352,7 +1367,7 @@ extern "C" int lbm_midplane_slice(lbm_handle *h, int32_t axis, double *us, doubl
         return fail(h, LBM_ERR_NOMEM, "mid-plane buffer");
     for (int comp = 1; comp <= 4 && !rc; ++comp) {
         slice[comp].resize(cnt);
-        hipError_t e = launch_plane(h->kp, h->stream, h->mac + (size_t)comp * h->N, axis, iL - 1,
+        hipError_t e = launch_plane(h->kp, h->stream, h->mac + (size_t)comp * h->kp.P, axis, iL - 1,
                                     iR - 1, wgt, 0, dplane);
         if (e == hipSuccess)
             e = hipMemcpyAsync(slice[comp].data(), dplane, cnt * sizeof(double),
@@ -1460,8 +1475,8 @@ extern "C" int lbm_psi_vorticity_2d(lbm_handle *h, double *psi_out, double *vort
     const int nx = c.nx, ny = c.ny;
     const size_t N = (size_t)nx * ny;
     std::vector<double> uu(N), vv(N), psi(N);
-    HIPCHK(h, hipMemcpyAsync(uu.data(), h->mac + N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(vv.data(), h->mac + 2 * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(uu.data(), h->mac + h->kp.P, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(vv.data(), h->mac + 2 * h->kp.P, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     const double L = h->p.L, dx = 1.0 / L, dy = 1.0 / L, coef = L / h->p.diffusivity;
     auto U = [&](int i, int j) { return uu[(i - 1) + (size_t)nx * (j - 1)]; };
@@ -1649,8 +1664,8 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
                 e = launch_lp(h->cfg.precision, LP_STATE, h->kp, h->bx, h->stream, a);
             }
         } else if (plain) {
-            e = launch_transpose(false, h->stream, h->F[0], fo, h->qf, h->N, c0, cnt);
-            if (e == hipSuccess) e = launch_transpose(false, h->stream, h->G[0], go, h->qt, h->N, c0, cnt);
+            e = launch_transpose(false, h->stream, h->F[0], fo, h->qf, h->kp.P, c0, cnt);
+            if (e == hipSuccess) e = launch_transpose(false, h->stream, h->G[0], go, h->qt, h->kp.P, c0, cnt);
         } else if (h->aa) {
             e = launch_state(h->kp, h->bx, h->stream, true, h->F[0], h->G[0], h->aux[0], nullptr,
                              nullptr, fo, go, k0, n, h->fpar);
@@ -1669,17 +1684,22 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
     if (stage) (void)hipFree(stage);
     if (rc) return rc;
     if (bprev && !h->lp) {
-        HIPCHK(h, hipMemcpyAsync(bprev, h->Bp, 3 * h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        for (int q = 0; q < 3; ++q)   // planes at the pitch, packed on the host
+            HIPCHK(h, hipMemcpyAsync(bprev + q * h->N, h->Bp + q * h->kp.P, h->N * sizeof(double),
+                                     hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
     } else if (bprev) {
         double *tmp = nullptr;
         if (hipMalloc((void **)&tmp, 3 * h->N * sizeof(double)) != hipSuccess)
             return fail(h, LBM_ERR_NOMEM, "gather B_prev staging");
-        LPArgs a;
-        a.Bp = (float *)h->Bp;
-        a.fo = tmp;
-        a.n = 3 * h->N;
-        hipError_t e = launch_lp(h->cfg.precision, LP_BPREV, h->kp, h->bx, h->stream, a);
+        hipError_t e = hipSuccess;
+        for (int q = 0; q < 3 && e == hipSuccess; ++q) {
+            LPArgs a;
+            a.Bp = (float *)h->Bp + q * h->kp.P;
+            a.fo = tmp + q * h->N;
+            a.n = h->N;
+            e = launch_lp(h->cfg.precision, LP_BPREV, h->kp, h->bx, h->stream, a);
+        }
         if (e == hipSuccess)
             e = hipMemcpyAsync(bprev, tmp, 3 * h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1719,8 +1739,8 @@ extern "C" int lbm_scatter(lbm_handle *h, const double *f, const double *g)
             if (e == hipSuccess) e = launch_lp(h->cfg.precision, LP_FROM_ABS, h->kp, h->bx, h->stream, a);
             if (e == hipSuccess) e = launch_lp(h->cfg.precision, LP_RECON_AOS, h->kp, h->bx, h->stream, a);
         } else {
-            if (e == hipSuccess) e = launch_transpose(true, h->stream, fo, h->F[0], h->qf, h->N, c0, cnt);
-            if (e == hipSuccess) e = launch_transpose(true, h->stream, go, h->G[0], h->qt, h->N, c0, cnt);
+            if (e == hipSuccess) e = launch_transpose(true, h->stream, fo, h->F[0], h->qf, h->kp.P, c0, cnt);
+            if (e == hipSuccess) e = launch_transpose(true, h->stream, go, h->G[0], h->qt, h->kp.P, c0, cnt);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) rc = fail(h, LBM_ERR_HIP, "scatter: %s", hipGetErrorString(e));

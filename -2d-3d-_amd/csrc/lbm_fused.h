@@ -64,7 +64,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     const bool tile = in_dom && !xring && !yring;
     const int k0 = (int)chunk * fg.kchunk;
     const int k1 = min(nz, k0 + fg.kchunk);
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const uint64_t col = (uint64_t)i + (uint64_t)nx * (uint64_t)j;   // valid when in_dom
     const uint64_t plane = (uint64_t)nx * (uint64_t)ny;
     const typename Form::Node nd = Form::node(kp, fp, i, j);          // per-column constants

@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(256) k3_prologue(KParams kp, const double *__r
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const double *src = AA ? Fout : Fin;
     double f[19];
 #pragma unroll
@@ -302,7 +302,7 @@ __device__ __forceinline__ void thermal3_node(const KParams &kp, int i, int j, i
                                               double *__restrict__ aux,
                                               const double *__restrict__ mac)
 {
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
     const double *G = LAY == LAY_PULL ? Gin : Gout;
     double g[7], Tn;
@@ -361,7 +361,7 @@ __device__ __forceinline__ void flow3_node(const KParams &kp, int i, int j, int 
                                            const double *__restrict__ Gpost,
                                            const double *__restrict__ aux)
 {
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
     double g[7];
     in_heat<LAY>(Gpost, N, i, j, k, c, border, kp, g);
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(256) k3_state(KParams kp, const double *__rest
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
     double g[7], f[19];
     in_heat<LAY>(Gpost, N, i, j, k, c, border, kp, g);
@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(256) k2_prologue(KParams kp, const double *__r
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     double f[9];
 #pragma unroll
     for (int a = 0; a < 9; ++a) f[a] = Fin[a * N + c];
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(256) k2_thermal(KParams kp, const double *__re
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
     double g[5], Tn;
     if (FIRST) {
@@ -584,7 +584,7 @@ __global__ void __launch_bounds__(256) k2_flow(KParams kp, const double *__restr
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
     double g[5];
     pull_heat<D2Q5>(Gpost, N, i, j, k, c, border, kp, g);
@@ -608,7 +608,7 @@ __global__ void __launch_bounds__(256) k2_state(KParams kp, const double *__rest
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
     double g[5], f[9];
     pull_heat<D2Q5>(Gpost, N, i, j, k, c, border, kp, g);
@@ -650,7 +650,7 @@ __global__ void __launch_bounds__(256) k_init(KParams kp, InitParams ip, double 
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     double T;
     if (ip.use_T0) {
         T = mac[4 * N + c];
@@ -714,8 +714,8 @@ __global__ void k_reconstruct(KParams kp, const double *__restrict__ F, const do
                               int *bad)
 {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t N = kp.N;
-    if (c >= N) return;
+    const uint64_t N = kp.P;   // plane pitch
+    if (c >= kp.N) return;
     double T = 0.0;
     for (int a = 0; a < qt; ++a) T += G[a * N + c];
     double f[19];
@@ -820,10 +820,10 @@ __global__ void __launch_bounds__(256) k_reduce_volume(KParams kp, const double 
                                                        double L_over_kappa, double dy, int side,
                                                        double *__restrict__ out)
 {
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const int nx = kp.nx, ny = kp.ny;
     KS acc[6] = {};
-    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < N;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < kp.N;
          c += (uint64_t)gridDim.x * blockDim.x) {
         const double rho = mac[c], u = mac[N + c], v = mac[2 * N + c], w = mac[3 * N + c];
         const double T = mac[4 * N + c];
@@ -865,7 +865,7 @@ __global__ void __launch_bounds__(256) k_reduce_walls(KParams kp, const double *
                                                       double *__restrict__ out)
 {
     const int i = blockIdx.x;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const int nx = kp.nx, ny = kp.ny, nz = kp.nz;
     const double *T = mac + 4 * N, *v = mac + 2 * N, *u = mac + N;
     const double dT = kp.thot - kp.tcold;
@@ -1076,9 +1076,9 @@ __global__ void __launch_bounds__(256) k_reduce_check(KParams kp, const double *
                                                       double *__restrict__ prev,
                                                       double *__restrict__ out)
 {
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     KS acc[4] = {};
-    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < N;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < kp.N;
          c += (uint64_t)gridDim.x * blockDim.x) {
         const double u = mac[N + c], v = mac[2 * N + c], w = mac[3 * N + c], T = mac[4 * N + c];
         const double du = u - prev[c], dv = v - prev[N + c], dw = w - prev[2 * N + c];

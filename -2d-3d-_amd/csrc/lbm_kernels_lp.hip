@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(256) lp3_prologue(KParams kp, LPParams lp, con
     int i, j, k;
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     float f[19];
     load_all<D3Q19, S>(AA ? Fout : Fin, N, c, f, SEQ(D3Q19));
     const float drho = (float)(mac[c] - 1.0);
@@ -444,7 +444,7 @@ __device__ __forceinline__ void lp3_thermal_node(const KParams &kp, const LPPara
                                                  float *__restrict__ Bp, float *__restrict__ aux,
                                                  const double *__restrict__ mac)
 {
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border_lp(i, j, k, kp);
     const float t0 = t0_of(lp, i, j), b0 = (float)((double)t0 - lp.tref);
     const S *G = LAY == LAY_PULL ? Gin : Gout;
@@ -497,7 +497,7 @@ __device__ __forceinline__ void lp3_flow_node(const KParams &kp, const LPParams 
                                               S *__restrict__ Fnext, const S *__restrict__ Gpost,
                                               const float *__restrict__ aux)
 {
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border_lp(i, j, k, kp);
     float g[7];
     in_heat_lp<D3Q7, S, LAY>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
@@ -590,7 +590,7 @@ __global__ void __launch_bounds__(256) lp3_state(KParams kp, LPParams lp, const 
     int i, j, k;
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border_lp(i, j, k, kp);
     float g[7], f[19];
     in_heat_lp<D3Q7, S, LAY>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
@@ -643,7 +643,7 @@ __global__ void __launch_bounds__(256) lp_init(KParams kp, LPParams lp, S *__res
     int i, j, k;
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
-    const uint64_t N = kp.N;
+    const uint64_t N = kp.P;   // plane pitch
     float f[19] = {}, g[7];
     const double T = mac[4 * N + c], t0 = (double)t0_of(lp, i, j);
 #pragma unroll
@@ -662,7 +662,7 @@ __global__ void lp_from_abs(KParams kp, LPParams lp, const double *__restrict__ 
 {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const uint64_t c = c0 + r, N = kp.N;
+    const uint64_t c = c0 + r, N = kp.P;   // N: plane pitch
     const double t0 = (double)t0_of(lp, (int)(c % (uint64_t)kp.nx),
                                     (int)((c / (uint64_t)kp.nx) % (uint64_t)kp.ny));
     float f[19], g[7];
@@ -681,7 +681,7 @@ __global__ void lp_to_abs(KParams kp, LPParams lp, const S *__restrict__ F, cons
 {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const uint64_t c = c0 + r, N = kp.N;
+    const uint64_t c = c0 + r, N = kp.P;   // N: plane pitch
     float f[19], g[7];
     load_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
     load_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
@@ -699,7 +699,7 @@ __global__ void lp_recon_aos(KParams kp, const double *__restrict__ fo, const do
 {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const uint64_t c = c0 + r, N = kp.N;
+    const uint64_t c = c0 + r, N = kp.P;   // N: plane pitch
     const double *F = fo + r * 19, *G = go + r * 7;
     const double T = G[0] + G[1] + G[2] + G[3] + G[4] + G[5] + G[6];
     const double rho = F[0] + F[1] + F[2] + F[3] + F[4] + F[5] + F[6] + F[7] + F[8] + F[9] + F[10] +
@@ -726,8 +726,8 @@ __global__ void lp_recon_aos(KParams kp, const double *__restrict__ fo, const do
 __global__ void lp_bp_from_mac(KParams kp, const double *__restrict__ mac, float *__restrict__ Bp)
 {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t N = kp.N;
-    if (c >= N) return;
+    const uint64_t N = kp.P;   // plane pitch
+    if (c >= kp.N) return;
     const double T = mac[4 * N + c];
     Bp[c] = kp.use_g ? (float)(mac[N + c] * T) : 0.0f;
     Bp[N + c] = kp.use_g ? (float)(mac[2 * N + c] * T) : 0.0f;

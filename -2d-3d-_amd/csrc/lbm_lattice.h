@@ -72,6 +72,9 @@ struct KParams {
     int nxb;          // x blocks per row
     unsigned nblocks; // total blocks of the launch (for the XCD remap)
     uint64_t N;       // nodes
+    uint64_t P;       // plane pitch (elements) of every plane-structured array: F, G, B_prev,
+                      // aux, macros; N rounded up so that planes do not start at the same
+                      // offset modulo the memory interleave
     double gbeta, tref, thot, tcold;
     double sv, sq;    // 1/tau_f (s_e = s_eps = s_nu = s_pi) and s_q = s_m
     double qk, qnu, sg;
