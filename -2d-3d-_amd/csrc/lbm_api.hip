@@ -149,12 +149,10 @@ static void release(lbm_handle *h)
 static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **out);
 
 // Plane pitch (elements) of the plane-structured arrays.  A step streams up to 40 planes at
-// once; with a pitch of N nodes at 512^3 every plane is exactly 2^30 bytes from the next, all
-// streams start at the same offset modulo every power of two, and the step's speed then
-// depended on where the allocation landed (17.9 or 16.2 ms for the same FP64 kernels in two
-// allocations of one process).  The pitch adds 1 KiB plus up to 1 MiB (a sixteenth of a plane
-// for small planes) of population elements of size es; with 1 MiB + 1 KiB the FP64 step ran
-// in 15.6-15.8 ms in every allocation measured.
+// once; with a pitch of N nodes at 512^3 every plane is exactly 2^30 bytes from the next and all
+// streams start at the same offset modulo every power of two.  The pitch adds 1 KiB plus up to
+// 1 MiB (a sixteenth of a plane for small planes) of population elements of size es, so that
+// the planes spread over the memory channels.
 static uint64_t plane_pitch(uint64_t N, int es)
 {
     const uint64_t plane = N * (uint64_t)es;
