@@ -770,7 +770,12 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
     size_t free_b = 0, total_b = 0;
     HIPCHK(h, hipSetDevice(h->cfg.device));
     HIPCHK(h, hipMemGetInfo(&free_b, &total_b));
-    // time per full-size step of the fastest candidate over the layouts open to the domain
+    // time per full-size step of the fastest candidate over the layouts open to the domain.  The
+    // first candidate (the two-pass form in the first layout timed) is the incumbent; another
+    // replaces the best only when it is more than 3% faster: the same kernels run up to 11%
+    // apart depending on where an allocation lands, and the scratch domain's placement is not
+    // the domain's own.
+    constexpr double margin = 0.97;
     double best = 1e300;
     int rc = LBM_OK;
     bool timed = false;
@@ -806,7 +811,7 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
             double ms = 0.0;
             if (!rc) rc = lbm_step(t, 2);                   // warm-up with this form
             if (!rc) rc = lbm_step_timed(t, steps, &ms);
-            if (!rc && ms / steps * scale < best) {
+            if (!rc && (!timed || ms / steps * scale < margin * best)) {
                 best = ms / steps * scale;
                 win = cd;
                 timed = true;

@@ -52,11 +52,12 @@ def parse():
     ap.add_argument("--tile", type=int, default=0, help="one-pass thread tile (0: 32x16, 1: 16x16, 2: 32x8)")
     ap.add_argument("--prefetch", type=int, default=0, help="one-pass prefetching (0/1)")
     ap.add_argument("--kchunk", type=int, default=0, help="z-planes per one-pass block (0: default)")
-    ap.add_argument("--layout", choices=["tuned", "two-array", "aa", "auto"], default="tuned",
-                    help="population layout; tuned (default): an LBM_LAYOUT_AUTO domain whose "
-                         "layout and kernel form lbm_autotune picks before the first step "
-                         "(--variant, --kchunk, --tile, --prefetch are then ignored); aa: one "
-                         "array per lattice updated in place (two-pass form only)")
+    ap.add_argument("--layout", choices=["two-array", "aa", "auto", "tuned"], default="two-array",
+                    help="population layout (default two-array, with the default two-pass "
+                         "form); aa: one array per lattice updated in place (two-pass form only); "
+                         "tuned: an LBM_LAYOUT_AUTO domain whose layout and kernel form "
+                         "lbm_autotune picks before the first step (--variant, --kchunk, --tile, "
+                         "--prefetch are then ignored)")
     ap.add_argument("--tune-steps", type=int, default=8, help="timed steps per autotune candidate")
     ap.add_argument("--full-config", choices=["c1", "c2"], default="",
                     help="instead of the benchmark: the whole 2D 257x129 Ra=1e5 run (20,000 "
