@@ -232,6 +232,50 @@ __device__ __forceinline__ void in_heat(const double *P, uint64_t N, int i, int 
 // the thermal lattice is in state LAY (pass A runs between the two flow states)
 constexpr int lay_flip(int lay) { return lay == LAY_PULL ? LAY_PULL : 3 - lay; }
 
+// Offset rows (PullOffsets, lbm_lattice.h): each population is read through a buffer descriptor
+// whose base is the array plus the population's byte offset (built in scalar registers), with
+// the node's 32-bit byte offset as the only vector operand.  Plain pointer arithmetic here was
+// compiled into one 64-bit vector add per load.  Word 3 of the descriptor is the raw-buffer
+// format of the gfx9 family; the record count covers 4 GiB (the offsets fit 32 bits, so no
+// access is clamped).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const double *P, int64_t off)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)P + off), (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ double ld_off(const double *P, int64_t off, uint32_t vo)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(P, off), vo, 0, 0));
+}
+__device__ __forceinline__ void offset_flow_in(const double *P, const PullOffsets &po, uint32_t vo,
+                                               int i, int j, int k, uint64_t c, const KParams &kp,
+                                               double *f)
+{
+#pragma unroll
+    for (int a = 0; a < 19; ++a) f[a] = ld_off(P, po.f[a], vo);
+    if (i == 0 || i == kp.nx - 1) pull_flow<D3Q19>(P, kp.P, i, j, k, c, true, kp, f);
+}
+__device__ __forceinline__ void offset_heat_in(const double *P, const PullOffsets &po, uint32_t vo,
+                                               int i, int j, int k, uint64_t c, const KParams &kp,
+                                               double *g)
+{
+#pragma unroll
+    for (int a = 0; a < 7; ++a) g[a] = ld_off(P, po.g[a], vo);
+    if (i == 0 || i == kp.nx - 1)
+        heat_in_all<D3Q7>(P, kp.P, i, j, k, c, kp, g, std::make_integer_sequence<int, 7>{});
+}
+template <int Q>
+__device__ __forceinline__ void offset_out(double *P, const PullOffsets &po, uint32_t vo,
+                                           const double *v)
+{
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        const auto bits = __builtin_bit_cast(unsigned long long, v[a]);
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(row_rsrc(P, 0), 0, 0, 0)), bits),
+            row_rsrc(P, po.s[a]), vo, 0, 0);
+    }
+}
+
 __device__ __forceinline__ bool on_border(int i, int j, int k, const KParams &kp)
 {
     return i == 0 || j == 0 || k == 0 || i == kp.nx - 1 || j == kp.ny - 1 || k == kp.nz - 1;
@@ -300,10 +344,13 @@ __device__ __forceinline__ void thermal3_node(const KParams &kp, int i, int j, i
                                               const double *__restrict__ Gin,
                                               double *__restrict__ Gout, double *__restrict__ Bp,
                                               double *__restrict__ aux,
-                                              const double *__restrict__ mac)
+                                              const double *__restrict__ mac,
+                                              const PullOffsets *po = nullptr)
 {
     const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
+    const bool fast = po != nullptr && offset_row(*po, j, k, kp);
+    const uint32_t vo = (uint32_t)c * 8u;   // used when fast
     const double *G = LAY == LAY_PULL ? Gin : Gout;
     double g[7], Tn;
     if (FIRST) {   // the plain state; in place it is LAY_ODD, read the same way
@@ -311,11 +358,13 @@ __device__ __forceinline__ void thermal3_node(const KParams &kp, int i, int j, i
         for (int a = 0; a < 7; ++a) g[a] = G[a * N + c];
         Tn = mac[4 * N + c];
     } else {
-        in_heat<LAY>(G, N, i, j, k, c, border, kp, g);
+        if (fast) offset_heat_in(G, *po, vo, i, j, k, c, kp, g);
+        else in_heat<LAY>(G, N, i, j, k, c, border, kp, g);
         Tn = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
     }
     double f[19];
-    in_flow<lay_flip(LAY)>(Fcur, N, i, j, k, c, border, kp, f);
+    if (fast) offset_flow_in(Fcur, *po, vo, i, j, k, c, kp, f);
+    else in_flow<lay_flip(LAY)>(Fcur, N, i, j, k, c, border, kp, f);
     const double rho = sum19(f);
     const double Fy = rho * kp.gbeta * (Tn - kp.tref);
     const double u = jx19(f) / rho;
@@ -324,7 +373,9 @@ __device__ __forceinline__ void thermal3_node(const KParams &kp, int i, int j, i
     double bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
     double p[7];
     ref_collide_d3q7(g, p, Tn, u, v, w, bp, kp);
-    if (LAY == LAY_PULL) {
+    if (fast) {
+        offset_out<7>(Gout, *po, vo, p);
+    } else if (LAY == LAY_PULL) {
 #pragma unroll
         for (int a = 0; a < 7; ++a) Gout[a * N + c] = p[a];
     } else {
@@ -344,12 +395,13 @@ __global__ void __launch_bounds__(256) k3_thermal(KParams kp, const double *__re
                                                   double *__restrict__ Gout,
                                                   double *__restrict__ Bp,
                                                   double *__restrict__ aux,
-                                                  const double *__restrict__ mac)
+                                                  const double *__restrict__ mac, PullOffsets po)
 {
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    thermal3_node<FIRST, LAY>(kp, i, j, k, c, Fcur, Gin, Gout, Bp, aux, mac);
+    thermal3_node<FIRST, LAY>(kp, i, j, k, c, Fcur, Gin, Gout, Bp, aux, mac,
+                              LAY == LAY_PULL && !FIRST ? &po : nullptr);
 }
 
 // Pass B of step n: flow collision of step n+1.  LAY: layout of both lattices (LAY_PULL:
@@ -359,22 +411,29 @@ __device__ __forceinline__ void flow3_node(const KParams &kp, int i, int j, int 
                                            const double *__restrict__ Fcur,
                                            double *__restrict__ Fnext,
                                            const double *__restrict__ Gpost,
-                                           const double *__restrict__ aux)
+                                           const double *__restrict__ aux,
+                                           const PullOffsets *po = nullptr)
 {
     const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border(i, j, k, kp);
+    const bool fast = po != nullptr && offset_row(*po, j, k, kp);
+    const uint32_t vo = (uint32_t)c * 8u;   // used when fast
     double g[7];
-    in_heat<LAY>(Gpost, N, i, j, k, c, border, kp, g);
+    if (fast) offset_heat_in(Gpost, *po, vo, i, j, k, c, kp, g);
+    else in_heat<LAY>(Gpost, N, i, j, k, c, border, kp, g);
     const double T1 = g[0] + g[1] + g[2] + g[3] + g[4] + g[5] + g[6];
     double f[19];
-    in_flow<LAY>(LAY == LAY_PULL ? Fcur : Fnext, N, i, j, k, c, border, kp, f);
+    if (fast) offset_flow_in(Fcur, *po, vo, i, j, k, c, kp, f);
+    else in_flow<LAY>(LAY == LAY_PULL ? Fcur : Fnext, N, i, j, k, c, border, kp, f);
     const double rho = sum19(f);
     const double Fy = rho * kp.gbeta * (aux[c] - kp.tref);
     const double u = jx19(f) / rho;
     const double v = (jy19(f) + 0.5 * Fy) / rho;
     const double w = jz19(f) / rho;
     ref_collide_d3q19(f, f, rho, u, v, w, T1, kp);
-    if (LAY == LAY_PULL) {
+    if (fast) {
+        offset_out<19>(Fnext, *po, vo, f);
+    } else if (LAY == LAY_PULL) {
 #pragma unroll
         for (int a = 0; a < 19; ++a) Fnext[a * N + c] = f[a];
     } else {
@@ -386,12 +445,12 @@ template <int LAY>
 __global__ void __launch_bounds__(256) k3_flow(KParams kp, const double *__restrict__ Fcur,
                                                double *__restrict__ Fnext,
                                                const double *__restrict__ Gpost,
-                                               const double *__restrict__ aux)
+                                               const double *__restrict__ aux, PullOffsets po)
 {
     int i, j, k;
     uint64_t c;
     if (!node_of(kp, i, j, k, c)) return;
-    flow3_node<LAY>(kp, i, j, k, c, Fcur, Fnext, Gpost, aux);
+    flow3_node<LAY>(kp, i, j, k, c, Fcur, Fnext, Gpost, aux, LAY == LAY_PULL ? &po : nullptr);
 }
 
 // Batched two-pass step: many domains per launch.  Block b (after the XCD remap) belongs to
@@ -1216,8 +1275,9 @@ hipError_t launch_prologue(const KParams &kp, int bx, hipStream_t s, const doubl
 hipError_t launch_flow3_aa(const KParams &kp, int bx, hipStream_t s, int lay, double *F,
                            const double *G, const double *aux)
 {
-    if (lay == LAY_ODD) k3_flow<LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, nullptr, F, G, aux);
-    else if (lay == LAY_EVEN) k3_flow<LAY_EVEN><<<step_grid(kp), bx, 0, s>>>(kp, nullptr, F, G, aux);
+    const PullOffsets po = pull_offsets(kp, 8);
+    if (lay == LAY_ODD) k3_flow<LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, nullptr, F, G, aux, po);
+    else if (lay == LAY_EVEN) k3_flow<LAY_EVEN><<<step_grid(kp), bx, 0, s>>>(kp, nullptr, F, G, aux, po);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -1226,14 +1286,15 @@ hipError_t launch_thermal3_aa(const KParams &kp, int bx, hipStream_t s, int lay,
                               const double *F, double *G, double *Bp, double *aux,
                               const double *mac)
 {
+    const PullOffsets po = pull_offsets(kp, 8);
     if (first && lay == LAY_ODD)
-        k3_thermal<true, LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac);
+        k3_thermal<true, LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac, po);
     else if (first)
         return hipErrorInvalidValue;   // the plain state is LAY_ODD
     else if (lay == LAY_ODD)
-        k3_thermal<false, LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac);
+        k3_thermal<false, LAY_ODD><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac, po);
     else if (lay == LAY_EVEN)
-        k3_thermal<false, LAY_EVEN><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac);
+        k3_thermal<false, LAY_EVEN><<<step_grid(kp), bx, 0, s>>>(kp, F, nullptr, G, Bp, aux, mac, po);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1243,9 +1304,10 @@ hipError_t launch_step3(const KParams &kp, int bx, hipStream_t s, bool first, co
                         double *Fnext, const double *Gin, double *Gout, double *Bp, double *aux,
                         const double *mac)
 {
-    if (first) k3_thermal<true, LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
-    else k3_thermal<false, LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac);
-    k3_flow<LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Fnext, Gout, aux);
+    const PullOffsets po = pull_offsets(kp, 8);
+    if (first) k3_thermal<true, LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac, po);
+    else k3_thermal<false, LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Gin, Gout, Bp, aux, mac, po);
+    k3_flow<LAY_PULL><<<step_grid(kp), bx, 0, s>>>(kp, Fcur, Fnext, Gout, aux, po);
     return hipGetLastError();
 }
 

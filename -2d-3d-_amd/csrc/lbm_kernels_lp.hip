@@ -253,6 +253,44 @@ __device__ __forceinline__ void load_all(const S *__restrict__ P, uint64_t N, ui
 }
 #define SEQ(L) std::make_integer_sequence<int, L::Q>{}
 
+template <typename S, int Q, int A>
+__device__ __forceinline__ float ld_at(const S *P, int64_t off, uint32_t vo)
+{
+    return Pop<S>::template ld<Q, A>((const S *)((const char *)P + off + vo));
+}
+template <typename S, int Q, int A>
+__device__ __forceinline__ void st_at(S *P, int64_t off, uint32_t vo, float v)
+{
+    Pop<S>::template st<Q, A>((S *)((char *)P + off + vo), v);
+}
+template <typename S, int... A>
+__device__ __forceinline__ void fast_flow_in(const S *P, const PullOffsets &pl, uint32_t vo, int i, int j,
+                                             int k, uint64_t c, const KParams &kp, float *f,
+                                             std::integer_sequence<int, A...>)
+{
+    ((f[A] = ld_at<S, 19, A>(P, pl.f[A], vo)), ...);
+    if (i == 0 || i == kp.nx - 1) {
+        ((f[A] = D3Q19::ex[A] != 0 ? flow_in<D3Q19, S, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
+    }
+}
+template <typename S, int... A>
+__device__ __forceinline__ void fast_heat_in(const S *P, const PullOffsets &pl, uint32_t vo, int i, int j,
+                                             int k, uint64_t c, const KParams &kp,
+                                             const LPParams &lp, float *g,
+                                             std::integer_sequence<int, A...>)
+{
+    ((g[A] = ld_at<S, 7, A>(P, pl.g[A], vo) + t0_pull<A>(lp)), ...);
+    if (i == 0 || i == kp.nx - 1) {
+        ((g[A] = D3Q7::ex[A] != 0 ? heat_in<D3Q7, S, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
+    }
+}
+template <typename S, int Q, int... A>
+__device__ __forceinline__ void fast_out(S *P, const PullOffsets &pl, uint32_t vo, const float *v,
+                                         std::integer_sequence<int, A...>)
+{
+    (st_at<S, Q, A>(P, pl.s[A], vo, v[A]), ...);
+}
+
 // -------------------------------------------------------------------------------------------
 // deviation-form collisions (FP32)
 // -------------------------------------------------------------------------------------------
@@ -442,22 +480,27 @@ __device__ __forceinline__ void lp3_thermal_node(const KParams &kp, const LPPara
                                                  int k, uint64_t c, const S *__restrict__ Fcur,
                                                  const S *__restrict__ Gin, S *__restrict__ Gout,
                                                  float *__restrict__ Bp, float *__restrict__ aux,
-                                                 const double *__restrict__ mac)
+                                                 const double *__restrict__ mac,
+                                                 const PullOffsets *pl = nullptr)
 {
     const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border_lp(i, j, k, kp);
     const float t0 = t0_of(lp, i, j), b0 = (float)((double)t0 - lp.tref);
     const S *G = LAY == LAY_PULL ? Gin : Gout;
+    const bool fast = pl != nullptr && offset_row(*pl, j, k, kp);
+    const uint32_t vo = (uint32_t)c * (uint32_t)sizeof(S);   // used when fast
     float g[7], dT;   // dT = T - T0(x)
     if (FIRST) {   // the plain state; in place it is LAY_ODD, read the same way
         load_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
         dT = (float)(mac[4 * N + c] - (double)t0);
     } else {
-        in_heat_lp<D3Q7, S, LAY>(G, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+        if (fast) fast_heat_in<S>(G, *pl, vo, i, j, k, c, kp, lp, g, SEQ(D3Q7));
+        else in_heat_lp<D3Q7, S, LAY>(G, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
         dT = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
     }
     float f[19];
-    in_flow_lp<D3Q19, S, lay_flip_lp(LAY)>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    if (fast) fast_flow_in<S>(Fcur, *pl, vo, i, j, k, c, kp, f, SEQ(D3Q19));
+    else in_flow_lp<D3Q19, S, lay_flip_lp(LAY)>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
     float drho, jx, jy, jz;
     moments19(f, drho, jx, jy, jz);
     const float rho = 1.0f + drho;
@@ -468,7 +511,8 @@ __device__ __forceinline__ void lp3_thermal_node(const KParams &kp, const LPPara
     float bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
     float p[7];
     collide7_dev(g, p, dT, t0 + dT, u, v, w, bp, kp.use_g, lp);
-    out_lp<D3Q7, S, LAY>(Gout, N, i, j, k, c, border, kp, p, SEQ(D3Q7));
+    if (fast) fast_out<S, 7>(Gout, *pl, vo, p, SEQ(D3Q7));
+    else out_lp<D3Q7, S, LAY>(Gout, N, i, j, k, c, border, kp, p, SEQ(D3Q7));
     if (kp.use_g) {
         Bp[c] = bp[0];
         Bp[N + c] = bp[1];
@@ -481,12 +525,13 @@ template <typename S, bool FIRST, int LAY>
 __global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, const S *__restrict__ Fcur,
                                                    const S *__restrict__ Gin, S *__restrict__ Gout,
                                                    float *__restrict__ Bp, float *__restrict__ aux,
-                                                   const double *__restrict__ mac)
+                                                   const double *__restrict__ mac, PullOffsets pl)
 {
     int i, j, k;
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
-    lp3_thermal_node<S, FIRST, LAY>(kp, lp, i, j, k, c, Fcur, Gin, Gout, Bp, aux, mac);
+    lp3_thermal_node<S, FIRST, LAY>(kp, lp, i, j, k, c, Fcur, Gin, Gout, Bp, aux, mac,
+                                    LAY == LAY_PULL && !FIRST ? &pl : nullptr);
 }
 
 // LAY: layout of both lattices (LAY_PULL: Fcur -> Fnext; in place: Fnext holds state LAY and
@@ -495,17 +540,22 @@ template <typename S, int LAY>
 __device__ __forceinline__ void lp3_flow_node(const KParams &kp, const LPParams &lp, int i, int j,
                                               int k, uint64_t c, const S *__restrict__ Fcur,
                                               S *__restrict__ Fnext, const S *__restrict__ Gpost,
-                                              const float *__restrict__ aux)
+                                              const float *__restrict__ aux,
+                                              const PullOffsets *pl = nullptr)
 {
     const uint64_t N = kp.P;   // plane pitch
     const bool border = on_border_lp(i, j, k, kp);
+    const bool fast = pl != nullptr && offset_row(*pl, j, k, kp);
+    const uint32_t vo = (uint32_t)c * (uint32_t)sizeof(S);   // used when fast
     float g[7];
-    in_heat_lp<D3Q7, S, LAY>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
+    if (fast) fast_heat_in<S>(Gpost, *pl, vo, i, j, k, c, kp, lp, g, SEQ(D3Q7));
+    else in_heat_lp<D3Q7, S, LAY>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
     const float dT1 = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
     const float b0 = (float)((double)t0_of(lp, i, j) - lp.tref);
     float f[19];
-    in_flow_lp<D3Q19, S, LAY>(LAY == LAY_PULL ? Fcur : Fnext, N, i, j, k, c, border, kp, f,
-                              SEQ(D3Q19));
+    if (fast) fast_flow_in<S>(Fcur, *pl, vo, i, j, k, c, kp, f, SEQ(D3Q19));
+    else in_flow_lp<D3Q19, S, LAY>(LAY == LAY_PULL ? Fcur : Fnext, N, i, j, k, c, border, kp, f,
+                                   SEQ(D3Q19));
     float drho, jx, jy, jz;
     moments19(f, drho, jx, jy, jz);
     const float rho = 1.0f + drho;
@@ -513,18 +563,20 @@ __device__ __forceinline__ void lp3_flow_node(const KParams &kp, const LPParams 
     const float Fy = rho * lp.gbeta * aux[c];
     const float u = jx * ir, v = (jy + 0.5f * Fy) * ir, w = jz * ir;
     collide19_dev(f, f, drho, u, v, w, b0 + dT1, lp);
-    out_lp<D3Q19, S, LAY>(Fnext, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    if (fast) fast_out<S, 19>(Fnext, *pl, vo, f, SEQ(D3Q19));
+    else out_lp<D3Q19, S, LAY>(Fnext, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
 }
 
 template <typename S, int LAY>
 __global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S *__restrict__ Fcur,
                                                 S *__restrict__ Fnext, const S *__restrict__ Gpost,
-                                                const float *__restrict__ aux)
+                                                const float *__restrict__ aux, PullOffsets pl)
 {
     int i, j, k;
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
-    lp3_flow_node<S, LAY>(kp, lp, i, j, k, c, Fcur, Fnext, Gpost, aux);
+    lp3_flow_node<S, LAY>(kp, lp, i, j, k, c, Fcur, Fnext, Gpost, aux,
+                          LAY == LAY_PULL ? &pl : nullptr);
 }
 
 // Batched two-pass step over a per-domain table (see k3_thermal_batch).
@@ -887,6 +939,7 @@ template <typename S>
 static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, const LPArgs &a)
 {
     const LPParams lp = lp_params(kp);
+    const PullOffsets pl = pull_offsets(kp, (int64_t)sizeof(S));
     const dim3 grid(kp.nblocks);
     const S *Fc = (const S *)a.Fc;
     switch (op) {
@@ -896,28 +949,28 @@ static hipError_t lp_dispatch(int op, const KParams &kp, int bx, hipStream_t s, 
         break;
     case LP_STEP:
         if (a.first)
-            lp3_thermal<S, true, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
+            lp3_thermal<S, true, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac, pl);
         else
-            lp3_thermal<S, false, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac);
-        lp3_flow<S, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, (const S *)a.Go, a.aux);
+            lp3_thermal<S, false, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (const S *)a.Gi, (S *)a.Go, a.Bp, a.aux, a.mac, pl);
+        lp3_flow<S, LAY_PULL><<<grid, bx, 0, s>>>(kp, lp, Fc, (S *)a.Fn, (const S *)a.Go, a.aux, pl);
         break;
     case LP_FLOW_AA:
         if (a.lay == LAY_ODD)
-            lp3_flow<S, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, nullptr, (S *)a.Fn, (const S *)a.Gi, a.aux);
+            lp3_flow<S, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, nullptr, (S *)a.Fn, (const S *)a.Gi, a.aux, pl);
         else if (a.lay == LAY_EVEN)
-            lp3_flow<S, LAY_EVEN><<<grid, bx, 0, s>>>(kp, lp, nullptr, (S *)a.Fn, (const S *)a.Gi, a.aux);
+            lp3_flow<S, LAY_EVEN><<<grid, bx, 0, s>>>(kp, lp, nullptr, (S *)a.Fn, (const S *)a.Gi, a.aux, pl);
         else
             return hipErrorInvalidValue;
         break;
     case LP_THERMAL_AA:
         if (a.first && a.lay == LAY_ODD)
-            lp3_thermal<S, true, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac);
+            lp3_thermal<S, true, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac, pl);
         else if (a.first)
             return hipErrorInvalidValue;   // the plain state is LAY_ODD
         else if (a.lay == LAY_ODD)
-            lp3_thermal<S, false, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac);
+            lp3_thermal<S, false, LAY_ODD><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac, pl);
         else if (a.lay == LAY_EVEN)
-            lp3_thermal<S, false, LAY_EVEN><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac);
+            lp3_thermal<S, false, LAY_EVEN><<<grid, bx, 0, s>>>(kp, lp, Fc, nullptr, (S *)a.Go, a.Bp, a.aux, a.mac, pl);
         else
             return hipErrorInvalidValue;
         break;

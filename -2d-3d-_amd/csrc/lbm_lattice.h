@@ -84,6 +84,40 @@ struct KParams {
     double neqc, abb, wt0, wt1;
 };
 
+// Rows with no y or z wall, two arrays: every pull is one load at a fixed byte offset from the
+// array (population plane minus streaming shift, computed on the host) plus the node's own
+// 32-bit byte offset.  The offsets stay in scalar registers and the loads take the node offset
+// as their only vector operand, so a wave spends no per-population address arithmetic.  Nodes
+// on an x wall then redo their pulls with the generic wall rules; the addresses loaded are
+// those of the generic pulls, so the results are identical.
+struct PullOffsets {
+    int64_t f[19];    // bytes: (a P - (ex + ey nx + ez nx ny)) es, flow lattice
+    int64_t g[7];     // the same for the thermal lattice
+    int64_t s[19];    // bytes: a P es, the node's own slot of direction a
+    int fast;         // 1: P es < 2^32, so node byte offsets fit 32 bits
+};
+
+inline PullOffsets pull_offsets(const KParams &kp, int64_t es)
+{
+    PullOffsets po;
+    const int64_t P = (int64_t)kp.P, sy = kp.nx, sz = (int64_t)kp.nx * kp.ny;
+    for (int a = 0; a < 19; ++a) {
+        po.f[a] = (a * P - (D3Q19::ex[a] + D3Q19::ey[a] * sy + D3Q19::ez[a] * sz)) * es;
+        po.s[a] = a * P * es;
+    }
+    for (int a = 0; a < 7; ++a)
+        po.g[a] = (a * P - (D3Q7::ex[a] + D3Q7::ey[a] * sy + D3Q7::ez[a] * sz)) * es;
+    po.fast = (uint64_t)P * (uint64_t)es < ((uint64_t)1 << 32);
+    return po;
+}
+
+// A row takes the offset path when it has no y or z wall (a block of a two-pass launch covers
+// part of one row, so the choice is the same for the whole block).
+__device__ __forceinline__ bool offset_row(const PullOffsets &po, int j, int k, const KParams &kp)
+{
+    return po.fast && j > 0 && k > 0 && j < kp.ny - 1 && k < kp.nz - 1;
+}
+
 // Launch geometry of the one-pass kernel (lbm_fused.h): ntx x nty tiles of (TX-2) x (TY-2)
 // nodes, z split into chunks of kchunk planes; tile x fastest, then tile y, then chunk.
 struct FusedGeom {

@@ -102,3 +102,35 @@ def test_batch_fp16_table_launch_equals_per_domain(variant):
     for m, r in zip(run(rblbm.BATCH_GRAPH), run(rblbm.BATCH_PER_DOMAIN)):
         for k in ("rho", "u", "v", "w", "T"):
             np.testing.assert_array_equal(m[k], r[k])
+
+
+WALLS = [dict(),                                                        # RB cell, adiabatic sides
+         dict(vel_x=1, vel_z=1),                                       # periodic flow in x, z
+         dict(side_heated=1, temp_x=0, temp_y=1, thot=1.0, tcold=0.0),  # isothermal x walls
+         dict(use_g=0)]                                                # no M1G correction
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+def test_lowprec_domain_kernels_equal_table_kernels_on_every_wall(prec):
+    # A domain's own two-pass kernels load interior rows at precomputed offsets and redo the
+    # x-wall nodes with the generic pulls; the batch table kernels use the generic pulls for
+    # every node.  Both must agree bitwise for every kind of wall, including nodes on x walls.
+    shapes = [(40, 20, 12), (33, 18, 14), (24, 16, 10), (20, 14, 12)]
+    cfgs = [rblbm.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=1e6, precision=prec, **w)
+            for (a, b, c), w in zip(shapes, WALLS)]
+
+    def run(mode):
+        b = rblbm.Batch(cfgs)
+        b.set_mode(mode)
+        b.set_kernel_variant(rblbm.KERNEL_TWO_PASS)
+        b.initial_seeded(17, 0.05)
+        b.step(13)
+        out = [(d.macros(), d.gather(with_bprev=False)) for d in b.domains]
+        b.close()
+        return out
+    for n, ((m1, s1), (m2, s2)) in enumerate(zip(run(rblbm.BATCH_PER_DOMAIN),
+                                                run(rblbm.BATCH_ONE_LAUNCH))):
+        for k in ("rho", "u", "v", "w", "T"):
+            np.testing.assert_array_equal(m1[k], m2[k], err_msg=f"{WALLS[n]} {k}")
+        for k in ("f", "g"):
+            np.testing.assert_array_equal(s1[k], s2[k], err_msg=f"{WALLS[n]} {k}")
