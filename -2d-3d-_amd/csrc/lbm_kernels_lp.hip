@@ -263,32 +263,47 @@ __device__ __forceinline__ void st_at(S *P, int64_t off, uint32_t vo, float v)
 {
     Pop<S>::template st<Q, A>((S *)((char *)P + off + vo), v);
 }
-template <typename S, int... A>
+// Offset rows in layout LAY (two arrays: pulls; in place: the slots of state LAY).
+template <typename S, int LAY, int... A>
 __device__ __forceinline__ void fast_flow_in(const S *P, const PullOffsets &pl, uint32_t vo, int i, int j,
                                              int k, uint64_t c, const KParams &kp, float *f,
                                              std::integer_sequence<int, A...>)
 {
-    ((f[A] = ld_at<S, 19, A>(P, pl.f[A], vo)), ...);
+    ((f[A] = ld_at<S, 19, A>(P, LAY == LAY_PULL ? pl.f[A] : (LAY == LAY_ODD ? pl.s[A] : pl.fe[A]), vo)), ...);
     if (i == 0 || i == kp.nx - 1) {
-        ((f[A] = D3Q19::ex[A] != 0 ? flow_in<D3Q19, S, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
+        if constexpr (LAY == LAY_PULL)
+            ((f[A] = D3Q19::ex[A] != 0 ? flow_in<D3Q19, S, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
+        else
+            ((f[A] = D3Q19::ex[A] != 0 ? flow_in_aa<D3Q19, S, LAY, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
     }
 }
-template <typename S, int... A>
+template <typename S, int LAY, int... A>
 __device__ __forceinline__ void fast_heat_in(const S *P, const PullOffsets &pl, uint32_t vo, int i, int j,
                                              int k, uint64_t c, const KParams &kp,
                                              const LPParams &lp, float *g,
                                              std::integer_sequence<int, A...>)
 {
-    ((g[A] = ld_at<S, 7, A>(P, pl.g[A], vo) + t0_pull<A>(lp)), ...);
+    ((g[A] = ld_at<S, 7, A>(P, LAY == LAY_PULL ? pl.g[A] : (LAY == LAY_ODD ? pl.s[A] : pl.ge[A]), vo) +
+             t0_pull<A>(lp)), ...);
     if (i == 0 || i == kp.nx - 1) {
-        ((g[A] = D3Q7::ex[A] != 0 ? heat_in<D3Q7, S, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
+        if constexpr (LAY == LAY_PULL)
+            ((g[A] = D3Q7::ex[A] != 0 ? heat_in<D3Q7, S, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
+        else
+            ((g[A] = D3Q7::ex[A] != 0 ? heat_in_aa<D3Q7, S, LAY, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
     }
 }
-template <typename S, int Q, int... A>
-__device__ __forceinline__ void fast_out(S *P, const PullOffsets &pl, uint32_t vo, const float *v,
+template <class L, typename S, int LAY, int... A>
+__device__ __forceinline__ void fast_out(S *P, const PullOffsets &pl, uint32_t vo, int i, int j, int k,
+                                         uint64_t c, const KParams &kp, const float *v,
                                          std::integer_sequence<int, A...>)
 {
-    (st_at<S, Q, A>(P, pl.s[A], vo, v[A]), ...);
+    if (LAY == LAY_EVEN && (i == 0 || i == kp.nx - 1)) {   // populations leaving through x walls
+        (store_aa<L, S, LAY, A>(P, kp.P, i, j, k, c, true, kp, v[A]), ...);
+        return;
+    }
+    constexpr bool F = L::Q == 19;
+    (st_at<S, L::Q, A>(P, LAY == LAY_PULL ? pl.s[A] : (LAY == LAY_ODD ? pl.s[L::opp[A]] : (F ? pl.fw[A] : pl.gw[A])),
+                       vo, v[A]), ...);
 }
 
 // -------------------------------------------------------------------------------------------
@@ -494,12 +509,12 @@ __device__ __forceinline__ void lp3_thermal_node(const KParams &kp, const LPPara
         load_all<D3Q7, S>(G, N, c, g, SEQ(D3Q7));
         dT = (float)(mac[4 * N + c] - (double)t0);
     } else {
-        if (fast) fast_heat_in<S>(G, *pl, vo, i, j, k, c, kp, lp, g, SEQ(D3Q7));
+        if (fast) fast_heat_in<S, LAY>(G, *pl, vo, i, j, k, c, kp, lp, g, SEQ(D3Q7));
         else in_heat_lp<D3Q7, S, LAY>(G, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
         dT = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
     }
     float f[19];
-    if (fast) fast_flow_in<S>(Fcur, *pl, vo, i, j, k, c, kp, f, SEQ(D3Q19));
+    if (fast) fast_flow_in<S, lay_flip_lp(LAY)>(Fcur, *pl, vo, i, j, k, c, kp, f, SEQ(D3Q19));
     else in_flow_lp<D3Q19, S, lay_flip_lp(LAY)>(Fcur, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
     float drho, jx, jy, jz;
     moments19(f, drho, jx, jy, jz);
@@ -511,7 +526,7 @@ __device__ __forceinline__ void lp3_thermal_node(const KParams &kp, const LPPara
     float bp[3] = {Bp[c], Bp[N + c], Bp[2 * N + c]};
     float p[7];
     collide7_dev(g, p, dT, t0 + dT, u, v, w, bp, kp.use_g, lp);
-    if (fast) fast_out<S, 7>(Gout, *pl, vo, p, SEQ(D3Q7));
+    if (fast) fast_out<D3Q7, S, LAY>(Gout, *pl, vo, i, j, k, c, kp, p, SEQ(D3Q7));
     else out_lp<D3Q7, S, LAY>(Gout, N, i, j, k, c, border, kp, p, SEQ(D3Q7));
     if (kp.use_g) {
         Bp[c] = bp[0];
@@ -531,7 +546,7 @@ __global__ void __launch_bounds__(256) lp3_thermal(KParams kp, LPParams lp, cons
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
     lp3_thermal_node<S, FIRST, LAY>(kp, lp, i, j, k, c, Fcur, Gin, Gout, Bp, aux, mac,
-                                    LAY == LAY_PULL && !FIRST ? &pl : nullptr);
+                                    &pl);
 }
 
 // LAY: layout of both lattices (LAY_PULL: Fcur -> Fnext; in place: Fnext holds state LAY and
@@ -548,12 +563,12 @@ __device__ __forceinline__ void lp3_flow_node(const KParams &kp, const LPParams 
     const bool fast = pl != nullptr && offset_row(*pl, j, k, kp);
     const uint32_t vo = (uint32_t)c * (uint32_t)sizeof(S);   // used when fast
     float g[7];
-    if (fast) fast_heat_in<S>(Gpost, *pl, vo, i, j, k, c, kp, lp, g, SEQ(D3Q7));
+    if (fast) fast_heat_in<S, LAY>(Gpost, *pl, vo, i, j, k, c, kp, lp, g, SEQ(D3Q7));
     else in_heat_lp<D3Q7, S, LAY>(Gpost, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
     const float dT1 = (g[0] + g[1]) + (g[2] + g[3]) + (g[4] + g[5]) + g[6];
     const float b0 = (float)((double)t0_of(lp, i, j) - lp.tref);
     float f[19];
-    if (fast) fast_flow_in<S>(Fcur, *pl, vo, i, j, k, c, kp, f, SEQ(D3Q19));
+    if (fast) fast_flow_in<S, LAY>(LAY == LAY_PULL ? Fcur : Fnext, *pl, vo, i, j, k, c, kp, f, SEQ(D3Q19));
     else in_flow_lp<D3Q19, S, LAY>(LAY == LAY_PULL ? Fcur : Fnext, N, i, j, k, c, border, kp, f,
                                    SEQ(D3Q19));
     float drho, jx, jy, jz;
@@ -563,7 +578,7 @@ __device__ __forceinline__ void lp3_flow_node(const KParams &kp, const LPParams 
     const float Fy = rho * lp.gbeta * aux[c];
     const float u = jx * ir, v = (jy + 0.5f * Fy) * ir, w = jz * ir;
     collide19_dev(f, f, drho, u, v, w, b0 + dT1, lp);
-    if (fast) fast_out<S, 19>(Fnext, *pl, vo, f, SEQ(D3Q19));
+    if (fast) fast_out<D3Q19, S, LAY>(Fnext, *pl, vo, i, j, k, c, kp, f, SEQ(D3Q19));
     else out_lp<D3Q19, S, LAY>(Fnext, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
 }
 
@@ -576,7 +591,7 @@ __global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S
     uint64_t c;
     if (!node_of_lp(kp, i, j, k, c)) return;
     lp3_flow_node<S, LAY>(kp, lp, i, j, k, c, Fcur, Fnext, Gpost, aux,
-                          LAY == LAY_PULL ? &pl : nullptr);
+                          &pl);
 }
 
 // Batched two-pass step over a per-domain table (see k3_thermal_batch).

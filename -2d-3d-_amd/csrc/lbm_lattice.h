@@ -90,10 +90,16 @@ struct KParams {
 // as their only vector operand, so a wave spends no per-population address arithmetic.  Nodes
 // on an x wall then redo their pulls with the generic wall rules; the addresses loaded are
 // those of the generic pulls, so the results are identical.
+// The in-place layout uses the same scheme: state LAY_ODD reads slot (a, x) and writes
+// (opp a, x), both at s[]; state LAY_EVEN reads (opp a, x - e_a) and writes (a, x + e_a).
 struct PullOffsets {
     int64_t f[19];    // bytes: (a P - (ex + ey nx + ez nx ny)) es, flow lattice
     int64_t g[7];     // the same for the thermal lattice
     int64_t s[19];    // bytes: a P es, the node's own slot of direction a
+    int64_t fe[19];   // bytes: (opp(a) P - (ex + ey nx + ez nx ny)) es, read in LAY_EVEN
+    int64_t ge[7];
+    int64_t fw[19];   // bytes: (a P + (ex + ey nx + ez nx ny)) es, written from LAY_EVEN
+    int64_t gw[7];
     int fast;         // 1: P es < 2^32, so node byte offsets fit 32 bits
 };
 
@@ -102,11 +108,18 @@ inline PullOffsets pull_offsets(const KParams &kp, int64_t es)
     PullOffsets po;
     const int64_t P = (int64_t)kp.P, sy = kp.nx, sz = (int64_t)kp.nx * kp.ny;
     for (int a = 0; a < 19; ++a) {
-        po.f[a] = (a * P - (D3Q19::ex[a] + D3Q19::ey[a] * sy + D3Q19::ez[a] * sz)) * es;
+        const int64_t sh = D3Q19::ex[a] + D3Q19::ey[a] * sy + D3Q19::ez[a] * sz;
+        po.f[a] = (a * P - sh) * es;
         po.s[a] = a * P * es;
+        po.fe[a] = (D3Q19::opp[a] * P - sh) * es;
+        po.fw[a] = (a * P + sh) * es;
     }
-    for (int a = 0; a < 7; ++a)
-        po.g[a] = (a * P - (D3Q7::ex[a] + D3Q7::ey[a] * sy + D3Q7::ez[a] * sz)) * es;
+    for (int a = 0; a < 7; ++a) {
+        const int64_t sh = D3Q7::ex[a] + D3Q7::ey[a] * sy + D3Q7::ez[a] * sz;
+        po.g[a] = (a * P - sh) * es;
+        po.ge[a] = (D3Q7::opp[a] * P - sh) * es;
+        po.gw[a] = (a * P + sh) * es;
+    }
     po.fast = (uint64_t)P * (uint64_t)es < ((uint64_t)1 << 32);
     return po;
 }
