@@ -238,8 +238,8 @@ def main():
         print(json.dumps(full_config(a.full_config)), flush=True)
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); the solver "
-              "runs a single domain per device and has no multi-GPU path.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (got --gpus {a.gpus}); the solver runs a single "
+              "domain per device and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
     import rblbm
