@@ -582,12 +582,49 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
     return LBM_OK;
 }
 
+// Runs of many steps on a small domain replay a HIP graph of kReplaySteps steps: there a step is
+// launch-bound (a 257 x 129 2D step is two launches of a few microseconds).  kReplaySteps is
+// even, so every alternating buffer, in-place state and history buffer is back in its place
+// after each replay; the first step after an initialisation reads the plain state and runs
+// eagerly.  The graph holds the same launches on the same buffers as the eager loop.
+constexpr int kReplaySteps = 32;
+constexpr uint64_t kReplayNodes = (uint64_t)1 << 22;
+
+static int enqueue_steps_replayed(lbm_handle *h, int32_t nsteps)
+{
+    if (h->capturing || h->in_batch || h->N > kReplayNodes || nsteps < 4 * kReplaySteps)
+        return enqueue_steps(h, nsteps);
+    int rc = LBM_OK;
+    if (h->phase == 0 || h->first_thermal) {
+        rc = enqueue_steps(h, 1);
+        if (rc) return rc;
+        --nsteps;
+    }
+    const int replays = nsteps / kReplaySteps;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    const int64_t itc0 = h->itc;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    h->capturing = true;
+    rc = enqueue_steps(h, kReplaySteps);
+    h->capturing = false;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (!rc && e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    for (int r = 0; r < replays && !rc && e == hipSuccess; ++r) e = hipGraphLaunch(ge, h->stream);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(h, LBM_ERR_HIP, "step graph: %s", hipGetErrorString(e));
+    h->itc = itc0 + (int64_t)replays * kReplaySteps;   // the capture advanced the host state once
+    return enqueue_steps(h, nsteps - replays * kReplaySteps);
+}
+
 extern "C" int lbm_step(lbm_handle *h, int32_t nsteps)
 {
     if (!h || nsteps < 0) return LBM_ERR_ARG;
     if (h->phase < 0) return fail(h, LBM_ERR_STATE, "lbm_step before initialisation");
     HIPCHK(h, hipSetDevice(h->cfg.device));
-    return enqueue_steps(h, nsteps);
+    return enqueue_steps_replayed(h, nsteps);
 }
 
 extern "C" int lbm_step_timed(lbm_handle *h, int32_t nsteps, double *ms)
@@ -600,7 +637,7 @@ extern "C" int lbm_step_timed(lbm_handle *h, int32_t nsteps, double *ms)
         if (rc) return rc;
     }
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    int rc = enqueue_steps(h, nsteps);
+    int rc = enqueue_steps_replayed(h, nsteps);
     if (rc) return rc;
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     HIPCHK(h, hipEventSynchronize(h->ev1));
