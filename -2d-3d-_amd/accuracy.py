@@ -12,7 +12,7 @@ The FP64 form is bitwise identical to the reference's CPU path (tests/test_gpu_p
 it stands in for the CPU oracle at sizes the CPU cannot run.
 
 Example (512^3, Ra=1e8, 20,000 steps):
-    python -2d-3d-_amd/accuracy.py --n 512 --ra 1e8 --steps 20000 --every 500 --forms fp32
+    python ./-2d-3d-_amd/accuracy.py --n 512 --ra 1e8 --steps 20000 --every 500 --forms fp32
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 """Build the gfx950 HIP library librblbm.so (the C ABI of include/lbm_rb.h) in-tree.
 
-Usage: python -2d-3d-_amd/build.py   (or __graft_entry__.build()).  Always recompiles.
+Usage: python ./-2d-3d-_amd/build.py   (or __graft_entry__.build()).  Always recompiles.
 """
 from __future__ import annotations
 

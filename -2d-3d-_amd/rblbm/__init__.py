@@ -76,7 +76,7 @@ def library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FileNotFoundError(
-            f"{path} is missing: build it with `python -2d-3d-_amd/build.py` "
+            f"{path} is missing: build it with `python ./-2d-3d-_amd/build.py` "
             "(or __graft_entry__.build()); there is no fallback path")
     L = C.CDLL(path)
     dp = C.POINTER(C.c_double)

@@ -8,7 +8,7 @@ depend on the form (the FP64 form is bitwise identical in all of them).  lbm_aut
 same comparison on a scratch domain; this script lists every candidate.
 
 Example:
-    python -2d-3d-_amd/sweep.py --n 512 --forms fp64,fp32,fp16 --kchunks 8,16 --out sweep.json
+    python ./-2d-3d-_amd/sweep.py --n 512 --forms fp64,fp32,fp16 --kchunks 8,16 --out sweep.json
 """
 from __future__ import annotations
 
