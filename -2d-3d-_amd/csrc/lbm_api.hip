@@ -214,17 +214,21 @@ static void fused_fill(const lbm_handle *h, void *dst, const FusedPtrs &p, unsig
 
 // One-pass step of a single domain: launch from the handle's two-entry table, rebuilding it
 // when the buffers are not in either state it describes.
+// Entry of the one-pass launch table that matches the domain's buffers (-1: out of date).
+static int fused_table_parity(const lbm_handle *h)
+{
+    if (!h->ftab_ok) return -1;
+    // a one-pass step flips F and G, and swaps the two history buffers when use_g
+    const bool bp_same = h->Bp == h->ftab_bp;
+    if (h->fcur == h->ftab_fcur && h->gcur == h->ftab_gcur && bp_same) return 0;
+    if (h->fcur != h->ftab_fcur && h->gcur != h->ftab_gcur && bp_same != (bool)h->cfg.use_g) return 1;
+    return -1;
+}
+
 static int launch_fused_single(lbm_handle *h)
 {
     const size_t isz = fused_entry_bytes(h);
-    int par = -1;
-    if (h->ftab_ok) {
-        // a one-pass step flips F and G, and swaps the two history buffers when use_g
-        const bool bp_same = h->Bp == h->ftab_bp;
-        if (h->fcur == h->ftab_fcur && h->gcur == h->ftab_gcur && bp_same) par = 0;
-        else if (h->fcur != h->ftab_fcur && h->gcur != h->ftab_gcur && bp_same != (bool)h->cfg.use_g)
-            par = 1;
-    }
+    int par = fused_table_parity(h);
     if (par < 0) {
         if (h->capturing)   // the rebuild below synchronises: not inside a capture
             return fail(h, LBM_ERR_STATE, "one-pass launch table out of date during graph capture");
@@ -590,16 +594,27 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
 constexpr int kReplaySteps = 32;
 constexpr uint64_t kReplayNodes = (uint64_t)1 << 22;
 
+// A step can be captured once the domain has left its plain state (the first step after an
+// initialisation runs the two-pass form from it) and, for the one-pass form, once the launch
+// table matches the buffers (rebuilding it synchronises, which a capture cannot).
+static bool capture_ready(const lbm_handle *h)
+{
+    if (h->phase == 0 || h->first_thermal) return false;
+    const bool one_pass = !h->aa && h->fused_ok && h->variant == LBM_KERNEL_ONE_PASS;
+    return !one_pass || fused_table_parity(h) >= 0;
+}
+
 static int enqueue_steps_replayed(lbm_handle *h, int32_t nsteps)
 {
     if (h->capturing || h->in_batch || h->N > kReplayNodes || nsteps < 4 * kReplaySteps)
         return enqueue_steps(h, nsteps);
     int rc = LBM_OK;
-    if (h->phase == 0 || h->first_thermal) {
+    for (int k = 0; k < 2 && !capture_ready(h); ++k) {   // at most two eager steps
         rc = enqueue_steps(h, 1);
         if (rc) return rc;
         --nsteps;
     }
+    if (!capture_ready(h)) return enqueue_steps(h, nsteps);
     const int replays = nsteps / kReplaySteps;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
@@ -1209,10 +1224,16 @@ extern "C" int lbm_run_diagnostics(lbm_handle *h, int32_t nsteps, int32_t every,
         int rc = enqueue_steps(h, every);
         return rc ? rc : diag_append(h, ser.buf, ns, ser.count);
     };
-    // the first period eagerly: it leaves the plain state and settles the buffer parities
+    // the first period eagerly: it leaves the plain state and settles the buffer parities (and
+    // a second one when that first period was a single two-pass step before one-pass steps)
     int rc = period();
     if (rc) return rc;
     int done = 1;
+    if (!capture_ready(h) && done < ns) {
+        rc = period();
+        if (rc) return rc;
+        ++done;
+    }
     const int P = graph_periods(every);
     const int replays = (ns - done) / P;
     if (replays > 0) {
