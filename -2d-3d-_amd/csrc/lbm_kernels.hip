@@ -457,6 +457,7 @@ __global__ void __launch_bounds__(256) k3_flow(KParams kp, const double *__restr
 // domain owner[b] and covers its nodes [(b - first_block) * 256, + 256) in linear order.
 struct TwoPassEntryF64 {
     KParams kp;
+    PullOffsets po;   // offset rows, decided per node (a block may span rows)
     const double *Fcur;
     double *Fnext;
     const double *Gin;
@@ -487,7 +488,7 @@ __global__ void __launch_bounds__(256) k3_thermal_batch(const TwoPassEntryF64 *_
     uint64_t c;
     if (!batch_node(d.kp, b - d.first_block, i, j, k, c)) return;
     thermal3_node<false, LAY_PULL>(d.kp, i, j, k, c, gptr(d.Fcur), gptr(d.Gin), gptr(d.Gout),
-                                   gptr(d.Bp), gptr(d.aux), nullptr);
+                                   gptr(d.Bp), gptr(d.aux), nullptr, &d.po);
 }
 
 __global__ void __launch_bounds__(256) k3_flow_batch(const TwoPassEntryF64 *__restrict__ tab,
@@ -499,7 +500,8 @@ __global__ void __launch_bounds__(256) k3_flow_batch(const TwoPassEntryF64 *__re
     int i, j, k;
     uint64_t c;
     if (!batch_node(d.kp, b - d.first_block, i, j, k, c)) return;
-    flow3_node<LAY_PULL>(d.kp, i, j, k, c, gptr(d.Fcur), gptr(d.Fnext), gptr(d.Gout), gptr(d.aux));
+    flow3_node<LAY_PULL>(d.kp, i, j, k, c, gptr(d.Fcur), gptr(d.Fnext), gptr(d.Gout), gptr(d.aux),
+                         &d.po);
 }
 
 size_t twopass_entry_bytes_f64() { return sizeof(TwoPassEntryF64); }
@@ -509,6 +511,7 @@ void twopass_entry_fill_f64(void *dst, const KParams &kp, const FusedPtrs &p, do
 {
     TwoPassEntryF64 d;
     d.kp = kp;
+    d.po = pull_offsets(kp, 8);
     d.Fcur = (const double *)p.Fcur;
     d.Fnext = (double *)p.Fnext;
     d.Gin = (const double *)p.Gin;

@@ -598,6 +598,7 @@ __global__ void __launch_bounds__(256) lp3_flow(KParams kp, LPParams lp, const S
 template <typename S> struct TwoPassEntryLP {
     KParams kp;
     LPParams lp;
+    PullOffsets po;   // offset rows, decided per node (a block may span rows)
     const S *Fcur;
     S *Fnext;
     const S *Gin;
@@ -629,7 +630,7 @@ __global__ void __launch_bounds__(256) lp3_thermal_batch(const TwoPassEntryLP<S>
     uint64_t c;
     if (!batch_node_lp(d.kp, b - d.first_block, i, j, k, c)) return;
     lp3_thermal_node<S, false, LAY_PULL>(d.kp, d.lp, i, j, k, c, gptr(d.Fcur), gptr(d.Gin), gptr(d.Gout),
-                               gptr(d.Bp), gptr(d.aux), nullptr);
+                                         gptr(d.Bp), gptr(d.aux), nullptr, &d.po);
 }
 
 template <typename S>
@@ -643,7 +644,7 @@ __global__ void __launch_bounds__(256) lp3_flow_batch(const TwoPassEntryLP<S> *_
     uint64_t c;
     if (!batch_node_lp(d.kp, b - d.first_block, i, j, k, c)) return;
     lp3_flow_node<S, LAY_PULL>(d.kp, d.lp, i, j, k, c, gptr(d.Fcur), gptr(d.Fnext), gptr(d.Gout),
-                     gptr(d.aux));
+                               gptr(d.aux), &d.po);
 }
 
 // Reference state (absolute FP64) from the pipelined buffers; see k3_state.
@@ -1059,6 +1060,7 @@ static void fill_twopass_lp(void *dst, const KParams &kp, const FusedPtrs &p, fl
     TwoPassEntryLP<S> d;
     d.kp = kp;
     d.lp = lp_params(kp);
+    d.po = pull_offsets(kp, (int64_t)sizeof(S));
     d.Fcur = (const S *)p.Fcur;
     d.Fnext = (S *)p.Fnext;
     d.Gin = (const S *)p.Gin;
