@@ -625,12 +625,18 @@ static int enqueue_steps_replayed(lbm_handle *h, int32_t nsteps)
     h->capturing = false;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (!rc && e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    for (int r = 0; r < replays && !rc && e == hipSuccess; ++r) e = hipGraphLaunch(ge, h->stream);
+    int launched = 0;
+    for (; launched < replays && !rc && e == hipSuccess; ++launched) {
+        e = hipGraphLaunch(ge, h->stream);
+        if (e != hipSuccess) break;
+    }
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
+    // the capture advanced the host state by an even number of steps (buffers back in place);
+    // the step counter follows the replays that were launched
+    h->itc = itc0 + (int64_t)launched * kReplaySteps;
     if (rc) return rc;
     if (e != hipSuccess) return fail(h, LBM_ERR_HIP, "step graph: %s", hipGetErrorString(e));
-    h->itc = itc0 + (int64_t)replays * kReplaySteps;   // the capture advanced the host state once
     return enqueue_steps(h, nsteps - replays * kReplaySteps);
 }
 
@@ -1246,13 +1252,18 @@ extern "C" int lbm_run_diagnostics(lbm_handle *h, int32_t nsteps, int32_t every,
         h->capturing = false;
         hipError_t e = hipStreamEndCapture(h->stream, &g);
         if (!rc && e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        // host bookkeeping now stands one replay ahead; every further replay adds P periods
-        for (int r = 0; r < replays && !rc && e == hipSuccess; ++r) e = hipGraphLaunch(ge, h->stream);
+        // host bookkeeping now stands one replay ahead (buffers back in place); the step counter
+        // follows the replays that were launched
+        int launched = 0;
+        for (; launched < replays && !rc && e == hipSuccess; ++launched) {
+            e = hipGraphLaunch(ge, h->stream);
+            if (e != hipSuccess) break;
+        }
         if (ge) (void)hipGraphExecDestroy(ge);
         if (g) (void)hipGraphDestroy(g);
+        h->itc = itc0 + (int64_t)launched * P * every;
         if (rc) return rc;
         if (e != hipSuccess) return fail(h, LBM_ERR_HIP, "diagnostics graph: %s", hipGetErrorString(e));
-        h->itc = itc0 + (int64_t)replays * P * every;
         done += replays * P;
     }
     for (; done < ns; ++done) {
