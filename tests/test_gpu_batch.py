@@ -134,3 +134,25 @@ def test_lowprec_domain_kernels_equal_table_kernels_on_every_wall(prec):
             np.testing.assert_array_equal(m1[k], m2[k], err_msg=f"{WALLS[n]} {k}")
         for k in ("f", "g"):
             np.testing.assert_array_equal(s1[k], s2[k], err_msg=f"{WALLS[n]} {k}")
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT, rblbm.FP64])
+def test_wide_rows_domain_kernels_equal_table_kernels(prec):
+    # rows of 160 and 300 nodes: the per-row kernels' last block of a row has idle threads
+    # (192 threads for 160 nodes; 256 + 44 for 300); the table kernels' 256-node blocks do not
+    # follow rows
+    cfgs = [rblbm.Config(dims=3, nx=nx, ny=10, nz=8, rayleigh=1e6, precision=prec)
+            for nx in (160, 300)]
+
+    def run(mode):
+        b = rblbm.Batch(cfgs)
+        b.set_mode(mode)
+        b.set_kernel_variant(rblbm.KERNEL_TWO_PASS)
+        b.initial_seeded(23, 0.05)
+        b.step(9)
+        out = [d.gather(with_bprev=False) for d in b.domains]
+        b.close()
+        return out
+    for s1, s2 in zip(run(rblbm.BATCH_PER_DOMAIN), run(rblbm.BATCH_ONE_LAUNCH)):
+        for k in ("f", "g"):
+            np.testing.assert_array_equal(s1[k], s2[k])
