@@ -452,15 +452,25 @@ static int do_init(lbm_handle *h, const double *T0, uint64_t seed, double amp, b
     return LBM_OK;
 }
 
+// A domain of a batch changes state only through the lbm_batch_* calls: its buffers are also
+// described by the batch's launch tables, which a step, initialisation or scatter of the domain
+// alone would leave out of date.
+static int refuse_batch_domain(lbm_handle *h, const char *what)
+{
+    return fail(h, LBM_ERR_STATE, "%s on a domain of a batch: use the lbm_batch_* calls", what);
+}
+
 extern "C" int lbm_init_exact(lbm_handle *h, const double *T0)
 {
     if (!h) return LBM_ERR_ARG;
+    if (h->in_batch) return refuse_batch_domain(h, "lbm_init_exact");
     return do_init(h, T0, 0, 0.0, false);
 }
 
 extern "C" int lbm_init_seeded(lbm_handle *h, uint64_t seed, double amp)
 {
     if (!h || !(amp >= 0.0)) return LBM_ERR_ARG;
+    if (h->in_batch) return refuse_batch_domain(h, "lbm_init_seeded");
     return do_init(h, nullptr, seed, amp, true);
 }
 
@@ -643,6 +653,7 @@ static int enqueue_steps_replayed(lbm_handle *h, int32_t nsteps)
 extern "C" int lbm_step(lbm_handle *h, int32_t nsteps)
 {
     if (!h || nsteps < 0) return LBM_ERR_ARG;
+    if (h->in_batch) return refuse_batch_domain(h, "lbm_step");
     if (h->phase < 0) return fail(h, LBM_ERR_STATE, "lbm_step before initialisation");
     HIPCHK(h, hipSetDevice(h->cfg.device));
     return enqueue_steps_replayed(h, nsteps);
@@ -651,6 +662,7 @@ extern "C" int lbm_step(lbm_handle *h, int32_t nsteps)
 extern "C" int lbm_step_timed(lbm_handle *h, int32_t nsteps, double *ms)
 {
     if (!h || nsteps < 0 || !ms) return LBM_ERR_ARG;
+    if (h->in_batch) return refuse_batch_domain(h, "lbm_step_timed");
     if (h->phase < 0) return fail(h, LBM_ERR_STATE, "lbm_step before initialisation");
     HIPCHK(h, hipSetDevice(h->cfg.device));
     if (h->phase == 0) {            // keep the one-off prologue out of the timed region
@@ -1800,6 +1812,7 @@ extern "C" int lbm_gather(lbm_handle *h, double *f, double *g, double *bprev)
 extern "C" int lbm_scatter(lbm_handle *h, const double *f, const double *g)
 {
     if (!h || !f || !g) return LBM_ERR_ARG;
+    if (h->in_batch) return refuse_batch_domain(h, "lbm_scatter");
     HIPCHK(h, hipSetDevice(h->cfg.device));
     const int nk = chunk_planes(h);
     const size_t plane = (size_t)h->cfg.nx * h->cfg.ny;
@@ -1969,8 +1982,9 @@ static void batch_invalidate(lbm_batch *b)
 extern "C" int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp)
 {
     if (!b) return LBM_ERR_ARG;
+    if (!(amp >= 0.0)) return LBM_ERR_ARG;
     for (int i = 0; i < b->n; ++i) {
-        int rc = lbm_init_seeded(b->dom[i], seed, amp);
+        int rc = do_init(b->dom[i], nullptr, seed, amp, true);
         if (rc) return rc;
     }
     batch_invalidate(b);
@@ -2197,9 +2211,22 @@ extern "C" int lbm_batch_run_diagnostics(lbm_batch *b, int32_t nsteps, int32_t e
     if (!rc) rc = diag_all();
     if (rc) return rc;
     int done = 1;
+    const bool table = b->mode != LBM_BATCH_PER_DOMAIN;
+    // one launch per domain: a second eager period when that first one was a single two-pass
+    // step before one-pass steps (the domains' one-pass launch tables are built eagerly)
+    auto all_ready = [&]() {
+        for (int i = 0; i < n; ++i)
+            if (!capture_ready(b->dom[i])) return false;
+        return true;
+    };
+    if (!table && done < ns && !all_ready()) {
+        rc = batch_enqueue(b, every);
+        if (!rc) rc = diag_all();
+        if (rc) return rc;
+        ++done;
+    }
     const int P = graph_periods(every);
     const int replays = (ns - done) / P;
-    const bool table = b->mode != LBM_BATCH_PER_DOMAIN;
     if (replays > 0 && table && !b->tables_ready) rc = batch_tables(b);   // outside the capture
     if (rc) return rc;
     if (replays > 0) {
@@ -2224,12 +2251,18 @@ extern "C" int lbm_batch_run_diagnostics(lbm_batch *b, int32_t nsteps, int32_t e
         for (int i = 0; i < n; ++i) b->dom[i]->capturing = false;
         hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (!rc && e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        for (int r = 0; r < replays && !rc && e == hipSuccess; ++r) e = hipGraphLaunch(ge, b->stream);
+        // host bookkeeping now stands one replay ahead (buffers back in place); the step
+        // counters follow the replays that were launched
+        int launched = 0;
+        for (; launched < replays && !rc && e == hipSuccess; ++launched) {
+            e = hipGraphLaunch(ge, b->stream);
+            if (e != hipSuccess) break;
+        }
         if (ge) (void)hipGraphExecDestroy(ge);
         if (g) (void)hipGraphDestroy(g);
+        for (int i = 0; i < n; ++i) b->dom[i]->itc = itc0[i] + (int64_t)launched * P * every;
         if (rc) return rc;
         if (e != hipSuccess) return fail(h0, LBM_ERR_HIP, "diagnostics graph: %s", hipGetErrorString(e));
-        for (int i = 0; i < n; ++i) b->dom[i]->itc = itc0[i] + (int64_t)replays * P * every;
         done += replays * P;
     }
     for (; done < ns; ++done) {

@@ -221,8 +221,12 @@ int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *m
 
 /* ---- Batched ensembles (no reference counterpart: the reference runs one cavity per
  * process).  A batch owns n independent domains of any shape and parameters; they share one
- * HIP stream and advance together.  Domain i is an ordinary handle for every per-domain call
- * (init, diagnostics, macros, gather); its seeded noise uses i as the instance key.
+ * HIP stream and advance together.  Domain i is an ordinary handle for the per-domain calls that
+ * read it (diagnostics, check, macros, gather, mid-plane and slice output); its seeded noise uses
+ * i as the instance key.  The calls that change a domain's state on their own (lbm_step,
+ * lbm_step_timed, lbm_init_exact, lbm_init_seeded, lbm_scatter, lbm_set_kernel_variant,
+ * lbm_autotune, lbm_run_diagnostics, lbm_destroy) return LBM_ERR_STATE for a batch domain: the
+ * batch's launch tables describe its buffers, and only the lbm_batch_* calls keep them current.
  * The one-launch modes run the two-pass form (two launches a step over 256-node blocks of all
  * domains) unless every domain is set to LBM_KERNEL_ONE_PASS (lbm_batch_set_kernel_variant or
  * lbm_batch_autotune), then the one-pass form (one launch a step; the tile shape and prefetch

@@ -156,3 +156,28 @@ def test_wide_rows_domain_kernels_equal_table_kernels(prec):
     for s1, s2 in zip(run(rblbm.BATCH_PER_DOMAIN), run(rblbm.BATCH_ONE_LAUNCH)):
         for k in ("f", "g"):
             np.testing.assert_array_equal(s1[k], s2[k])
+
+
+def test_batch_domain_refuses_state_changes_of_its_own():
+    # stepping, initialising or scattering one domain of a batch would leave the batch's launch
+    # tables describing stale buffers: refused, and the batch stays bitwise on the oracle
+    b = rblbm.Batch(_cfgs(rblbm.FP64)[:2])
+    b.set_mode(rblbm.BATCH_ONE_LAUNCH)
+    b.initial_seeded(17, 0.05)
+    ora = _oracles(17, 0.05)[:2]
+    b.step(3)
+    d = b.domains[1]
+    st = d.gather()
+    for call in (lambda: d.step(1), lambda: d.step_timed(1), lambda: d.initial_seeded(1, 0.01),
+                 lambda: d.initial(), lambda: d.scatter(st["f"], st["g"]),
+                 lambda: d.run_diagnostics(2, 1)):
+        with pytest.raises(rblbm.LbmError) as e:
+            call()
+        assert e.value.code == -4
+    b.step(4)
+    for o in ora:
+        o.step(7)
+    for dd, o in zip(b.domains, ora):
+        np.testing.assert_array_equal(dd.gather()["f"], o.state()["f"])
+        np.testing.assert_array_equal(dd.gather()["g"], o.state()["g"])
+    b.close()

@@ -109,3 +109,34 @@ def test_batch_run_diagnostics_equals_eager(mode):
     for d, o in zip(bt.domains, ora):
         np.testing.assert_array_equal(d.gather()["f"], o.state()["f"])
     bt.close()
+
+
+@pytest.mark.parametrize("mode", [rblbm.BATCH_PER_DOMAIN, rblbm.BATCH_ONE_LAUNCH, rblbm.BATCH_GRAPH])
+def test_batch_run_diagnostics_one_pass_every_step(mode):
+    # every = 1 with one-pass domains: the first period is the single two-pass step from the
+    # plain state, after which each domain's one-pass launch table still has to be built before
+    # a period can be captured; samples and step counters must equal the oracle's
+    shapes = [(24, 20, 16, 1e5), (32, 24, 20, 1e6)]
+    cfgs = [rblbm.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=ra) for a, b, c, ra in shapes]
+    bt = rblbm.Batch(cfgs)
+    bt.set_mode(mode)
+    bt.set_kernel_variant(rblbm.KERNEL_ONE_PASS)
+    bt.initial_seeded(8, 0.05)
+    ora = []
+    for i, (a, b, c, ra) in enumerate(shapes):
+        o = orc.Oracle(orc.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=ra))
+        o.init_seeded(8, 0.05, instance=i)
+        ora.append(o)
+    series = bt.run_diagnostics(9, 1)
+    assert len(series) == 9
+    for k in range(9):
+        for o in ora:
+            o.step(1)
+        for i, o in enumerate(ora):
+            ref = o.diagnostics()
+            for key in ("nu_vol", "re_vol", "nu_global"):
+                assert series[k][i][key] == pytest.approx(ref[key], rel=1e-12, abs=1e-14), (k, i, key)
+    for d, o in zip(bt.domains, ora):
+        assert d.itc == 9
+        np.testing.assert_array_equal(d.gather()["f"], o.state()["f"])
+    bt.close()

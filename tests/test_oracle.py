@@ -517,3 +517,150 @@ def test_psi_vorticity_2d_closed_form():
     vort = -coef * a * (Y - 2 * yp)
     np.testing.assert_allclose(r["vort"], np.broadcast_to(vort[:, None], r["vort"].shape),
                                rtol=1e-9, atol=1e-9 * coef * a)
+
+
+# --------------------------------------------------------------------------------------------
+# check (2DRB/3DRB.F90:1475-1528, 2DRB/2DRB.F90:1213-1260), the exact initial state
+# (2DRB/3DRB.F90:517-600, 2DRB/2DRB.F90:540-608) and the kinetic-energy sum, each against values
+# the chosen fields fix in closed form
+# --------------------------------------------------------------------------------------------
+def _load(o, cfg, U, T, rho=1.0):
+    """Load f = f_eq(rho, U) node by node (U: (..., 3)), g = wT T; returns (u, v, w) the reload
+    gives in closed form: U, with the half force 0.5 g beta (T - Tref) added to v."""
+    m = Model(cfg.dims, cfg.nx, cfg.ny, cfg.nz)
+    U = np.broadcast_to(np.asarray(U, dtype=float), cfg.shape + (3,)).copy()
+    if cfg.dims == 2:
+        U[..., 2] = 0.0
+    T = np.broadcast_to(np.asarray(T, dtype=float), cfg.shape).copy()
+    assert o.set_fg(m.feq(np.full(cfg.shape, rho), U), m.geq(T, np.zeros(cfg.shape + (3,)))) == 0
+    p = orc.params(cfg)
+    v = U[..., 1] + 0.5 * p["gbeta"] * (T - p["Tref"])
+    return U[..., 0], v, U[..., 2]
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_check_relative_l2_velocity_and_l1_temperature(dims):
+    cfg = orc.Config(dims=dims, nx=6, ny=5, nz=4, rayleigh=1e5, thot=1.0, tcold=0.0)
+    tref = orc.params(cfg)["Tref"]
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    # first call: the previous fields are zero, so both errors are 1 exactly
+    U1 = np.array([0.01, -0.02, 0.005])
+    _load(o, cfg, U1, tref)
+    eu, et = o.check()
+    assert eu == pytest.approx(1.0, rel=1e-14) and et == pytest.approx(1.0, rel=1e-14)
+    # second call: half the nodes (a checkerboard) move by dU, the temperature by eps on the
+    # other half, at T = Tref + eps there the reload adds the half force to v
+    k, j, i = np.indices(cfg.shape)
+    odd = (i + j + k) % 2 == 1
+    n_odd, n_even = int(odd.sum()), int((~odd).sum())
+    dU = np.array([0.003, 0.004, -0.012])
+    if dims == 2:
+        U1 = U1 * np.array([1, 1, 0])
+        dU = dU * np.array([1, 1, 0])
+    eps = 0.25
+    U2 = np.where(odd[..., None], U1 + dU, U1)
+    T2 = np.where(odd, tref, tref + eps)
+    u, v, w = _load(o, cfg, U2, T2)
+    eu, et = o.check()
+    du = np.where(odd[..., None], dU, np.zeros(3))
+    dv_force = np.where(odd, 0.0, 0.5 * orc.params(cfg)["gbeta"] * eps)
+    num = (du[..., 0] ** 2 + (du[..., 1] + dv_force) ** 2 + du[..., 2] ** 2).sum()
+    den = (u ** 2 + v ** 2 + w ** 2).sum()
+    assert eu == pytest.approx(np.sqrt(num) / np.sqrt(den), rel=1e-12)
+    # L1 ratio: sum |T2 - T1| / sum |T2| = n_even eps / (n_odd Tref + n_even (Tref + eps))
+    assert et == pytest.approx(n_even * eps / (n_odd * tref + n_even * (tref + eps)), rel=1e-13)
+    # an L2 form of the temperature error would give a different number on these fields
+    l2 = np.sqrt(n_even) * eps / np.sqrt(n_odd * tref ** 2 + n_even * (tref + eps) ** 2)
+    assert abs(et - l2) > 1e-3
+    # a third call on unchanged fields: zero change
+    _load(o, cfg, U2, T2)
+    eu, et = o.check()
+    assert eu == 0.0 and et == 0.0
+
+
+def test_check_zero_field_guard_3d_only():
+    # 3D: with u = 0 and T = 0 the denominators vanish and the reference reports the plain norms
+    # sqrt(sum |du|^2) and sum |dT| (2DRB/3DRB.F90:1513-1522); 2D has no guard
+    # (2DRB/2DRB.F90:1244-1245), so the velocity ratio divides by zero
+    for dims in (3, 2):
+        cfg = orc.Config(dims=dims, nx=6, ny=5, nz=4, rayleigh=1e5, thot=0.5, tcold=-0.5)
+        n = int(np.prod(cfg.shape))
+        o = orc.Oracle(cfg)
+        o.init_exact()
+        U1 = np.array([0.01, -0.02, 0.005 if dims == 3 else 0.0])
+        T1 = 0.125
+        u, v, w = _load(o, cfg, U1, T1)
+        o.check()
+        _load(o, cfg, np.zeros(3), 0.0)   # T = Tref = 0: no half force, u = 0 everywhere
+        with np.errstate(divide="ignore"):
+            eu, et = o.check()
+        if dims == 3:
+            assert eu == pytest.approx(np.sqrt(((u ** 2 + v ** 2 + w ** 2)).sum()), rel=1e-13)
+            assert eu == pytest.approx(np.sqrt(n * (U1[0] ** 2 + v.flat[0] ** 2 + U1[2] ** 2)), rel=1e-13)
+            assert et == pytest.approx(n * T1, rel=1e-14)
+        else:
+            assert np.isinf(eu)
+            assert np.isinf(et)
+
+
+@pytest.mark.parametrize("dims,ra,perturbed", [(3, 1e5, False), (3, 1e4, True), (3, 1.0001e4, False),
+                                               (3, 3e3, True), (2, 1e4, False), (2, 9999.0, True),
+                                               (2, 1e5, False)])
+def test_exact_initial_state_closed_form(dims, ra, perturbed):
+    # T = Thot + (j - 1/2)/ny (Tcold - Thot) at the cell centres (L = ny for the RB cell); the
+    # perturbation 1e-3 dT cos(2 pi x/X) sin(pi y/Y) (3D, Ra <= 1e4) or sin(2 pi x/X) sin(pi y/Y)
+    # (2D, Ra < 1e4) with x = (i - 1/2)/L, X = nx/L, y = (j - 1/2)/L, Y = ny/L; rho = 1, u = 0,
+    # f = w, g = wT T, B_prev = 0
+    cfg = orc.Config(dims=dims, nx=10, ny=7, nz=5, rayleigh=ra, thot=1.5, tcold=-0.25)
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    s = o.state()
+    k, j, i = np.indices(cfg.shape)
+    dT = cfg.thot - cfg.tcold
+    T = cfg.thot + (j + 0.5) / cfg.ny * (cfg.tcold - cfg.thot)
+    if perturbed:
+        trig = np.cos if dims == 3 else np.sin
+        T = T + 1e-3 * dT * trig(2 * np.pi * (i + 0.5) / cfg.nx) * np.sin(np.pi * (j + 0.5) / cfg.ny)
+    np.testing.assert_allclose(s["T"], T, rtol=0, atol=4e-16)
+    if perturbed:   # the perturbation is really there (not a no-op within the tolerance)
+        assert np.abs(s["T"] - (cfg.thot + (j + 0.5) / cfg.ny * (cfg.tcold - cfg.thot))).max() > 5e-4
+    np.testing.assert_array_equal(s["rho"], 1.0)
+    for key in ("u", "v", "w", "bx", "by", "bz"):
+        np.testing.assert_array_equal(s[key], 0.0)
+    wf = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12) if dims == 3 else \
+        np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    wt = np.array([0.25] + [0.125] * 6) if dims == 3 else np.array([1 / 3] + [1 / 6] * 4)
+    np.testing.assert_allclose(s["f"], np.broadcast_to(wf, s["f"].shape), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(s["g"], T[..., None] * wt, rtol=1e-15, atol=1e-17)
+
+
+def test_exact_initial_state_side_heated_profile():
+    # VerticalWallsConstT: the profile runs along x between the hot (x = 0) and cold walls, L = nx
+    cfg = orc.Config(dims=3, nx=9, ny=6, nz=5, rayleigh=1e3, thot=1.0, tcold=0.0, side_heated=1,
+                     temp_x=orc.T_CONST, temp_y=orc.T_ADIABATIC)
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    k, j, i = np.indices(cfg.shape)
+    np.testing.assert_allclose(o.state()["T"], 1.0 - (i + 0.5) / cfg.nx, rtol=0, atol=4e-16)
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("dims", [3, 2])
+def test_kinetic_energy_sum_closed_form(dims, side):
+    # kinetic = 0.5 sum rho |u|^2: a uniform flow at density rho0 through a uniform T (the reload
+    # adds the half force to v), so the sum is 0.5 rho0 N (Ux^2 + v^2 + Uz^2); the RB and the
+    # side-heated cells sum it in separate routines
+    extra = dict(side_heated=1, temp_x=orc.T_CONST, temp_y=orc.T_ADIABATIC) if side else {}
+    cfg = orc.Config(dims=dims, nx=7, ny=6, nz=5, rayleigh=1e5, thot=1.0, tcold=0.0, **extra)
+    n = int(np.prod(cfg.shape))
+    o = orc.Oracle(cfg)
+    o.init_exact()
+    rho0 = 1.2
+    U = np.array([0.02, -0.01, 0.015])
+    u, v, w = _load(o, cfg, U, 0.75, rho=rho0)
+    d = o.diagnostics()
+    uz = U[2] if dims == 3 else 0.0
+    assert d["kinetic"] == pytest.approx(0.5 * rho0 * n * (U[0] ** 2 + v.flat[0] ** 2 + uz ** 2), rel=1e-12)
+    assert d["mass"] == pytest.approx(rho0 * n, rel=1e-14)
+    assert d["heat"] == pytest.approx(0.75 * n, rel=1e-14)
