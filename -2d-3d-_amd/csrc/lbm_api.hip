@@ -170,17 +170,24 @@ static double node_bytes(const lbm_config &c, bool aa, bool fused)
     return (aa ? 1 : 2) * q * es + (fused ? 6 : 3) * ea + (c.dims == 2 ? 2 : 1) * ea + 40.0;
 }
 
-// One-pass launch geometry for z-chunks of `kchunk` planes and thread tile shape `tile`.
+// Default z-chunk of the one-pass form.  Measured at 512^3 FP64 with the 34x15 tile (XCD-remapped
+// order: each XCD's blocks a contiguous z-slab): 16.8 ms per step with chunks of 8 planes, 14.9 at
+// 32, 15.2 at 64, 16.1 at 128, 18.6 at 256.
+static int default_kchunk(int nz) { return nz >= 32 ? 32 : nz; }
+
+// One-pass launch geometry for z-chunks of `kchunk` planes (LBM_KCHUNK_AUTO: default_kchunk)
+// and thread tile shape `tile`.
 static void set_fused_geom(lbm_handle *h, int kchunk, int tile, int prefetch)
 {
     int tx, ty;
     fused_tile_dims(tile, &tx, &ty);
     FusedGeom &g = h->fg;
-    g.kchunk = kchunk > h->cfg.nz ? h->cfg.nz : kchunk;
     g.tile = tile;
     g.pf = prefetch;
     g.ntx = (unsigned)((h->cfg.nx + tx - 3) / (tx - 2));
     g.nty = (unsigned)((h->cfg.ny + ty - 3) / (ty - 2));
+    if (kchunk <= 0) kchunk = default_kchunk(h->cfg.nz);
+    g.kchunk = kchunk > h->cfg.nz ? h->cfg.nz : kchunk;
     g.nzc = (unsigned)((h->cfg.nz + g.kchunk - 1) / g.kchunk);
     g.nblocks = g.ntx * g.nty * g.nzc;
     h->ftab_ok = false;
@@ -234,12 +241,12 @@ static int launch_fused_single(lbm_handle *h)
             return fail(h, LBM_ERR_STATE, "one-pass launch table out of date during graph capture");
         if (!h->ftab && hipMalloc(&h->ftab, 2 * isz) != hipSuccess)
             return fail(h, LBM_ERR_NOMEM, "one-pass launch table");
-        std::vector<unsigned char> host(2 * isz);
-        for (int q = 0; q < 2; ++q)
-            fused_fill(h, host.data() + q * isz, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, q), 0);
         // synchronous: the table is rebuilt only after an initialisation, a two-pass step or a
         // change of the kernel form, never inside a run of one-pass steps
         HIPCHK(h, hipStreamSynchronize(h->stream));
+        std::vector<unsigned char> host(2 * isz);
+        for (int q = 0; q < 2; ++q)
+            fused_fill(h, host.data() + q * isz, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, q), 0);
         HIPCHK(h, hipMemcpy(h->ftab, host.data(), 2 * isz, hipMemcpyHostToDevice));
         h->ftab_ok = true;
         h->ftab_fcur = h->fcur;
@@ -337,7 +344,9 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     }
     h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC && !h->aa;
     h->variant = LBM_KERNEL_AUTO;
-    set_fused_geom(h, c.nz >= 256 ? 64 : (c.nz >= 64 ? 32 : c.nz), 0, 0);
+    // the one-pass form's default shape: the 34x15 tile with prefetching, the fastest one-pass
+    // form at 512^3 in FP64 (the reduced-precision forms run the two-pass form under AUTO)
+    set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_34X15, 1);
 
     int rc = LBM_OK;
     hipError_t e = hipSetDevice(c.device);
@@ -474,6 +483,19 @@ extern "C" int lbm_init_seeded(lbm_handle *h, uint64_t seed, double amp)
     return do_init(h, nullptr, seed, amp, true);
 }
 
+// The kernel form a step of domain h runs (after its first step, which is always two-pass):
+// ONE_PASS as requested; under AUTO the one-pass form for FP64 domains of 2^24 nodes or more
+// (1.1x faster than the two-pass form at 512^3), else the two-pass form: the reduced-precision
+// one-pass kernel is 1.3-1.8x slower at 512^3, and a smaller domain has too few one-pass tiles
+// to fill the device.  The one-pass form needs the configuration to admit it.
+constexpr uint64_t kOnePassNodes = (uint64_t)1 << 24;
+static bool runs_one_pass(const lbm_handle *h)
+{
+    if (!h->fused_ok) return false;
+    if (h->variant == LBM_KERNEL_ONE_PASS) return true;
+    return h->variant == LBM_KERNEL_AUTO && !h->lp && h->N >= kOnePassNodes;
+}
+
 // In-place layout.  A step is the flow collision (pass B) of the step before, then pass A, so
 // that between steps F[0] and G[0] hold f_post^n and g_post^n (both in state fpar == gpar) and
 // the reference state f^{n+1}, g^{n+1} can still be read from them.  The first step has no
@@ -561,7 +583,7 @@ static int enqueue_steps(lbm_handle *h, int32_t nsteps)
         // the first step after (re)initialisation reads the plain state: two-pass form.  AUTO
         // runs the two-pass form, the faster one at 512^3 on MI355X in every precision form;
         // lbm_autotune measures both per shape.
-        const bool fused = h->fused_ok && !h->first_thermal && h->variant == LBM_KERNEL_ONE_PASS;
+        const bool fused = !h->first_thermal && runs_one_pass(h);
         if (fused) {
             int rc = launch_fused_single(h);
             if (rc) return rc;
@@ -610,7 +632,7 @@ constexpr uint64_t kReplayNodes = (uint64_t)1 << 22;
 static bool capture_ready(const lbm_handle *h)
 {
     if (h->phase == 0 || h->first_thermal) return false;
-    const bool one_pass = !h->aa && h->fused_ok && h->variant == LBM_KERNEL_ONE_PASS;
+    const bool one_pass = !h->aa && runs_one_pass(h);
     return !one_pass || fused_table_parity(h) >= 0;
 }
 
@@ -713,8 +735,8 @@ extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
 // Kernel form of one domain (no batch check: the batch code retunes its own domains).
 static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefetch)
 {
-    if (variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < 0 ||
-        tile < LBM_TILE_32X16 || tile > LBM_TILE_32X32 || prefetch < 0 || prefetch > 1)
+    if (variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < LBM_KCHUNK_AUTO ||
+        tile < LBM_TILE_32X16 || tile > LBM_TILE_34X15 || prefetch < 0 || prefetch > 1)
         return LBM_ERR_ARG;
     if (tile == LBM_TILE_32X32 && !h->lp)
         return fail(h, LBM_ERR_UNSUPPORTED, "the 32x32 tile needs a reduced-precision form");
@@ -722,7 +744,7 @@ static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefet
         return fail(h, LBM_ERR_UNSUPPORTED,
                     "one-pass form needs 3D, non-periodic thermal walls and two population arrays");
     h->variant = variant;
-    set_fused_geom(h, kchunk > 0 ? kchunk : h->fg.kchunk, tile, prefetch);
+    set_fused_geom(h, kchunk != 0 ? kchunk : h->fg.kchunk, tile, prefetch);
     return LBM_OK;
 }
 
@@ -753,7 +775,7 @@ extern "C" int lbm_get_kernel_form(lbm_handle *h, int32_t *variant, int32_t *kch
                                    int32_t *prefetch)
 {
     if (!h) return LBM_ERR_ARG;
-    if (variant) *variant = h->variant;
+    if (variant) *variant = runs_one_pass(h) ? LBM_KERNEL_ONE_PASS : LBM_KERNEL_TWO_PASS;
     if (kchunk) *kchunk = h->fg.kchunk;
     if (tile) *tile = h->fg.tile;
     if (prefetch) *prefetch = h->fg.pf;
@@ -883,6 +905,10 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
         rc = lbm_init_seeded(t, 1, 1e-3);
         if (!rc) rc = lbm_step(t, 1);   // leave the plain state: the candidates time steady steps
         std::vector<TuneChoice> cands;
+        // the incumbent: the form AUTO runs in this layout (its default one-pass shape for large
+        // FP64 domains)
+        if (!aa && t->fused_ok && !t->lp && h->N >= kOnePassNodes)
+            cands.push_back({LBM_KERNEL_ONE_PASS, t->fg.kchunk, t->fg.tile, t->fg.pf, 0});
         cands.push_back({LBM_KERNEL_TWO_PASS, h->fg.kchunk, h->fg.tile, h->fg.pf, aa ? 1 : 0});
         if (t->fused_ok)
             for (int tile = 0; tile < FUSED_TILES; ++tile)

@@ -204,8 +204,11 @@ k3_fused(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restric
 }
 
 // Run-time tile shapes: index -> (TX, TY).
-constexpr int fused_tx(int tile) { return tile == 1 ? 16 : 32; }
-constexpr int fused_ty(int tile) { return tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : 32)); }
+constexpr int fused_tx(int tile) { return tile == 1 ? 16 : (tile == 4 ? 34 : 32); }
+constexpr int fused_ty(int tile)
+{
+    return tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : (tile == 3 ? 32 : 15)));
+}
 
 // One launch over `nblocks` blocks of the domains in table `inst` (owner null: one domain).
 template <class Form>
@@ -223,6 +226,8 @@ hipError_t launch_fused_table(const FusedGeom &fg0, const void *inst, const uint
     case 3: RBLBM_FUSED_B(1, true); break;
     case 4: RBLBM_FUSED_B(2, false); break;
     case 5: RBLBM_FUSED_B(2, true); break;
+    case 8: RBLBM_FUSED_B(4, false); break;
+    case 9: RBLBM_FUSED_B(4, true); break;
     default:
         // 32 x 32 = 1024 threads leave 128 registers per lane: reduced-precision forms only
         if constexpr (sizeof(typename Form::R) == 4) {
@@ -236,5 +241,6 @@ hipError_t launch_fused_table(const FusedGeom &fg0, const void *inst, const uint
 #undef RBLBM_FUSED_B
     return hipGetLastError();
 }
+
 
 } // namespace rblbm

@@ -43,13 +43,14 @@ hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, 
                         double *mac, double *fo, double *go, int k0, int nk, int lay = 0);
 
 // One-pass form: tile of (TX - 2) x (TY - 2) nodes plus a one-node ring per block; the thread
-// tile TX x TY is one of four shapes (FusedGeom::tile): 32x16, 16x16, 32x8, and 32x32 for the
-// reduced-precision forms only.
-constexpr int FUSED_TILES = 4;
+// tile TX x TY is one of five shapes (FusedGeom::tile): 32x16, 16x16, 32x8, 32x32 (the
+// reduced-precision forms only) and 34x15 (32 x 13 nodes: rows of 32 nodes start on a 256-byte
+// boundary, and a 512-wide domain splits into 16 tiles across).
+constexpr int FUSED_TILES = 5;
 inline void fused_tile_dims(int tile, int *tx, int *ty)
 {
-    *tx = tile == 1 ? 16 : 32;
-    *ty = tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : 32));
+    *tx = tile == 1 ? 16 : (tile == 4 ? 34 : 32);
+    *ty = tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : (tile == 3 ? 32 : 15)));
 }
 hipError_t launch_init(const KParams &kp, int bx, hipStream_t s, const InitParams &ip, double *F,
                        double *G, double *Bp, double *mac, int qf, int qt);

@@ -136,7 +136,8 @@ __device__ __forceinline__ bool offset_row(const PullOffsets &po, int j, int k, 
 struct FusedGeom {
     unsigned ntx, nty, nzc, nblocks;
     int kchunk;
-    int tile;     // tile shape: 0 = 32 x 16 threads, 1 = 16 x 16, 2 = 32 x 8, 3 = 32 x 32
+    int tile;     // tile shape: 0 = 32 x 16 threads, 1 = 16 x 16, 2 = 32 x 8, 3 = 32 x 32,
+                  // 4 = 34 x 15
     int pf;       // 1: issue the next plane's loads before the current flow collision
 };
 

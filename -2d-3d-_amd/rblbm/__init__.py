@@ -23,7 +23,8 @@ T_CONST, T_ADIABATIC, T_PERIODIC = 0, 1, 2
 FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_ONE_PASS = 0, 1, 2
 LAYOUT_TWO_ARRAY, LAYOUT_AA, LAYOUT_AUTO = 0, 1, 2
-TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32 = 0, 1, 2, 3
+TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32, TILE_34X15 = 0, 1, 2, 3, 4
+KCHUNK_AUTO = -1
 
 EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
            "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",

@@ -48,10 +48,15 @@ def parse():
                     help="timed oracle steps of the CPU baseline (about 10 s on 16 host cores)")
     ap.add_argument("--cpu-nz", type=int, default=64)
     ap.add_argument("--variant", choices=["auto", "two-pass", "one-pass"], default="auto",
-                    help="kernel form of the step (auto: the two-pass form)")
-    ap.add_argument("--tile", type=int, default=0, help="one-pass thread tile (0: 32x16, 1: 16x16, 2: 32x8)")
-    ap.add_argument("--prefetch", type=int, default=0, help="one-pass prefetching (0/1)")
-    ap.add_argument("--kchunk", type=int, default=0, help="z-planes per one-pass block (0: default)")
+                    help="kernel form of the step (auto: the library's choice, the one-pass form "
+                         "for FP64 and the two-pass form for FP32 / FP16 at 512^3)")
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="one-pass thread tile (0: 32x16, 1: 16x16, 2: 32x8, 3: 32x32, 4: 34x15; "
+                         "-1: the library's default, 34x15)")
+    ap.add_argument("--prefetch", type=int, default=-1,
+                    help="one-pass prefetching (0/1; -1: the library's default, 1)")
+    ap.add_argument("--kchunk", type=int, default=0,
+                    help="z-planes per one-pass block (0: default, -1: from the resident blocks)")
     ap.add_argument("--layout", choices=["two-array", "aa", "auto", "tuned"], default="two-array",
                     help="population layout (default two-array, with the default two-pass "
                          "form); aa: one array per lattice updated in place (two-pass form only); "
@@ -177,7 +182,7 @@ def copy_gbs(torch, gib=2.0, reps=5):
     return 2 * n * 8 / (ms / 1e3) / 1e9
 
 
-def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=0, prefetch=0,
+def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=-1, prefetch=-1,
             layout=0, tune_steps=0, keep=None):
     """Time `steps` steps of one storage form on the n^3 cavity (inputs resident in HBM).
     tune_steps > 0: lbm_autotune picks the layout and kernel form first (outside the timing)."""
@@ -191,7 +196,10 @@ def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=
         tune_s = time.perf_counter() - t0
     else:
         sol.set_kernel_variant(variant, kchunk)
-        sol.set_fused_tile(tile, prefetch)
+        if tile >= 0 or prefetch >= 0:
+            f0 = sol.kernel_form()
+            sol.set_fused_tile(tile if tile >= 0 else f0["tile"],
+                               prefetch if prefetch >= 0 else f0["prefetch"])
     form = sol.kernel_form()
     aa = sol.layout == rblbm.LAYOUT_AA
     sol.initial_seeded(SEED, 1e-3)

@@ -184,29 +184,35 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 /* Kernel form of lbm_step (no reference counterpart: the reference runs eight sweeps).
  * TWO_PASS: a thermal and a flow launch per step.  ONE_PASS: one launch per step (3D without
  * periodic thermal walls only); it moves fewer bytes but recomputes a ring of nodes per tile.
- * AUTO (the default) runs TWO_PASS, the faster form at 512^3 on MI355X; lbm_autotune measures
- * both per shape.  Both give bitwise identical results in every precision form.  ONE_PASS needs
- * the two-array layout.  kchunk > 0 sets the number of z-planes per block of the one-pass form
- * (0 keeps the current value). */
+ * AUTO (the default) runs ONE_PASS for FP64 domains of 2^24 nodes or more (the faster form at
+ * 512^3 on MI355X) and TWO_PASS otherwise; lbm_autotune measures both per shape.  Both give
+ * bitwise identical results in every precision form.  ONE_PASS needs the two-array layout.
+ * kchunk > 0 sets the number of z-planes per block of the one-pass form, LBM_KCHUNK_AUTO the
+ * default (32), 0 keeps the current value. */
 #define LBM_KERNEL_AUTO 0
 #define LBM_KERNEL_TWO_PASS 1
 #define LBM_KERNEL_ONE_PASS 2
+#define LBM_KCHUNK_AUTO (-1)
 int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 
 /* Thread tile of the one-pass form (no reference counterpart): 32x16 threads, 16x16 or 32x8
- * (256-thread blocks, more of them per CU), or 32x32 (1024 threads, the smallest ring share;
- * FP32 / FP16 forms only, else LBM_ERR_UNSUPPORTED); prefetch = 1 issues the next plane's loads
- * before the current flow collision.  Results do not depend on the choice.
+ * (256-thread blocks, more of them per CU), 32x32 (1024 threads, the smallest ring share;
+ * FP32 / FP16 forms only, else LBM_ERR_UNSUPPORTED), or 34x15 (32 x 13 nodes, rows aligned to
+ * 32 nodes); prefetch = 1 issues the next plane's loads before the current flow collision.
+ * Results do not depend on the choice.
  * Batch domains reject this and lbm_set_kernel_variant (LBM_ERR_STATE): a batch is tuned with
  * lbm_batch_autotune or lbm_batch_set_kernel_variant. */
 #define LBM_TILE_32X16 0
 #define LBM_TILE_16X16 1
 #define LBM_TILE_32X8 2
 #define LBM_TILE_32X32 3
+#define LBM_TILE_34X15 4
 int lbm_set_fused_tile(lbm_handle *h, int32_t tile, int32_t prefetch);
 /* The population layout in effect (LBM_LAYOUT_TWO_ARRAY or LBM_LAYOUT_AA). */
 int lbm_get_layout(lbm_handle *h, int32_t *layout);
-/* The kernel form in effect: variant as requested (LBM_KERNEL_*), z-chunk, tile, prefetch. */
+/* The kernel form steps run (after the first step from an initial state, which is two-pass):
+ * variant LBM_KERNEL_TWO_PASS or LBM_KERNEL_ONE_PASS (AUTO resolved), and the one-pass form's
+ * z-chunk, tile and prefetch. */
 int lbm_get_kernel_form(lbm_handle *h, int32_t *variant, int32_t *kchunk, int32_t *tile,
                         int32_t *prefetch);
 

@@ -20,7 +20,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("kchunk", [5, 64])
+@pytest.mark.parametrize("kchunk", [5, 64, rblbm.KCHUNK_AUTO])
 def test_one_pass_fp64_bitwise(name, kchunk):
     dims = dict(dims=3, nx=45, ny=37, nz=29, rayleigh=1e5, mach=0.3, **CASES[name])
     ora = orc.Oracle(orc.Config(**dims))
@@ -49,7 +49,7 @@ def test_one_pass_fp64_bitwise(name, kchunk):
 
 
 @pytest.mark.parametrize("tile", [rblbm.TILE_32X16, rblbm.TILE_16X16, rblbm.TILE_32X8,
-                                  rblbm.TILE_32X32])
+                                  rblbm.TILE_32X32, rblbm.TILE_34X15])
 @pytest.mark.parametrize("prefetch", [0, 1])
 @pytest.mark.parametrize("prec", [rblbm.FP64, rblbm.FP32_SHIFT])
 def test_one_pass_tiles_and_prefetch(tile, prefetch, prec):
@@ -134,4 +134,25 @@ def test_one_pass_lowprec_matches_two_pass(prec):
     a, b = one.macros(), two.macros()
     # the same arithmetic per node in both kernel forms: bitwise equal after 40 steps
     for k in ("rho", "u", "v", "w", "T"):
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP64, rblbm.FP16_SHIFT])
+@pytest.mark.parametrize("shape", [(96, 80, 40), (130, 70, 33)])
+def test_one_pass_round_schedule_covers_every_tile(prec, shape):
+    """Domains with several rounds of resident blocks and partial strips, bands and rounds: the
+    block schedule visits every tile and z-chunk once (bitwise equal to the two-pass form)."""
+    nx, ny, nz = shape
+    dims = dict(dims=3, nx=nx, ny=ny, nz=nz, rayleigh=1e6, mach=0.2, precision=prec)
+    one = rblbm.Solver(rblbm.Config(**dims))
+    two = rblbm.Solver(rblbm.Config(**dims))
+    one.set_kernel_variant(rblbm.KERNEL_ONE_PASS, 4)   # many z-chunks: several rounds
+    one.set_fused_tile(rblbm.TILE_34X15, 1)
+    two.set_kernel_variant(rblbm.KERNEL_TWO_PASS)
+    one.initial_seeded(6, 0.1)
+    two.initial_seeded(6, 0.1)
+    one.step(9)
+    two.step(9)
+    a, b = one.gather(), two.gather()
+    for k in ("f", "g", "bx", "by", "bz"):
         np.testing.assert_array_equal(a[k], b[k])
