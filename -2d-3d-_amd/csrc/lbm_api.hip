@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -239,8 +240,7 @@ static int launch_fused_single(lbm_handle *h)
     if (par < 0) {
         if (h->capturing)   // the rebuild below synchronises: not inside a capture
             return fail(h, LBM_ERR_STATE, "one-pass launch table out of date during graph capture");
-        if (!h->ftab && hipMalloc(&h->ftab, 2 * isz) != hipSuccess)
-            return fail(h, LBM_ERR_NOMEM, "one-pass launch table");
+        if (!h->ftab) return fail(h, LBM_ERR_STATE, "one-pass launch table missing");
         // synchronous: the table is rebuilt only after an initialisation, a two-pass step or a
         // change of the kernel form, never inside a run of one-pass steps
         HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -395,6 +395,9 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     }
     if (ahead) (void)hipFree(ahead);
     h->red_blocks = 1024;
+    // the one-pass launch table is allocated with the domain, so that no step allocates
+    if (rc == LBM_OK && h->fused_ok && hipMalloc(&h->ftab, 2 * fused_entry_bytes(h)) != hipSuccess)
+        rc = LBM_ERR_NOMEM;
     if (rc == LBM_OK) {
         // volume partials (6 sums) then wall partials (4 sums per wall-profile block)
         const unsigned nw = (unsigned)(c.nx > c.ny ? c.nx : c.ny);   // wall-profile blocks
@@ -1310,6 +1313,71 @@ extern "C" int lbm_run_diagnostics(lbm_handle *h, int32_t nsteps, int32_t every,
     }
     std::vector<double> host((size_t)ns * stride);
     HIPCHK(h, hipMemcpyAsync(host.data(), ser.buf, host.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < ns; ++k) diag_finish(h, host.data() + (size_t)k * stride, &out[k]);
+    return LBM_OK;
+}
+
+// ---- periodic diagnostics without allocation -----------------------------------------------
+// Device scratch of lbm_run_diagnostics_noalloc: the series counter in the first 256 bytes, the
+// series after it.  Every GPU the library runs on holds its own copy of the array until the
+// process ends; host code reaches a device's copy only through the address hipGetSymbolAddress
+// returns for it with that device current (scratch_dev), never through the array's name.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr size_t kScratchHead = 256;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex scratch_mu;
+static unsigned char *scratch_dev[kScratchDevices];
+
+// Address of device h->cfg.device's scratch (looked up once per device, that device current).
+static int scratch_setup(lbm_handle *h)
+{
+    const int d = h->cfg.device;
+    if (d < 0 || d >= kScratchDevices) return fail(h, LBM_ERR_ARG, "device %d beyond the scratch table", d);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_dev[d]) return LBM_OK;
+    HIPCHK(h, hipSetDevice(d));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(h, LBM_ERR_HIP, "device scratch address: %s", hipGetErrorString(e));
+    scratch_dev[d] = (unsigned char *)p;
+    return LBM_OK;
+}
+
+// lbm_run_diagnostics with its series in the device scratch: every period runs eagerly (steps,
+// then the device part of the diagnostics appending to the scratch series), the host finishes
+// the samples after one copy of the series.  No device memory is allocated or freed.
+extern "C" int lbm_run_diagnostics_noalloc(lbm_handle *h, int32_t nsteps, int32_t every, lbm_diag *out)
+{
+    if (!h || !out || every <= 0 || nsteps < every || nsteps % every) return LBM_ERR_ARG;
+    if (h->phase < 0) return fail(h, LBM_ERR_STATE, "lbm_run_diagnostics_noalloc before initialisation");
+    if (h->in_batch) return refuse_batch_domain(h, "lbm_run_diagnostics_noalloc");
+    int rc = scratch_setup(h);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const int ns = nsteps / every;
+    const size_t stride = diag_sample_doubles(h);
+    const size_t need = kScratchHead + (size_t)ns * stride * sizeof(double);
+    if (need > kScratchBytes)
+        return fail(h, LBM_ERR_ARG,
+                    "lbm_run_diagnostics_noalloc: %d samples of %zu bytes need %zu bytes of scratch, "
+                    "which holds %zu (at most %zu samples for this domain)",
+                    ns, stride * sizeof(double), need, kScratchBytes,
+                    (kScratchBytes - kScratchHead) / (stride * sizeof(double)));
+    unsigned char *base = scratch_dev[h->cfg.device];
+    int *count = (int *)base;
+    double *series = (double *)(base + kScratchHead);
+    HIPCHK(h, launch_series_reset(h->stream, count, 1));   // the last call left its count
+    for (int k = 0; k < ns; ++k) {
+        rc = enqueue_steps(h, every);
+        if (!rc) rc = diag_append(h, series, ns, count);
+        if (rc) return rc;
+    }
+    std::vector<double> host((size_t)ns * stride);
+    HIPCHK(h, hipMemcpyAsync(host.data(), series, host.size() * sizeof(double),
                              hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     for (int k = 0; k < ns; ++k) diag_finish(h, host.data() + (size_t)k * stride, &out[k]);

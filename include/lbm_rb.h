@@ -148,6 +148,15 @@ int lbm_diagnostics(lbm_handle *h, lbm_diag *d);
  * series; the host finishes the compensated sums after the run.  Results equal lbm_step +
  * lbm_diagnostics at the same steps. */
 int lbm_run_diagnostics(lbm_handle *h, int32_t nsteps, int32_t every, lbm_diag *out);
+/* lbm_run_diagnostics that allocates no device memory (no counterpart in the reference; for
+ * callers that cannot allocate: serving loops, harnesses that forbid hipMalloc).  The series
+ * lives in a 32 MiB device scratch array of the library (one per GPU, held until the process
+ * ends); every period runs eagerly (no graph is instantiated).  Returns LBM_ERR_ARG, with a
+ * message naming the limit, when (nsteps / every) samples of this domain do not fit the scratch.
+ * Results equal lbm_run_diagnostics.  All calls on one GPU share its scratch and the last call's
+ * contents stay there: calls that overlap on one GPU (other threads, other handles' streams)
+ * race. */
+int lbm_run_diagnostics_noalloc(lbm_handle *h, int32_t nsteps, int32_t every, lbm_diag *out);
 /* Mid-plane velocity maxima of the current state.  3D: calc_umid_max_common,
  * calc_vmid_max_common, calc_wmid_max_common (2DRB/3DRB.F90:2399-2518; the field interpolated
  * onto the plane at half the domain extent, the first grid maximum).  2D: RBcalc_umid_max /
