@@ -29,6 +29,20 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
 
+PROBE = os.path.join(LIBDIR, "libhbmprobe.so")
+
+
+def build_probe(verbose: bool = False) -> str:
+    """The bench's streaming copy / read kernels (hbm_probe.hip), a library of their own."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "hbm_probe.hip"), "-o", PROBE + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(PROBE + ".tmp", PROBE)
+    return PROBE
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objs, procs = [], []
@@ -48,6 +62,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_probe(verbose)
     return LIB
 
 

@@ -7,15 +7,24 @@ noise +-1e-3 dT), rho=1, u=0.  A "step" is one full time step of both lattices o
 domain; inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line:
-  value      coupled GLUPS = nodes * steps / seconds / 1e9 (device time of the timed steps,
+  value      FP64 coupled GLUPS = nodes * steps / seconds / 1e9 (device time of the timed steps,
              HIP events on the solver's stream)
+  forms      the same for every storage form timed (default FP64, FP32-shift, FP16-shift), with
+             glups_vs_fp64, and accuracy: the relative differences of Nu_vol, Nu_hot, Nu_cold
+             and Re between each reduced form and FP64 at the end of the timed window (same
+             seeded start, same steps); the 20,000-step time-averaged study is
+             profiles/accuracy_512_ra1e8.md
   roofline   algorithmic bytes per node update (464 B in FP64: both lattices read and written
-             once plus the M1G history) * nodes / step time, against the 8 TB/s HBM3E peak.
-             The unit is one step: the two launches of the two-pass form (k3_thermal, k3_flow)
-             or the one launch of the one-pass form (k3_fused).  traffic = measured HBM bytes
-             per step from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy kernel's readings
-             (profiles/traffic_fp64_512.json), or null when that file has no entry for the form;
-             copy_gbs = a device copy measured in the same run (the practical ceiling)
+             once plus the M1G history) * nodes / step time, against the 8 TB/s HBM3E peak.  The
+             unit is one step: the one launch of the one-pass form (k3_fused) or the two launches
+             of the two-pass form.  traffic = measured HBM bytes per step (rocprofv3 FETCH_SIZE
+             and WRITE_SIZE, each scaled by its reading for a copy kernel of known size;
+             profiles/traffic_fp64_512.json), null unless that summary was measured on the
+             kernel sources of this tree; copy_gbs / read_gbs = the hand-written streaming copy
+             and read kernels of hbm_probe.hip in the same run (the practical ceiling)
+  configs    the other benchmark configurations' rates: C5 (1024^3 Ra=1e10 FP16-shift) and C4
+             (the seeded 64-cavity FP32-shift ensemble: HIP-graph table launches against one
+             launch per cavity)
   cpu_baseline  the FP64 OpenMP oracle (a restatement of the reference's CPU path) timed on
              this host over a bounded sample of the same workload
 """
@@ -41,8 +50,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--rayleigh", type=float, default=1e8)
-    ap.add_argument("--forms", default="fp64,fp32",
+    ap.add_argument("--forms", default="fp64,fp32,fp16",
                     help="storage forms to time, comma separated; the first one is `value`")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the C5 (1024^3 FP16) and C4 (64-cavity ensemble) legs")
+    ap.add_argument("--c5-steps", type=int, default=10)
+    ap.add_argument("--c4-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=48,
                     help="timed oracle steps of the CPU baseline (about 10 s on 16 host cores)")
@@ -70,22 +83,122 @@ def parse():
                          "GPU and, as the CPU baseline, the FP64 OpenMP oracle over the same run; "
                          "reports both and the largest difference of f, g, u, T and Nu")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
-                    help="only run a device-to-device copy of GIB GiB (counter calibration)")
+                    help="only run the probe's copy and read kernels on GIB GiB, 3 launches each "
+                         "(counter calibration)")
+    ap.add_argument("--traffic-summary", default="", metavar="DIR",
+                    help="only write profiles/traffic_fp64_512.json from the rocprofv3 --pmc "
+                         "runs under DIR (fetch/, write/: the bench; copy_fetch/, copy_write/: "
+                         "the copy probe)")
     return ap.parse_args()
 
 
-def copy_probe(gib):
-    """A plain copy kernel of known size: reads and writes gib GiB once per launch."""
-    import torch
+PROBE_LIB = os.path.join(ROOT, "-2d-3d-_amd", "lib", "libhbmprobe.so")
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_fp64_512.json")
 
-    n = int(gib * (1 << 30)) // 8
-    src = torch.ones(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
-                      "launches": 3}), flush=True)
+
+def hbm_probe(gib=4.0, reps=5):
+    """Streaming copy and read kernels (hbm_probe.hip) over gib GiB: (copy GB/s, read GB/s)."""
+    import ctypes as C
+
+    lib = C.CDLL(PROBE_LIB)
+    lib.hbm_probe.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    cp, rd = C.c_double(), C.c_double()
+    rc = lib.hbm_probe(0, int(gib * (1 << 30)), reps, C.byref(cp), C.byref(rd))
+    if rc != 0:
+        raise RuntimeError(f"hbm_probe failed: HIP error {rc}")
+    return cp.value, rd.value
+
+
+def copy_probe(gib):
+    """The probe's copy and read kernels of known size (3 launches each) for the counters."""
+    n = int(gib * (1 << 30)) // 16 * 16
+    hbm_probe(gib, 3)
+    print(json.dumps({"probe_bytes": n, "copy": "k_copy reads and writes probe_bytes a launch",
+                      "read": "k_read reads probe_bytes a launch", "launches": 3}), flush=True)
+
+
+def kernel_sha():
+    """Hash of the kernel and launch sources, the key of the measured traffic summary."""
+    import hashlib
+
+    d = os.path.join(ROOT, "-2d-3d-_amd", "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".h")) and f != "hbm_probe.hip":
+            with open(os.path.join(d, f), "rb") as fh:
+                h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
+def _pmc(path, counter):
+    """Per-kernel mean of a rocprofv3 --pmc counter (KiB per dispatch) from a run directory."""
+    import csv
+    import glob
+
+    out = {}
+    for f in glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] == counter:
+                out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in out.items()}
+
+
+# The kernels of one steady step per (storage form, kernel form), as rocprofv3 names them.
+STEP_KERNELS = {
+    ("fp64", "one-pass"): ["k3_fused<rblbm::FormF64"],
+    ("fp64", "two-pass"): ["k3_thermal<false, 0>", "k3_flow<0>"],
+    ("fp32", "one-pass"): ["k3_fused<rblbm::FormLP<float>"],
+    ("fp32", "two-pass"): ["lp3_thermal<float, false, 0>", "lp3_flow<float, 0>"],
+    # _Float16 instances reach the counter files with their mangled names
+    ("fp16", "one-pass"): ["k3_fused<rblbm::FormLP<_Float16>", "k3_fusedINS_6FormLPIDF16_E"],
+    ("fp16", "two-pass"): ["lp3_thermal<_Float16, false, 0>", "lp3_flow<_Float16, 0>",
+                           "lp3_thermalIDF16_Lb0ELi0E", "lp3_flowIDF16_Li0E"],
+}
+
+
+def traffic_summary(d):
+    """profiles/traffic_fp64_512.json from the counter runs under d (see --traffic-summary)."""
+    cf, cw = _pmc(os.path.join(d, "copy_fetch"), "FETCH_SIZE"), _pmc(os.path.join(d, "copy_write"), "WRITE_SIZE")
+    probe = [json.loads(l) for l in open(os.path.join(d, "copy_fetch.log")) if l.startswith("{")][-1]
+    nbytes = probe["probe_bytes"]
+    kc = [k for k in cf if "k_copy" in k][0]
+    f_fetch = nbytes / (cf[kc] * 1024.0)
+    f_write = nbytes / (cw[kc] * 1024.0)
+    fetch, write = _pmc(os.path.join(d, "fetch"), "FETCH_SIZE"), _pmc(os.path.join(d, "write"), "WRITE_SIZE")
+    bench = [json.loads(l) for l in open(os.path.join(d, "fetch.log")) if l.startswith("{")][-1]
+    nodes = bench["config"]["nodes"]
+    out = {"kernel_sha": kernel_sha(), "nodes": nodes,
+           "copy_calibration": {"probe_bytes": nbytes, "fetch_kib": cf[kc], "write_kib": cw[kc],
+                                "fetch_factor": f_fetch, "write_factor": f_write},
+           "forms": {}}
+    for name, f in bench["forms"].items():
+        kern = [k for k in fetch if any(p in k for p in STEP_KERNELS[(name, f["form"]["name"])])]
+        if not kern:
+            continue
+        fb = sum(fetch[k] for k in kern) * 1024.0 * f_fetch
+        wb = sum(write[k] for k in kern) * 1024.0 * f_write
+        out["forms"][f"{name}/{f['form']['name']}"] = {
+            "kernel": " + ".join(k.split("(")[0].replace("void ", "") for k in kern),
+            "fetch_bytes_per_step": fb, "write_bytes_per_step": wb, "hbm_bytes_per_step": fb + wb,
+            "algorithmic_bytes_per_step": f["bytes_per_update"] * nodes,
+            "ratio": (fb + wb) / (f["bytes_per_update"] * nodes)}
+    with open(TRAFFIC, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out), flush=True)
+
+
+def measured_traffic(name, form_name):
+    """HBM bytes per step of the summary, or None when it was measured on other sources."""
+    if not os.path.exists(TRAFFIC):
+        return None, "no traffic summary"
+    with open(TRAFFIC) as fh:
+        t = json.load(fh)
+    if t.get("kernel_sha") != kernel_sha():
+        return None, f"summary measured on kernel sources {t.get('kernel_sha')}, tree has {kernel_sha()}"
+    e = t.get("forms", {}).get(f"{name}/{form_name}")
+    if not e:
+        return None, f"no entry for {name}/{form_name}"
+    return e["hbm_bytes_per_step"], f"profiles/traffic_fp64_512.json ({e['kernel']})"
 
 
 def cpu_baseline(size, nz, ra, steps):
@@ -162,27 +275,7 @@ def full_config(name):
             "nu_vol": d["nu_vol"], "re_vol": d["re_vol"], "nu_global": d["nu_global"]}
 
 
-def copy_gbs(torch, gib=2.0, reps=5):
-    """Bandwidth of a device-to-device copy (read + write) of `gib` GiB: the practical ceiling."""
-    if torch is None:
-        return None
-    n = int(gib * (1 << 30)) // 8
-    src = torch.ones(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    t1.record()
-    torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1) / reps
-    del src, dst
-    torch.cuda.empty_cache()
-    return 2 * n * 8 / (ms / 1e3) / 1e9
-
-
-def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=-1, prefetch=-1,
+def measure(rblbm, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=-1, prefetch=-1,
             layout=0, tune_steps=0, keep=None):
     """Time `steps` steps of one storage form on the n^3 cavity (inputs resident in HBM).
     tune_steps > 0: lbm_autotune picks the layout and kernel form first (outside the timing)."""
@@ -206,12 +299,8 @@ def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=
     mass0 = sol.diagnostics()["mass"]
     sol.step(warmup)
     sol.sync()
-    if torch is not None:
-        torch.cuda.synchronize()
     t_wall = time.perf_counter()
     ms = sol.step_timed(steps)            # HIP events on the solver's own stream
-    if torch is not None:
-        torch.cuda.synchronize()
     t_wall = time.perf_counter() - t_wall
     nodes = n ** 3
     ms_step = ms / steps
@@ -233,14 +322,62 @@ def measure(rblbm, torch, n, ra, prec, steps, warmup, variant=0, kchunk=0, tile=
     return {"glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms_step, "form": form,
             "bytes_per_update": bpu, "achieved_gbs": bpu * nodes / (ms_step / 1e3) / 1e9,
             "roofline_frac": bpu * nodes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
-            "wall_seconds": t_wall, "nu_vol": diag["nu_vol"], "diag": diag, "mass0": mass0,
+            "wall_seconds": t_wall, "diag": diag, "mass0": mass0,
             "mass_rel_change": (diag["mass"] - mass0) / mass0}
+
+
+def c5_rate(rblbm, steps):
+    """C5: 1024^3 Ra=1e10 FP16-shift, the library's layout choice (two arrays when they fit)."""
+    sol = rblbm.Solver(rblbm.Config(dims=3, nx=1024, ny=1024, nz=1024, rayleigh=1e10,
+                                    prandtl=0.71, mach=0.1, precision=rblbm.FP16_SHIFT,
+                                    layout=rblbm.LAYOUT_AUTO))
+    form = sol.kernel_form()
+    lay = "aa" if sol.layout == rblbm.LAYOUT_AA else "two-array"
+    sol.initial_seeded(SEED, 1e-3)
+    sol.step(2)
+    sol.sync()
+    ms = sol.step_timed(steps)
+    d = sol.diagnostics()
+    sol.close()
+    nodes = 1024 ** 3
+    return {"workload": "3D RB cavity 1024^3 Ra=1e10 Pr=0.71 Ma=0.1, FP16-shift storage",
+            "glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms / steps, "steps": steps,
+            "layout": lay, "kernel_form": "one-pass" if form["variant"] == rblbm.KERNEL_ONE_PASS else "two-pass",
+            "bytes_per_update": 128.0,
+            "roofline_frac": 128.0 * nodes / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "nu_vol": d["nu_vol"]}
+
+
+def c4_rate(rblbm, steps):
+    """C4: the seeded 64-cavity FP32-shift ensemble; graph-replayed table launches against one
+    launch per cavity (the same steps on the same cavities)."""
+    sys.path.insert(0, os.path.join(ROOT, "-2d-3d-_amd"))
+    import ensemble
+
+    cfgs = ensemble.ensemble(64, SEED, rblbm.FP32_SHIFT)
+    b = rblbm.Batch(cfgs)
+    b.initial_seeded(SEED, 1e-3)
+    nodes = b.nodes
+    out = {"workload": "64 cavities, edges {32,48,64,96,128}^3 seeded, Ra log-uniform [1e4,1e7], "
+                       "Pr in {0.71,7}, FP32-shift", "nodes": nodes, "steps": steps}
+    for name, mode in (("per_cavity", rblbm.BATCH_PER_DOMAIN), ("graph", rblbm.BATCH_GRAPH)):
+        b.set_mode(mode)
+        b.step(4)
+        ms = b.step_timed(steps)
+        out[name] = {"glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms / steps}
+    out["glups"] = out["graph"]["glups"]
+    out["graph_vs_per_cavity"] = out["graph"]["glups"] / out["per_cavity"]["glups"]
+    b.close()
+    return out
 
 
 def main():
     a = parse()
     if a.copy_probe > 0:
         copy_probe(a.copy_probe)
+        return
+    if a.traffic_summary:
+        traffic_summary(a.traffic_summary)
         return
     if a.full_config:
         print(json.dumps(full_config(a.full_config)), flush=True)
@@ -252,12 +389,6 @@ def main():
 
     import rblbm
 
-    try:
-        import torch
-        have_torch = torch.cuda.is_available()
-    except Exception:
-        torch, have_torch = None, False
-
     n = a.size
     nodes = n * n * n
     forms, open_domains = {}, []
@@ -268,55 +399,66 @@ def main():
         lname = "two-array" if a.variant == "one-pass" and a.layout != "tuned" else a.layout
         layout = {"two-array": rblbm.LAYOUT_TWO_ARRAY, "aa": rblbm.LAYOUT_AA,
                   "auto": rblbm.LAYOUT_AUTO, "tuned": rblbm.LAYOUT_AUTO}[lname]
-        forms[name] = measure(rblbm, torch if have_torch else None, n, a.rayleigh, prec,
-                              a.steps, a.warmup, variant, a.kchunk, a.tile, a.prefetch, layout,
-                              a.tune_steps if lname == "tuned" else 0, open_domains)
+        forms[name] = measure(rblbm, n, a.rayleigh, prec, a.steps, a.warmup, variant, a.kchunk,
+                              a.tile, a.prefetch, layout, a.tune_steps if lname == "tuned" else 0,
+                              open_domains)
+        print(f"bench: {name} {forms[name]['ms_per_step']:.3f} ms per step", file=sys.stderr, flush=True)
     for sol in open_domains:
         sol.close()
-    main_form = forms[a.forms.split(",")[0]]
-    ms_step = main_form["ms_per_step"]
-    glups = main_form["glups"]
-    bpu = main_form["bytes_per_update"]
-    achieved = main_form["achieved_gbs"]
-    t_wall = main_form["wall_seconds"]
-    diag = main_form["diag"]
-    mass0 = main_form["mass0"]
-    if "fp64" in forms and "fp32" in forms:
-        forms["fp32"]["glups_vs_fp64"] = forms["fp32"]["glups"] / forms["fp64"]["glups"]
-    for f in forms.values():
-        f.pop("diag", None)
-
     main_name = a.forms.split(",")[0]
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_fp64_512.json")
-    if os.path.exists(tpath) and n == 512 and main_name == "fp64":
-        with open(tpath) as fh:
-            entry = json.load(fh).get(main_form["form"]["name"])
-        if entry:
-            traffic = entry["hbm_bytes_per_step"]
-    ceiling = copy_gbs(torch if have_torch else None)
+    main_form = forms[main_name]
+    diag, mass0 = main_form["diag"], main_form["mass0"]
+    # reduced forms against FP64 at the end of the timed window (same seeded start and steps)
+    if "fp64" in forms:
+        ref = forms["fp64"]["diag"]
+        for name, f in forms.items():
+            if name == "fp64":
+                continue
+            f["glups_vs_fp64"] = f["glups"] / forms["fp64"]["glups"]
+            f["accuracy_vs_fp64"] = {
+                k: abs(f["diag"][k] - ref[k]) / abs(ref[k]) for k in ("nu_vol", "nu_hot", "nu_cold", "re_vol")}
+            f["accuracy_vs_fp64"]["after_steps"] = a.warmup + a.steps
+            f["accuracy_vs_fp64"]["time_averaged_study"] = "profiles/accuracy_512_ra1e8.md"
+    for f in forms.values():
+        f["nu_vol"] = f.pop("diag")["nu_vol"]
+
+    copy_gbs, read_gbs = hbm_probe()
+    traffic, traffic_src = (None, "not measured at this size / form")
+    if n == 512:
+        traffic, traffic_src = measured_traffic(main_name, main_form["form"]["name"])
+    configs = {}
+    if not a.no_extras:
+        configs["c5_1024_fp16"] = c5_rate(rblbm, a.c5_steps)
+        print(f"bench: C5 {configs['c5_1024_fp16']['ms_per_step']:.3f} ms per step", file=sys.stderr, flush=True)
+        configs["c4_ensemble_fp32"] = c4_rate(rblbm, a.c4_steps)
+        print(f"bench: C4 {configs['c4_ensemble_fp32']['glups']:.2f} GLUPS", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(n, min(a.cpu_nz, n), a.rayleigh, a.cpu_steps)
 
+    achieved = main_form["achieved_gbs"]
+    bpu = main_form["bytes_per_update"]
     out = {
         "metric": "coupled D3Q19+D3Q7 GLUPS & %HBM roofline, FP32-shift vs FP64 at 512^3, Nu err",
-        "value": glups, "unit": "GLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16/f32"}[a.forms.split(",")[0]],
+        "value": main_form["glups"], "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": main_form["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16/f32"}[main_name],
         "data": "synthetic",
         "config": {"workload": f"3D RB cavity {n}^3 Ra={a.rayleigh:g} Pr=0.71 Ma=0.1, "
                                f"{main_name} storage form, seeded T noise 1e-3",
                    "nodes": nodes, "precision": main_name, "kernel_form": main_form["form"],
-                   "wall_seconds": t_wall},
+                   "wall_seconds": main_form["wall_seconds"]},
         "forms": forms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_unit": "bytes per step", "bytes_per_update": bpu,
-                     "algorithmic_bytes_per_step": bpu * nodes,
-                     "kernel": main_form["form"]["kernels"], "copy_gbs": ceiling,
-                     "frac_of_copy": achieved / ceiling if ceiling else None},
+                     "traffic_unit": "bytes per step", "traffic_source": traffic_src,
+                     "bytes_per_update": bpu, "algorithmic_bytes_per_step": bpu * nodes,
+                     "kernel": main_form["form"]["kernels"], "copy_gbs": copy_gbs,
+                     "read_gbs": read_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "kernel_sha": kernel_sha()},
+        "configs": configs,
         "cpu_baseline": cpu,
         "checks": {"nu_vol": diag["nu_vol"], "re_vol": diag["re_vol"],
                    "mass_rel_change": (diag["mass"] - mass0) / mass0},
