@@ -3,10 +3,17 @@
 ! 2DRB/2DRB.F90:202-358), stepping the gfx950 solver through the lbm_rb C ABI.
 !
 !   initial   -> lbm_create + lbm_init_exact / lbm_init_seeded   (or restart: read f, g + scatter)
-!   time loop -> lbm_step (the eight sweeps), check every checkInterval steps
-!   end       -> lbm_diagnostics (RBcalc_Nu_global, RBcalc_Nu_wall_avg, calNuRe), the mid-plane
-!                velocity maxima, MLUPS log, optional output_binary / backupData / Tecplot files
-!                in the reference's formats
+!   time loop -> lbm_step (the eight sweeps); steadyFlow: check every checkInterval steps and
+!                output_Tecplot every backupInterval free-fall times; unsteadyFlow: every
+!                outputFrequency free-fall times output_binary, backupData every backupInterval,
+!                output_Tecplot (2DRB/3DRB.F90:245-262, 2DRB/2DRB.F90:261-276)
+!   end       -> steadyFlow: output_Tecplot + output_binary; lbm_diagnostics (RBcalc_Nu_global,
+!                RBcalc_Nu_wall_avg, calNuRe), the mid-plane velocity maxima, MLUPS log; 3D: the
+!                final output_Tecplot, 2D: the stream-function report and its Tecplot file
+! Files keep the reference's names and formats: backupFile3D-<n>.bin / backupFile-<n>.bin (f, g),
+! <binFolderPrefix>-<n>.bin (3D: u, v, w scaled by L/kappa, T, rho; 2D: u, v, T, rho), the
+! Tecplot v101 files; records follow GNU Fortran's subrecord convention (rb_records), so restart
+! files over 2 GiB read back in the reference's own build.
 !
 ! Parameters keep the reference's names (module commondata3d) and defaults; a namelist file
 ! given as the first command-line argument overrides them at run time:
@@ -17,6 +24,7 @@
 program rb_main
   use iso_c_binding
   use lbm_rb
+  use rb_records
   use rb_tecplot
   implicit none
 
@@ -43,17 +51,24 @@ program rb_main
   integer(8) :: seed = 20261203_8
   real(8) :: noiseAmp = 1.0d-3
   integer :: loadInitField = 0
-  character(len=256) :: reloadFile = 'backupFile3D-0.bin'
+  character(len=256) :: reloadFile = ''   ! default: backupFile3D-<reloadbinFileNum>.bin (2D: backupFile-)
   integer :: outputBinFile = 0
   integer :: outputPltFile = 0
   character(len=100) :: pltFolderPrefix = ''
+  character(len=100) :: binFolderPrefix = ''   ! 3D 'buoyancyCavity3DbinFile', 2D './binFile/buoyancyCavity'
   integer :: backupAtEnd = 0
+  integer :: flowMode = 0            ! 0: steadyFlow (the reference's default), 1: unsteadyFlow
+  integer :: backupInterval = 1000   ! free-fall times between backups
+  integer :: endOutputs = 1          ! the reference's end-of-run Tecplot and binary files
+  integer :: reloadbinFileNum = 0    ! unsteadyFlow file numbering continues after a reload
+  integer(8) :: maxSubrecord = 2147483639_8   ! bytes of data per subrecord (gfortran's limit)
   character(len=256) :: settingsFile = 'SimulationSettings3D.txt'
   namelist /rb/ dims, nx, ny, nz, Rayleigh, Prandtl, Mach, Thot, Tcold, itc_max, checkInterval, &
        epsU, epsT, sideHeated, velX, velZ, tempX, tempY, tempZ, useG, precision, device, &
        seeded, seed, noiseAmp, loadInitField, reloadFile, outputBinFile, backupAtEnd, settingsFile, &
        outputPltFile, pltFolderPrefix, legacyThermal, layout, outputFrequency, &
-       reloadDimensionlessTime, nuReInLoop
+       reloadDimensionlessTime, nuReInLoop, binFolderPrefix, flowMode, backupInterval, &
+       endOutputs, reloadbinFileNum, maxSubrecord
 
   type(lbm_config) :: cfg
   type(lbm_params) :: prm
@@ -63,10 +78,13 @@ program rb_main
   integer :: itc, nchunk, qf, qt, ios
   real(8) :: errorU, errorT
   integer :: dimensionlessTime = 0, cadence, target, ns, lead, k
+  integer :: binFileNum = 0, tecCadence, bakCadence, lastPlt = -1
+  logical :: cadenceIO
   type(lbm_diag), allocatable :: series(:)
   integer(8) :: t0, t1, rate
   real(8) :: seconds
   character(len=256) :: nmlFile
+  character(len=32) :: tag0
 
   if (command_argument_count() >= 1) then
     call get_command_argument(1, nmlFile)
@@ -81,6 +99,12 @@ program rb_main
   if (dims == 2) nz = 1
   if (len_trim(pltFolderPrefix) == 0) &
     pltFolderPrefix = merge('buoyancyCavity3DTecplot', 'buoyancyCavityTecplot  ', dims == 3)
+  if (len_trim(binFolderPrefix) == 0) &
+    binFolderPrefix = merge('buoyancyCavity3DbinFile ', './binFile/buoyancyCavity', dims == 3)
+  if (len_trim(reloadFile) == 0) then   ! reloadFilePrefix-<reloadbinFileNum>.bin, 2DRB/3DRB.F90:609-611
+    write(tag0, '(i0)') reloadbinFileNum
+    reloadFile = trim(merge('backupFile3D', 'backupFile  ', dims == 3))//'-'//trim(tag0)//'.bin'
+  endif
   qf = merge(19, 9, dims == 3)
   qt = merge(7, 5, dims == 3)
 
@@ -94,51 +118,42 @@ program rb_main
   errorT = 100.0d0
   itc = 0
   call system_clock(t0, rate)
-  cadence = max(1, int(outputFrequency * prm%time_unit))   ! 2DRB/3DRB.F90:249
+  cadence = max(1, int(outputFrequency * prm%time_unit))              ! 2DRB/3DRB.F90:249
+  tecCadence = max(1, backupInterval * int(prm%time_unit))            ! :253 (steadyFlow)
+  bakCadence = max(1, int(dble(backupInterval) / outputFrequency)) * cadence   ! :259 (unsteadyFlow)
+  cadenceIO = (flowMode == 0 .and. outputPltFile == 1) .or. &
+              (flowMode == 1 .and. (outputBinFile == 1 .or. outputPltFile == 1))
   do while (((errorU > epsU) .or. (errorT > epsT)) .and. (itc <= itc_max))
-    ! advance to the next check point (or to the last step)
-    nchunk = min(checkInterval - mod(itc, checkInterval), itc_max + 1 - itc)
-    target = itc + nchunk
-    if (nuReInLoop == 1) then
-      ! calNuRe at every multiple of the cadence: the first sample point eagerly if it is not a
-      ! whole cadence away, the following ones from lbm_run_diagnostics (graph replays)
-      ns = target / cadence - itc / cadence
-      if (ns > 0) then
-        lead = cadence - mod(itc, cadence)
-        if (lead < cadence) then
-          call lbm_require(lbm_step(h, int(lead, c_int32_t)), 'lbm_step', h, 6)
-          itc = itc + lead
-          call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
-          call calNuRe(dg)
-          ns = ns - 1
+    ! advance to the next check point, output point or the last step
+    target = itc_max + 1
+    if (flowMode == 0) target = min(target, itc + checkInterval - mod(itc, checkInterval))
+    if (cadenceIO) target = min(target, itc + cadence - mod(itc, cadence))
+    call advance(target)
+    if (flowMode == 0 .and. mod(itc, checkInterval) == 0) call check()
+    if (cadenceIO .and. mod(itc, cadence) == 0) then
+      if (flowMode == 0) then
+        if (outputPltFile == 1 .and. mod(itc, tecCadence) == 0) call output_Tecplot()
+      else
+        if (outputBinFile == 1) then
+          call output_binary()
+          if (mod(itc, bakCadence) == 0) call backupData()
         endif
-        if (ns > 0) then
-          allocate(series(ns))
-          call lbm_require(lbm_run_diagnostics(h, int(ns * cadence, c_int32_t), &
-               int(cadence, c_int32_t), series), 'lbm_run_diagnostics', h, 6)
-          itc = itc + ns * cadence
-          do k = 1, ns
-            call calNuRe(series(k))
-          enddo
-          deallocate(series)
-        endif
+        if (outputPltFile == 1) call output_Tecplot()
       endif
-      if (target > itc) call lbm_require(lbm_step(h, int(target - itc, c_int32_t)), 'lbm_step', h, 6)
-    else
-      call lbm_require(lbm_step(h, int(nchunk, c_int32_t)), 'lbm_step', h, 6)
     endif
-    itc = target
-    if (mod(itc, checkInterval) == 0) call check()
   enddo
   call lbm_require(lbm_sync(h), 'lbm_sync', h, 6)
   call system_clock(t1)
   seconds = dble(t1 - t0) / dble(rate)
 
+  if (flowMode == 0 .and. endOutputs == 1) then       ! 2DRB/3DRB.F90:269-272
+    call output_Tecplot()
+    call output_binary()
+  endif
   call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
   call calNuRe(dg)                                     ! the end-of-run call, 2DRB/3DRB.F90:297
   call lbm_require(lbm_midplane_extrema(h, mp), 'lbm_midplane_extrema', h, 6)
-  if (outputBinFile == 1) call output_binary()
-  if (outputPltFile == 1) call output_Tecplot()
+  if (dims == 3 .and. endOutputs == 1 .and. lastPlt /= itc) call output_Tecplot()   ! :298
   if (backupAtEnd == 1) call backupData()
 
   write(*,'(a,1x,es16.8)') 'Nu_global =', dg%nu_global
@@ -169,6 +184,50 @@ program rb_main
   call lbm_require(lbm_destroy(h), 'lbm_destroy', c_null_ptr, 6)
 
 contains
+
+  !------------------------------------------------------------------------------------------
+  ! Steps from itc to target.  With nuReInLoop, calNuRe at every multiple of the cadence on the
+  ! way: the first sample point eagerly when it is not a whole cadence away, the following ones
+  ! from lbm_run_diagnostics (graph replays).
+  !------------------------------------------------------------------------------------------
+  subroutine advance(target)
+    integer, intent(in) :: target
+    if (nuReInLoop == 1) then
+      ns = target / cadence - itc / cadence
+      if (ns > 0) then
+        lead = cadence - mod(itc, cadence)
+        if (lead < cadence) then
+          call lbm_require(lbm_step(h, int(lead, c_int32_t)), 'lbm_step', h, 6)
+          itc = itc + lead
+          call lbm_require(lbm_diagnostics(h, dg), 'lbm_diagnostics', h, 6)
+          call calNuRe(dg)
+          ns = ns - 1
+        endif
+        if (ns > 0) then
+          allocate(series(ns))
+          call lbm_require(lbm_run_diagnostics(h, int(ns * cadence, c_int32_t), &
+               int(cadence, c_int32_t), series), 'lbm_run_diagnostics', h, 6)
+          itc = itc + ns * cadence
+          do k = 1, ns
+            call calNuRe(series(k))
+          enddo
+          deallocate(series)
+        endif
+      endif
+    endif
+    if (target > itc) call lbm_require(lbm_step(h, int(target - itc, c_int32_t)), 'lbm_step', h, 6)
+    itc = target
+  end subroutine advance
+
+  ! File-name number: itc in steadyFlow, the output count in unsteadyFlow (continuing from
+  ! reloadbinFileNum after a reload), 2DRB/3DRB.F90:1608-1614, :1646-1651
+  integer function file_number()
+    if (flowMode == 0) then
+      file_number = itc
+    else
+      file_number = binFileNum + merge(reloadbinFileNum, 0, loadInitField == 1)
+    endif
+  end function file_number
 
   !------------------------------------------------------------------------------------------
   ! Mid-plane velocity maxima, printed as calc_*mid_max_common (2DRB/3DRB.F90:2399-2518) and
@@ -239,17 +298,100 @@ contains
     else
       ! restart: f then g records, interior nodes, population index outermost (2DRB/3DRB.F90:611-616)
       allocate(f(0:qf-1, nx, ny, nz), g(0:qt-1, nx, ny, nz))
-      open(unit=01, file=trim(reloadFile), form='unformatted', access='sequential', status='old')
-      read(01) ((((f(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qf-1)
-      read(01) ((((g(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qt-1)
+      open(unit=01, file=trim(reloadFile), form='unformatted', access='stream', status='old', &
+           iostat=ios)
+      if (ios /= 0) then
+        write(*,*) 'cannot open restart file ', trim(reloadFile)
+        stop 1
+      endif
+      call read_populations(01, f, qf)
+      call read_populations(01, g, qt)
       close(01)
       write(00,*) 'Raw data is loaded from the file: ', trim(reloadFile)
       call lbm_require(lbm_scatter(h, c_loc(f), c_loc(g)), 'lbm_scatter', h, 6)
       deallocate(f, g)
     endif
+    if (outputBinFile == 1) then          ! 2DRB/3DRB.F90:364-369
+      open(unit=01, file=trim(binFolderPrefix)//'-readme', status='unknown', iostat=ios)
+      if (ios /= 0) then
+        write(*,*) 'cannot write into the folder of binFolderPrefix ', trim(binFolderPrefix)
+        stop 1
+      endif
+      write(01,*) 'binFile folder exist!'
+      close(01)
+      write(00,*) 'Data will be stored in ', trim(binFolderPrefix)
+    endif
+    write(00,*) 'flow mode = ', trim(merge('steadyFlow  ', 'unsteadyFlow', flowMode == 0))
+    write(00,*) 'outputFrequency =', outputFrequency, ' free-fall time units'
+    write(00,*) 'backupInterval =', backupInterval, ' free-fall time units'
     write(00,*) '-------------------------------------------------------------------------------'
     close(00)
   end subroutine initial
+
+  !------------------------------------------------------------------------------------------
+  ! One restart record (2DRB/3DRB.F90:611-616): q planes of nodes, population index outermost,
+  ! into p(0:q-1, nx, ny, nz) (population index fastest, the order of lbm_scatter)
+  !------------------------------------------------------------------------------------------
+  subroutine read_populations(unit, p, q)
+    integer, intent(in) :: unit, q
+    real(8), intent(inout) :: p(0:, :, :, :)
+    real(8), allocatable, target :: plane(:)
+    type(rec_reader) :: r
+    integer :: alpha, i, j, k, c
+    allocate(plane(int(nx, 8) * ny * nz))
+    call rec_open(r, unit)
+    do alpha = 0, q - 1
+      call rec_get_f64(r, plane)
+      c = 0
+      do k = 1, nz
+        do j = 1, ny
+          do i = 1, nx
+            c = c + 1
+            p(alpha, i, j, k) = plane(c)
+          enddo
+        enddo
+      enddo
+    enddo
+    call rec_close(r)
+    deallocate(plane)
+  end subroutine read_populations
+
+  subroutine write_populations(unit, p, q)
+    integer, intent(in) :: unit, q
+    real(8), intent(in) :: p(0:, :, :, :)
+    real(8), allocatable, target :: plane(:)
+    type(rec_writer) :: w
+    integer :: alpha, i, j, k, c
+    allocate(plane(int(nx, 8) * ny * nz))
+    call rec_begin(w, unit, 8_8 * q * size(plane, kind=8), maxSubrecord)
+    do alpha = 0, q - 1
+      c = 0
+      do k = 1, nz
+        do j = 1, ny
+          do i = 1, nx
+            c = c + 1
+            plane(c) = p(alpha, i, j, k)
+          enddo
+        enddo
+      enddo
+      call rec_put_f64(w, plane)
+    enddo
+    call rec_end(w)
+    deallocate(plane)
+  end subroutine write_populations
+
+  ! One snapshot record: a field of nx * ny * nz values times scale
+  subroutine write_field(unit, a, scale)
+    integer, intent(in) :: unit
+    real(8), intent(in), target, contiguous :: a(:, :, :)
+    real(8), intent(in) :: scale
+    real(8), allocatable, target :: b(:)
+    type(rec_writer) :: w
+    b = reshape(a, [size(a, kind=8)]) * scale
+    call rec_begin(w, unit, 8_8 * size(b, kind=8), maxSubrecord)
+    call rec_put_f64(w, b)
+    call rec_end(w)
+  end subroutine write_field
 
   !------------------------------------------------------------------------------------------
   ! check (2DRB/3DRB.F90:1475-1528)
@@ -280,7 +422,9 @@ contains
   end subroutine calNuRe
 
   !------------------------------------------------------------------------------------------
-  ! output_binary (2DRB/3DRB.F90:1599-1627): u, v, w scaled by L/kappa, then T, rho
+  ! output_binary (2DRB/3DRB.F90:1599-1627; 2D 2DRB/2DRB.F90:1357-1388): <binFolderPrefix>-<n>.bin
+  ! with u, v, w scaled by L/kappa, then T, rho (2D: u, v, T, rho); n zero-padded to 12 digits
+  ! in 3D, plain in 2D
   !------------------------------------------------------------------------------------------
   subroutine output_binary()
     real(8), allocatable, target :: rho(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:), T(:,:,:)
@@ -290,14 +434,19 @@ contains
     call lbm_require(lbm_macros(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(w), c_loc(T)), &
          'lbm_macros', h, 6)
     scale = prm%L / prm%diffusivity
-    write(filename,'(i12.12)') itc
-    open(unit=03, file='buoyancyCavity3DbinFile-'//trim(adjustl(filename))//'.bin', &
-         form='unformatted', access='sequential')
-    write(03) scale * u
-    write(03) scale * v
-    write(03) scale * w
-    write(03) T
-    write(03) rho
+    if (flowMode == 1) binFileNum = binFileNum + 1
+    if (dims == 3) then
+      write(filename,'(i12.12)') file_number()
+    else
+      write(filename,'(i0)') file_number()
+    endif
+    open(unit=03, file=trim(binFolderPrefix)//'-'//trim(adjustl(filename))//'.bin', &
+         form='unformatted', access='stream', status='replace')
+    call write_field(03, u, scale)
+    call write_field(03, v, scale)
+    if (dims == 3) call write_field(03, w, scale)
+    call write_field(03, T, 1.0d0)
+    call write_field(03, rho, 1.0d0)
     close(03)
   end subroutine output_binary
 
@@ -324,7 +473,7 @@ contains
     write(00,fmt1) 'abs(psi_center_fine) =', ps%psi_center_fine
     write(00,fmt3) 'max(|psi|) =', ps%psi_abs_max, 'x* =', ps%x_at_max, 'y* =', ps%y_at_max
     close(00)
-    if (outputPltFile == 1) then
+    if (outputPltFile == 1 .or. endOutputs == 1) then
       allocate(dat(4, nx*ny))
       do j = 1, ny
         do i = 1, nx
@@ -355,6 +504,7 @@ contains
     call lbm_require(lbm_macros(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(w), c_loc(T)), &
          'lbm_macros', h, 6)
     scale = prm%L / prm%diffusivity
+    lastPlt = itc
     write(tag,'(i12.12)') itc
     if (dims == 2) then
       allocate(dat(5, nx*ny))
@@ -422,20 +572,24 @@ contains
   end function coordinate
 
   !------------------------------------------------------------------------------------------
-  ! backupData (2DRB/3DRB.F90:1638-1665): f then g, population index outermost
+  ! backupData (2DRB/3DRB.F90:1638-1665; 2D 2DRB/2DRB.F90:1402-1430): f then g, population index
+  ! outermost, into backupFile3D-<n>.bin (2D: backupFile-<n>.bin)
   !------------------------------------------------------------------------------------------
   subroutine backupData()
     real(8), allocatable, target :: f(:,:,:,:), g(:,:,:,:)
     character(len=100) :: filename
-    integer :: i, j, k, alpha
     allocate(f(0:qf-1, nx, ny, nz), g(0:qt-1, nx, ny, nz))
     call lbm_require(lbm_gather(h, c_loc(f), c_loc(g), c_null_ptr), 'lbm_gather', h, 6)
-    write(filename,'(i0)') itc
-    open(unit=05, file='backupFile3D-'//trim(filename)//'.bin', form='unformatted', access='sequential')
-    write(05) ((((f(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qf-1)
-    write(05) ((((g(alpha,i,j,k), i=1,nx), j=1,ny), k=1,nz), alpha=0,qt-1)
+    write(filename,'(i0)') file_number()
+    filename = trim(merge('backupFile3D', 'backupFile  ', dims == 3))//'-'//trim(filename)//'.bin'
+    open(unit=05, file=trim(filename), form='unformatted', access='stream', status='replace')
+    call write_populations(05, f, qf)
+    call write_populations(05, g, qt)
     close(05)
     deallocate(f, g)
+    open(unit=00, file=trim(settingsFile), status='unknown', position='append')
+    write(00,*) 'Backup f and g to the file: ', trim(filename)
+    close(00)
   end subroutine backupData
 
 end program rb_main

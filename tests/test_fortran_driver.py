@@ -8,6 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import frecords
 import orc
 
 pytestmark = pytest.mark.gpu
@@ -16,22 +17,20 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "-2d-3d-_amd", "lib", "rb_main")
 
 
-def _read_records(path):
-    out = []
-    with open(path, "rb") as fh:
-        data = fh.read()
-    pos = 0
-    while pos < len(data):
-        n = int(np.frombuffer(data, "<i4", 1, pos)[0])
-        out.append(np.frombuffer(data, "<f8", n // 8, pos + 4))
-        assert int(np.frombuffer(data, "<i4", 1, pos + 4 + n)[0]) == n
-        pos += n + 8
-    return out
+def _read_records(path, subrecords=None):
+    """Logical records as float64 arrays (GNU Fortran's convention, tests/frecords.py); with
+    subrecords, also checks that every record of more than that many bytes was split."""
+    recs, subs = frecords.read_records(path, with_subrecords=True)
+    if subrecords is not None:
+        for r, lens in zip(recs, subs):
+            assert max(lens) <= subrecords and len(lens) == max(1, -(-len(r) // subrecords))
+    return [np.frombuffer(r, "<f8") for r in recs]
 
 
 def _run(tmp_path, nml):
     if not os.path.exists(EXE):
         pytest.skip("Fortran driver not built")
+    (tmp_path / "binFile").mkdir(exist_ok=True)   # the 2D program's binFolderPrefix folder
     (tmp_path / "rb.nml").write_text("&rb " + nml + " /\n")
     r = subprocess.run([EXE, "rb.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -77,21 +76,38 @@ def test_driver_matches_oracle(tmp_path, dims):
         assert float(got.group(2)) == pytest.approx(ps["x_at_max"], rel=1e-7, abs=1e-12)
         got = re.search(r"^abs\(psi_center_fine\) =\s+(\S+)", out, re.M)
         assert float(got.group(1)) == pytest.approx(ps["psi_center_fine"], rel=1e-7)
-    recs = _read_records(tmp_path / "backupFile3D-100.bin")
+    recs = _read_records(tmp_path / ("backupFile3D-100.bin" if dims == 3 else "backupFile-100.bin"))
     s = o.state()
     qf, qt = (19, 7) if dims == 3 else (9, 5)
     f = recs[0].reshape(qf, -1)
     g = recs[1].reshape(qt, -1)
     np.testing.assert_array_equal(f, s["f"].reshape(-1, qf).T)
     np.testing.assert_array_equal(g, s["g"].reshape(-1, qt).T)
+    # steadyFlow end of run: output_binary of the final state (2DRB/3DRB.F90:1599-1627,
+    # 2DRB/2DRB.F90:1357-1388): u, v(, w) scaled by L / kappa, then T, rho
+    p = orc.params(orc.Config(**kw))
+    scale = p["L"] / p["diffusivity"]
+    name = "buoyancyCavity3DbinFile-000000000100.bin" if dims == 3 else "binFile/buoyancyCavity-100.bin"
+    b = _read_records(tmp_path / name)
+    keys = ("u", "v", "w", "T", "rho") if dims == 3 else ("u", "v", "T", "rho")
+    assert len(b) == len(keys)
+    for rec, key in zip(b, keys):
+        want = s[key].reshape(-1) * (scale if key in ("u", "v", "w") else 1.0)
+        np.testing.assert_array_equal(rec, want, err_msg=key)
 
 
-def test_driver_restart_continues_bitwise(tmp_path):
-    base = "dims=3, nx=16, ny=12, nz=10, Rayleigh=1.0d5, Mach=0.3d0, checkInterval=1000"
+@pytest.mark.parametrize("max_sub", [2147483639, 1000])
+def test_driver_restart_continues_bitwise(tmp_path, max_sub):
+    # maxSubrecord = 1000 bytes splits the 15 KiB-per-plane records into subrecords, as records
+    # over 2 GiB split in a full-size restart file; the reload reads them back
+    base = (f"dims=3, nx=16, ny=12, nz=10, Rayleigh=1.0d5, Mach=0.3d0, checkInterval=1000, "
+            f"endOutputs=0, maxSubrecord={max_sub}")
     _run(tmp_path, base + ", itc_max=39, backupAtEnd=1")              # 40 steps -> backup
+    recs = _read_records(tmp_path / "backupFile3D-40.bin", subrecords=max_sub)
+    assert [len(r) for r in recs] == [19 * 16 * 12 * 10, 7 * 16 * 12 * 10]
     (tmp_path / "backupFile3D-40.bin").rename(tmp_path / "restart.bin")
     _run(tmp_path, base + ", itc_max=29, loadInitField=1, reloadFile='restart.bin', backupAtEnd=1")
-    recs = _read_records(tmp_path / "backupFile3D-30.bin")
+    recs = _read_records(tmp_path / "backupFile3D-30.bin", subrecords=max_sub)
     o = orc.Oracle(orc.Config(dims=3, nx=16, ny=12, nz=10, rayleigh=1e5, mach=0.3))
     o.init_seeded(20261203, 1e-3)
     o.step(40)
@@ -205,3 +221,42 @@ def test_driver_calnure_series(tmp_path, dims):
         d = o.diagnostics()
         assert nu[k, 1] == pytest.approx(d["nu_vol"], rel=1e-12, abs=1e-14), (k, n)
         assert re[k, 1] == pytest.approx(d["re_vol"], rel=1e-12, abs=1e-14), (k, n)
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_driver_unsteady_flow_cadence(tmp_path, dims):
+    # unsteadyFlow (2DRB/3DRB.F90:256-262, 2DRB/2DRB.F90:269-275): every outputFrequency
+    # free-fall times output_binary with the output count as the file number, every
+    # backupInterval free-fall times backupData with the same number; no check() calls
+    nx, ny, nz = (16, 12, 10) if dims == 3 else (40, 21, 1)
+    extra = "" if dims == 3 else ", velX=1, tempX=2, Thot=1.0d0, Tcold=0.0d0"
+    kw = dict(dims=dims, nx=nx, ny=ny, nz=nz, rayleigh=1e5, mach=0.3)
+    if dims == 2:
+        kw.update(vel_x=1, temp_x=2, thot=1.0, tcold=0.0)
+    cfg = orc.Config(**kw)
+    p = orc.params(cfg)
+    cadence = int(0.5 * p["time_unit"])
+    itc_max = 5 * cadence + 3
+    out = _run(tmp_path, f"dims={dims}, nx={nx}, ny={ny}, nz={nz}, Rayleigh=1.0d5, Mach=0.3d0, "
+                         f"itc_max={itc_max}, seed=13, flowMode=1, outputBinFile=1, "
+                         f"outputFrequency=0.5d0, backupInterval=1, checkInterval=7{extra}")
+    assert not re.search(r"^\s+\d+\s+\S+E[-+]\d+\s+\S+E[-+]\d+$", out, re.M), out   # no check()
+    o = orc.Oracle(cfg)
+    o.init_seeded(13, 1e-3)
+    scale = p["L"] / p["diffusivity"]
+    keys = ("u", "v", "w", "T", "rho") if dims == 3 else ("u", "v", "T", "rho")
+    qf = 19 if dims == 3 else 9
+    for n in range(1, 6):                       # output points at itc = n * cadence
+        o.step(cadence)
+        s = o.state()
+        name = (f"buoyancyCavity3DbinFile-{n:012d}.bin" if dims == 3 else f"binFile/buoyancyCavity-{n}.bin")
+        b = _read_records(tmp_path / name)
+        assert len(b) == len(keys)
+        for rec, key in zip(b, keys):
+            want = s[key].reshape(-1) * (scale if key in ("u", "v", "w") else 1.0)
+            np.testing.assert_array_equal(rec, want, err_msg=f"{name} {key}")
+        bak = tmp_path / (f"backupFile3D-{n}.bin" if dims == 3 else f"backupFile-{n}.bin")
+        assert bak.exists() == (n % 2 == 0), (n, bak)   # backupInterval / outputFrequency = 2
+        if n % 2 == 0:
+            np.testing.assert_array_equal(_read_records(bak)[0].reshape(qf, -1), s["f"].reshape(-1, qf).T)
+    assert not (tmp_path / (f"buoyancyCavity3DbinFile-{6:012d}.bin" if dims == 3 else "binFile/buoyancyCavity-6.bin")).exists()
