@@ -344,9 +344,11 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     }
     h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC && !h->aa;
     h->variant = LBM_KERNEL_AUTO;
-    // the one-pass form's default shape: the 34x15 tile with prefetching, the fastest one-pass
-    // form at 512^3 in FP64 (the reduced-precision forms run the two-pass form under AUTO)
-    set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_34X15, 1);
+    // the one-pass form's default shape, the fastest one-pass form at 512^3: FP64 the 34x15 tile
+    // with prefetching into registers, FP32-shift the 66x15 tile with staged prefetching (the
+    // FP16-shift form runs the two-pass form under AUTO)
+    if (c.precision == LBM_FP32_SHIFT) set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_66X15, LBM_PREFETCH_STAGED);
+    else set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_34X15, LBM_PREFETCH_REGS);
 
     int rc = LBM_OK;
     hipError_t e = hipSetDevice(c.device);
@@ -487,16 +489,20 @@ extern "C" int lbm_init_seeded(lbm_handle *h, uint64_t seed, double amp)
 }
 
 // The kernel form a step of domain h runs (after its first step, which is always two-pass):
-// ONE_PASS as requested; under AUTO the one-pass form for FP64 domains of 2^24 nodes or more
-// (1.1x faster than the two-pass form at 512^3), else the two-pass form: the reduced-precision
-// one-pass kernel is 1.3-1.8x slower at 512^3, and a smaller domain has too few one-pass tiles
-// to fill the device.  The one-pass form needs the configuration to admit it.
+// ONE_PASS as requested; under AUTO the one-pass form for FP64 and FP32-shift domains of 2^24
+// nodes or more (1.1x faster than the two-pass form at 512^3), else the two-pass form: the
+// FP16-shift one-pass kernel is about 1.7x slower at 512^3, and a smaller domain has too few
+// one-pass tiles to fill the device.  The one-pass form needs the configuration to admit it.
 constexpr uint64_t kOnePassNodes = (uint64_t)1 << 24;
+static bool auto_one_pass(const lbm_handle *h)
+{
+    return h->fused_ok && h->cfg.precision != LBM_FP16_SHIFT && h->N >= kOnePassNodes;
+}
 static bool runs_one_pass(const lbm_handle *h)
 {
     if (!h->fused_ok) return false;
     if (h->variant == LBM_KERNEL_ONE_PASS) return true;
-    return h->variant == LBM_KERNEL_AUTO && !h->lp && h->N >= kOnePassNodes;
+    return h->variant == LBM_KERNEL_AUTO && auto_one_pass(h);
 }
 
 // In-place layout.  A step is the flow collision (pass B) of the step before, then pass A, so
@@ -739,10 +745,15 @@ extern "C" int lbm_bytes_per_update(lbm_handle *h, double *bytes)
 static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefetch)
 {
     if (variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < LBM_KCHUNK_AUTO ||
-        tile < LBM_TILE_32X16 || tile > LBM_TILE_34X15 || prefetch < 0 || prefetch > 1)
+        tile < LBM_TILE_32X16 || tile > LBM_TILE_66X15 || prefetch < LBM_PREFETCH_NONE ||
+        prefetch > LBM_PREFETCH_STAGED)
         return LBM_ERR_ARG;
-    if (tile == LBM_TILE_32X32 && !h->lp)
-        return fail(h, LBM_ERR_UNSUPPORTED, "the 32x32 tile needs a reduced-precision form");
+    if ((tile == LBM_TILE_32X32 || tile == LBM_TILE_66X15) && !h->lp)
+        return fail(h, LBM_ERR_UNSUPPORTED, "the 32x32 and 66x15 tiles need a reduced-precision form");
+    if (prefetch == LBM_PREFETCH_STAGED &&
+        (h->cfg.precision != LBM_FP32_SHIFT || (tile != LBM_TILE_34X15 && tile != LBM_TILE_66X15)))
+        return fail(h, LBM_ERR_UNSUPPORTED,
+                    "staged prefetching needs the FP32-shift form and the 34x15 or 66x15 tile");
     if (variant == LBM_KERNEL_ONE_PASS && !h->fused_ok)
         return fail(h, LBM_ERR_UNSUPPORTED,
                     "one-pass form needs 3D, non-periodic thermal walls and two population arrays");
@@ -909,16 +920,20 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
         if (!rc) rc = lbm_step(t, 1);   // leave the plain state: the candidates time steady steps
         std::vector<TuneChoice> cands;
         // the incumbent: the form AUTO runs in this layout (its default one-pass shape for large
-        // FP64 domains)
-        if (!aa && t->fused_ok && !t->lp && h->N >= kOnePassNodes)
+        // FP64 and FP32-shift domains; the scratch domain is judged by the main domain's size)
+        if (!aa && t->fused_ok && h->cfg.precision != LBM_FP16_SHIFT && h->N >= kOnePassNodes)
             cands.push_back({LBM_KERNEL_ONE_PASS, t->fg.kchunk, t->fg.tile, t->fg.pf, 0});
         cands.push_back({LBM_KERNEL_TWO_PASS, h->fg.kchunk, h->fg.tile, h->fg.pf, aa ? 1 : 0});
         if (t->fused_ok)
             for (int tile = 0; tile < FUSED_TILES; ++tile)
-                for (int pf = 0; pf < 2; ++pf)
-                    if (tile != LBM_TILE_32X32 || h->lp)
+                for (int pf = LBM_PREFETCH_NONE; pf <= LBM_PREFETCH_STAGED; ++pf) {
+                    const bool lp_tile = tile == LBM_TILE_32X32 || tile == LBM_TILE_66X15;
+                    const bool staged_ok = h->cfg.precision == LBM_FP32_SHIFT &&
+                                           (tile == LBM_TILE_34X15 || tile == LBM_TILE_66X15);
+                    if ((lp_tile && !h->lp) || (pf == LBM_PREFETCH_STAGED && !staged_ok)) continue;
                     for (int kc : {8, 16, 32, 64, 128})
                         if (kc <= sc.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc, tile, pf, 0});
+                }
         const double scale = (double)h->cfg.nz / (double)sc.nz;
         for (const TuneChoice &cd : cands) {
             if (rc) break;

@@ -25,8 +25,18 @@
 //
 // Tile shapes and prefetching are compile-time variants (FusedGeom::tile / ::pf pick one at run
 // time): one 512-thread block per CU keeps every wave in step with the plane barrier, while
-// 256-thread tiles let two independent blocks share a CU, and prefetching the next plane's
-// loads overlaps them with the current flow collision.
+// 256-thread tiles let two independent blocks share a CU.  Prefetching issues the next plane's
+// loads before the current flow collision, so that their latency hides behind it:
+//   PF = 1  into registers (29 more live registers per lane);
+//   PF = 2  staged: each lane loads the elements its pulls read straight into LDS
+//           (global_load_lds_dword; a wave fills 64 consecutive slots of one population), so
+//           no register holds them in flight.  4-byte storage (the FP32 form) only.  The
+//           flow stores of a plane are deferred past the next plane's wait, which waits for
+//           every vector memory operation of the wave.
+// A tile row of 32 interior nodes spans one 128-byte line in FP32 and two in FP64; its two ring
+// columns touch the neighbouring lines, so the narrow tiles fetch up to 3x the FP32 rows they
+// use.  The 66x15 tile (64 interior columns, 16 waves, 128 registers per lane: reduced-precision
+// forms) halves that share.
 //
 // The including translation unit provides `Form` with the storage type S, arithmetic type R,
 // history type B, a per-column constant record Node, and the per-node operations (pulls, the
@@ -38,7 +48,7 @@
 namespace rblbm {
 
 // One block's work: tile number lb (tile x fastest, then tile y, then z-chunk) of one domain.
-template <class Form, int TX, int TY, bool PF>
+template <class Form, int TX, int TY, int PF>
 __device__ __forceinline__ void
 fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, unsigned lb,
            const typename Form::S *__restrict__ Fcur, typename Form::S *__restrict__ Fnext,
@@ -46,8 +56,14 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
            const typename Form::B *__restrict__ Bp, typename Form::B *__restrict__ Bpout)
 {
     using R = typename Form::R;
+    using S = typename Form::S;
     constexpr int NT = TX * TY;
     __shared__ R lds[3][4][NT];
+    // PF == 2: slot stage[a][tid] receives pulled population a (flow 0..18, thermal 19..25) of
+    // the next plane; NTP rounds the slot rows up to whole waves
+    constexpr int NTP = (NT + 63) / 64 * 64;
+    __shared__ S stage[PF == 2 ? 26 : 1][PF == 2 ? NTP : 1];
+    static_assert(PF != 2 || Form::can_stage, "staged loads need 4-byte storage");
 
     const unsigned tile_x = lb % fg.ntx;
     const unsigned rest = lb / fg.ntx;
@@ -88,17 +104,57 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         lb[2] = Bp[2 * N + c];
         return true;
     };
+    // PF == 2: the same loads, staged.  The lane's pulls go to its LDS slots (the wave's slots
+    // start at wbase); the wall mode of each thermal pull and the history stay in registers.
+    const int wbase = tid & ~63;
+    auto stage_load = [&](int kq, unsigned &modes, B(&lb)[3]) -> bool {
+        if constexpr (PF == 2) {
+            if (!active || kq < 0 || kq >= nz) return false;
+            const uint64_t c = col + plane * (uint64_t)kq;
+            const bool border = i == 0 || j == 0 || kq == 0 || i == nx - 1 || j == ny - 1 || kq == nz - 1;
+            Form::stage_f(Fcur, N, i, j, kq, c, border, kp, &stage[0][wbase], NTP);
+            modes = Form::stage_g(Gin, N, i, j, kq, c, border, kp, &stage[19][wbase], NTP);
+            lb[0] = Bp[c];
+            lb[1] = Bp[N + c];
+            lb[2] = Bp[2 * N + c];
+            return true;
+        } else {
+            return false;
+        }
+    };
     // With PF the loads of plane kk + 1 are issued before the flow part of iteration kk, so
-    // that their latency hides behind that collision (they then live across it in registers).
+    // that their latency hides behind that collision.
     R qg[7], qf[19];
     B qb[3];
-    bool q_have = PF ? load(k0 - 1, qg, qf, qb) : false;
+    unsigned qmodes = 0;
+    bool q_have = PF == 1 ? load(k0 - 1, qg, qf, qb)
+                          : (PF == 2 ? stage_load(k0 - 1, qmodes, qb) : false);
+    // PF == 2: the flow stores of plane `fpend_c`, issued after the next wait
+    R fout[PF == 2 ? 19 : 1];
+    bool fpend = false;
+    uint64_t fpend_c = 0;
 
     for (int kk = k0 - 1; kk <= k1; ++kk) {
         R lg[7], fnew[19];
         B bp[3];
         bool have;
-        if constexpr (PF) {
+        if constexpr (PF == 2) {
+            have = q_have;
+            // every vector memory operation of this wave has completed, the staged loads of
+            // plane kk among them: the LDS slots hold this lane's pulls (the compiler does not
+            // track the staged loads)
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if (fpend) {
+                Form::store_f(Fnext, N, fpend_c, fout);
+                fpend = false;
+            }
+            if (have) {
+                Form::unstage_f(&stage[0][tid], NTP, fnew);
+                Form::unstage_g(&stage[19][tid], NTP, qmodes, fp, nd, lg);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) bp[a] = qb[a];
+            }
+        } else if constexpr (PF == 1) {
             have = q_have;
 #pragma unroll
             for (int a = 0; a < 7; ++a) lg[a] = qg[a];
@@ -132,7 +188,8 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         lds[wb][2][tid] = gnew[3];
         lds[wb][3][tid] = gnew[4];
         __syncthreads();
-        if constexpr (PF) q_have = kk < k1 && load(kk + 1, qg, qf, qb);
+        if constexpr (PF == 1) q_have = kk < k1 && load(kk + 1, qg, qf, qb);
+        if constexpr (PF == 2) q_have = kk < k1 && stage_load(kk + 1, qmodes, qb);
 
         const int p = kk - 1;
         if (tile && p >= k0 && p < k1 && have_prev) {
@@ -155,7 +212,14 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
             const R T1 = Form::tsum(gin);
             R out[19];
             Form::flow(fprev, out, rho_p, u_p, v_p, w_p, T1, kp, fp, nd);
-            Form::store_f(Fnext, N, col + plane * (uint64_t)p, out);
+            if constexpr (PF == 2) {
+#pragma unroll
+                for (int a = 0; a < 19; ++a) fout[a] = out[a];
+                fpend = true;
+                fpend_c = col + plane * (uint64_t)p;
+            } else {
+                Form::store_f(Fnext, N, col + plane * (uint64_t)p, out);
+            }
         }
         // shift the plane pipeline
         g5pp = gprev[5];
@@ -170,6 +234,8 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         w_p = w_n;
         have_prev = have;
     }
+    if constexpr (PF == 2)
+        if (fpend) Form::store_f(Fnext, N, fpend_c, fout);
 }
 
 // The launch's domains and their parameters come from a table in device memory: block b (after
@@ -191,9 +257,10 @@ template <class Form> struct FusedInstance {
 };
 
 // At least two waves per SIMD: with 256 registers per lane at most, a 512-thread tile fits one
-// CU and 256-thread tiles two per CU.
-template <class Form, int TX, int TY, bool PF>
-__global__ void __launch_bounds__(TX *TY, 2)
+// CU and 256-thread tiles two per CU.  Staged (PF == 2): four waves per SIMD, two 34x15 blocks
+// per CU (staging slots and exchange buffers: 78 KB each) or one 66x15 block (154 KB).
+template <class Form, int TX, int TY, int PF>
+__global__ void __launch_bounds__(TX *TY, PF == 2 ? 4 : 2)
 k3_fused(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restrict__ owner,
          unsigned nblocks)
 {
@@ -204,7 +271,7 @@ k3_fused(const FusedInstance<Form> *__restrict__ inst, const uint16_t *__restric
 }
 
 // Run-time tile shapes: index -> (TX, TY).
-constexpr int fused_tx(int tile) { return tile == 1 ? 16 : (tile == 4 ? 34 : 32); }
+constexpr int fused_tx(int tile) { return tile == 1 ? 16 : (tile == 4 ? 34 : (tile == 5 ? 66 : 32)); }
 constexpr int fused_ty(int tile)
 {
     return tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : (tile == 3 ? 32 : 15)));
@@ -219,20 +286,38 @@ hipError_t launch_fused_table(const FusedGeom &fg0, const void *inst, const uint
 #define RBLBM_FUSED_B(T, PF)                                                                     \
     k3_fused<Form, fused_tx(T), fused_ty(T), PF><<<nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
         d, owner, nblocks)
+    constexpr bool lp = sizeof(typename Form::R) == 4;
+    if (fg0.pf == 2) {
+        // staged loads: 4-byte storage, the 34x15 and 66x15 tiles
+        if constexpr (Form::can_stage) {
+            if (fg0.tile == 4) RBLBM_FUSED_B(4, 2);
+            else if (fg0.tile == 5) RBLBM_FUSED_B(5, 2);
+            else return hipErrorInvalidValue;
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
     switch (fg0.tile * 2 + (fg0.pf ? 1 : 0)) {
-    case 0: RBLBM_FUSED_B(0, false); break;
-    case 1: RBLBM_FUSED_B(0, true); break;
-    case 2: RBLBM_FUSED_B(1, false); break;
-    case 3: RBLBM_FUSED_B(1, true); break;
-    case 4: RBLBM_FUSED_B(2, false); break;
-    case 5: RBLBM_FUSED_B(2, true); break;
-    case 8: RBLBM_FUSED_B(4, false); break;
-    case 9: RBLBM_FUSED_B(4, true); break;
+    case 0: RBLBM_FUSED_B(0, 0); break;
+    case 1: RBLBM_FUSED_B(0, 1); break;
+    case 2: RBLBM_FUSED_B(1, 0); break;
+    case 3: RBLBM_FUSED_B(1, 1); break;
+    case 4: RBLBM_FUSED_B(2, 0); break;
+    case 5: RBLBM_FUSED_B(2, 1); break;
+    case 8: RBLBM_FUSED_B(4, 0); break;
+    case 9: RBLBM_FUSED_B(4, 1); break;
     default:
-        // 32 x 32 = 1024 threads leave 128 registers per lane: reduced-precision forms only
-        if constexpr (sizeof(typename Form::R) == 4) {
-            if (fg0.pf) RBLBM_FUSED_B(3, true);
-            else RBLBM_FUSED_B(3, false);
+        // 32 x 32 = 1024 and 66 x 15 = 990 threads leave 128 registers per lane:
+        // reduced-precision forms only
+        if constexpr (lp) {
+            switch (fg0.tile * 2 + (fg0.pf ? 1 : 0)) {
+            case 6: RBLBM_FUSED_B(3, 0); break;
+            case 7: RBLBM_FUSED_B(3, 1); break;
+            case 10: RBLBM_FUSED_B(5, 0); break;
+            case 11: RBLBM_FUSED_B(5, 1); break;
+            default: return hipErrorInvalidValue;
+            }
         } else {
             return hipErrorInvalidValue;
         }

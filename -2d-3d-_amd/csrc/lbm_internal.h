@@ -46,10 +46,10 @@ hipError_t launch_state(const KParams &kp, int bx, hipStream_t s, bool three_d, 
 // tile TX x TY is one of five shapes (FusedGeom::tile): 32x16, 16x16, 32x8, 32x32 (the
 // reduced-precision forms only) and 34x15 (32 x 13 nodes: rows of 32 nodes start on a 256-byte
 // boundary, and a 512-wide domain splits into 16 tiles across).
-constexpr int FUSED_TILES = 5;
+constexpr int FUSED_TILES = 6;
 inline void fused_tile_dims(int tile, int *tx, int *ty)
 {
-    *tx = tile == 1 ? 16 : (tile == 4 ? 34 : 32);
+    *tx = tile == 1 ? 16 : (tile == 4 ? 34 : (tile == 5 ? 66 : 32));
     *ty = tile == 0 ? 16 : (tile == 1 ? 16 : (tile == 2 ? 8 : (tile == 3 ? 32 : 15)));
 }
 hipError_t launch_init(const KParams &kp, int bx, hipStream_t s, const InitParams &ip, double *F,

@@ -1163,6 +1163,7 @@ struct FormF64 {
     using R = double;
     using S = double;
     using B = double;
+    static constexpr bool can_stage = false;   // staged loads move 4-byte elements
     struct P {
         double thot, tcold, abb;
     };

@@ -76,39 +76,44 @@ template <int A> __device__ __forceinline__ float t0_pull(const LPParams &lp)
 // -------------------------------------------------------------------------------------------
 // pulls (walls as in lbm_kernels.hip: last outside axis decides for the flow lattice)
 // -------------------------------------------------------------------------------------------
+// The stored element a pull of direction A reads: its upstream neighbour's, the periodic image,
+// or (walls) the node's own population of the opposite direction.  The thermal version also
+// says how the read value becomes the incoming one: PULL_T0 (plus the shift correction),
+// PULL_BACK (adiabatic wall: the opposite population as it is), PULL_ABB (isothermal wall:
+// anti-bounce-back).
+enum : unsigned { PULL_T0 = 0, PULL_BACK = 1, PULL_ABB = 2 };
+
 template <class L, typename S, int A>
-__device__ __forceinline__ float flow_in(const S *__restrict__ P, uint64_t N, int i, int j, int k,
-                                         uint64_t c, bool border, const KParams &kp)
+__device__ __forceinline__ const S *flow_src(const S *__restrict__ P, uint64_t N, int i, int j,
+                                             int k, uint64_t c, bool border, const KParams &kp)
 {
     constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
     const int sy = kp.nx, sz = kp.nx * kp.ny;
-    if (!border) return Pop<S>::template ld<L::Q, A>(P + A * N + c - (ex + ey * sy + ez * sz));
+    if (!border) return P + A * N + c - (ex + ey * sy + ez * sz);
     const bool ox = (unsigned)(i - ex) >= (unsigned)kp.nx;
     const bool oy = (unsigned)(j - ey) >= (unsigned)kp.ny;
     const bool oz = (unsigned)(k - ez) >= (unsigned)kp.nz;
     if (oz) {
-        if (kp.vel_z == VEL_PERIODIC)
-            return Pop<S>::template ld<L::Q, A>(P + A * N + c + (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz);
-        return Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
+        if (kp.vel_z == VEL_PERIODIC) return P + A * N + c + (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz;
+        return P + L::opp[A] * N + c;
     }
-    if (oy) return Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
+    if (oy) return P + L::opp[A] * N + c;
     if (ox) {
-        if (kp.vel_x == VEL_PERIODIC)
-            return Pop<S>::template ld<L::Q, A>(P + A * N + c + (int64_t)(ex == 1 ? kp.nx - 1 - i : -i));
-        return Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
+        if (kp.vel_x == VEL_PERIODIC) return P + A * N + c + (int64_t)(ex == 1 ? kp.nx - 1 - i : -i);
+        return P + L::opp[A] * N + c;
     }
-    return Pop<S>::template ld<L::Q, A>(P + A * N + c - (int64_t)(ex + ey * sy + ez * sz));
+    return P + A * N + c - (int64_t)(ex + ey * sy + ez * sz);
 }
 
 template <class L, typename S, int A>
-__device__ __forceinline__ float heat_in(const S *__restrict__ P, uint64_t N, int i, int j, int k,
-                                         uint64_t c, bool border, const KParams &kp,
-                                         const LPParams &lp)
+__device__ __forceinline__ const S *heat_src(const S *__restrict__ P, uint64_t N, int i, int j,
+                                             int k, uint64_t c, bool border, const KParams &kp,
+                                             unsigned &mode)
 {
     constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
     const int sy = kp.nx, sz = kp.nx * kp.ny;
-    if (!border)
-        return Pop<S>::template ld<L::Q, A>(P + A * N + c - (ex + ey * sy + ez * sz)) + t0_pull<A>(lp);
+    mode = PULL_T0;
+    if (!border) return P + A * N + c - (ex + ey * sy + ez * sz);
     int kind = -1;
     int64_t wrap = 0;
     if (ex != 0) {
@@ -118,16 +123,77 @@ __device__ __forceinline__ float heat_in(const S *__restrict__ P, uint64_t N, in
     } else if (ez != 0) {
         if ((unsigned)(k - ez) >= (unsigned)kp.nz) { kind = kp.temp_z; wrap = (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz; }
     }
-    if (kind < 0)
-        return Pop<S>::template ld<L::Q, A>(P + A * N + c - (int64_t)(ex + ey * sy + ez * sz)) +
-               t0_pull<A>(lp);
+    if (kind < 0) return P + A * N + c - (int64_t)(ex + ey * sy + ez * sz);
     // a periodic pair of walls is never the shift axis (it has two isothermal walls)
-    if (kind == T_PERIODIC) return Pop<S>::template ld<L::Q, A>(P + A * N + c + wrap) + t0_pull<A>(lp);
-    const float back = Pop<S>::template ld<L::Q, L::opp[A]>(P + L::opp[A] * N + c);
-    if (kind == T_ADIABATIC) return back;
+    if (kind == T_PERIODIC) return P + A * N + c + wrap;
+    mode = kind == T_ADIABATIC ? PULL_BACK : PULL_ABB;
+    return P + L::opp[A] * N + c;
+}
+
+// Incoming thermal deviation of direction A from the value v read at heat_src (t0 = T0(i, j)).
+template <class L, int A>
+__device__ __forceinline__ float heat_fix(float v, unsigned mode, const LPParams &lp, float t0)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    if (mode == PULL_T0) return v + t0_pull<A>(lp);
+    if (mode == PULL_BACK) return v;
     constexpr bool low_wall = (ex + ey + ez) > 0;
-    const float tw = (low_wall ? lp.thot : lp.tcold) - t0_of(lp, i, j);
-    return -back + 2.0f * (float)wq<L::Q>(A) * tw;
+    const float tw = (low_wall ? lp.thot : lp.tcold) - t0;
+    return -v + 2.0f * (float)wq<L::Q>(A) * tw;
+}
+
+// Pulls (the weights of A and of its opposite are equal, so one FP16 scale serves both).
+template <class L, typename S, int A>
+__device__ __forceinline__ float flow_in(const S *__restrict__ P, uint64_t N, int i, int j, int k,
+                                         uint64_t c, bool border, const KParams &kp)
+{
+    return Pop<S>::template ld<L::Q, A>(flow_src<L, S, A>(P, N, i, j, k, c, border, kp));
+}
+
+template <class L, typename S, int A>
+__device__ __forceinline__ float heat_in(const S *__restrict__ P, uint64_t N, int i, int j, int k,
+                                         uint64_t c, bool border, const KParams &kp,
+                                         const LPParams &lp)
+{
+    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
+    if (!border)
+        return Pop<S>::template ld<L::Q, A>(P + A * N + c - (ex + ey * kp.nx + ez * kp.nx * kp.ny)) +
+               t0_pull<A>(lp);
+    unsigned mode;
+    const S *src = heat_src<L, S, A>(P, N, i, j, k, c, true, kp, mode);
+    return heat_fix<L, A>(Pop<S>::template ld<L::Q, A>(src), mode, lp, t0_of(lp, i, j));
+}
+
+// Staged pulls of the one-pass kernel (4-byte storage): every active lane loads the element its
+// pull reads straight into LDS, direction A to slot dst[A * pitch + lane] (dst: the wave's first
+// slot; the LDS address is the wave's, the lane offset is implicit).
+template <class L, typename S, int... A>
+__device__ __forceinline__ void stage_flow(const S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                           bool border, const KParams &kp, S *dst, int pitch,
+                                           std::integer_sequence<int, A...>)
+{
+    static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
+    (__builtin_amdgcn_global_load_lds(flow_src<L, S, A>(P, N, i, j, k, c, border, kp),
+                                      dst + A * pitch, 4, 0, 0), ...);
+}
+template <class L, typename S, int... A>
+__device__ __forceinline__ unsigned stage_heat(const S *P, uint64_t N, int i, int j, int k,
+                                               uint64_t c, bool border, const KParams &kp, S *dst,
+                                               int pitch, std::integer_sequence<int, A...>)
+{
+    static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
+    unsigned modes = 0, m;
+    ((__builtin_amdgcn_global_load_lds(heat_src<L, S, A>(P, N, i, j, k, c, border, kp, m),
+                                       dst + A * pitch, 4, 0, 0),
+      modes |= m << (2 * A)), ...);
+    return modes;
+}
+template <class L, int... A>
+__device__ __forceinline__ void unstage_heat(const float *slot, int pitch, unsigned modes,
+                                             const LPParams &lp, float t0, float *g,
+                                             std::integer_sequence<int, A...>)
+{
+    ((g[A] = heat_fix<L, A>(slot[A * pitch], (modes >> (2 * A)) & 3u, lp, t0)), ...);
 }
 
 // In-place (A-A) layout (LAY_ODD / LAY_EVEN, lbm_lattice.h): the same incoming values as the
@@ -895,6 +961,34 @@ template <typename SS> struct FormLP {
     static __device__ __forceinline__ void store_f(S *F, uint64_t N, uint64_t c, const float *f)
     {
         store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
+    }
+    // staged loads of the one-pass kernel (lbm_fused.h, PF == 2): 4-byte storage only
+    static constexpr bool can_stage = sizeof(S) == 4;
+    static __device__ __forceinline__ void stage_f(const S *F, uint64_t N, int i, int j, int k,
+                                                   uint64_t c, bool border, const KParams &kp,
+                                                   S *dst, int pitch)
+    {
+        if constexpr (can_stage) stage_flow<D3Q19, S>(F, N, i, j, k, c, border, kp, dst, pitch, SEQ(D3Q19));
+    }
+    // returns the wall modes of the seven thermal pulls, two bits each
+    static __device__ __forceinline__ unsigned stage_g(const S *G, uint64_t N, int i, int j, int k,
+                                                       uint64_t c, bool border, const KParams &kp,
+                                                       S *dst, int pitch)
+    {
+        if constexpr (can_stage) return stage_heat<D3Q7, S>(G, N, i, j, k, c, border, kp, dst, pitch, SEQ(D3Q7));
+        return 0;
+    }
+    static __device__ __forceinline__ void unstage_f(const S *slot, int pitch, float *f)
+    {
+        if constexpr (can_stage) {
+#pragma unroll
+            for (int a = 0; a < 19; ++a) f[a] = slot[a * pitch];
+        }
+    }
+    static __device__ __forceinline__ void unstage_g(const S *slot, int pitch, unsigned modes,
+                                                     const P &lp, const Node &nd, float *g)
+    {
+        if constexpr (can_stage) unstage_heat<D3Q7>(slot, pitch, modes, lp, nd.t0, g, SEQ(D3Q7));
     }
     static __device__ __forceinline__ float wall_g(int a, const float *gp, int kind, const P &lp,
                                                    const Node &nd)

@@ -137,8 +137,9 @@ struct FusedGeom {
     unsigned ntx, nty, nzc, nblocks;
     int kchunk;
     int tile;     // tile shape: 0 = 32 x 16 threads, 1 = 16 x 16, 2 = 32 x 8, 3 = 32 x 32,
-                  // 4 = 34 x 15
-    int pf;       // 1: issue the next plane's loads before the current flow collision
+                  // 4 = 34 x 15, 5 = 66 x 15
+    int pf;       // next plane's loads issued before the current flow collision: 1 into
+                  // registers, 2 into LDS (staged)
 };
 
 // Blocks are dealt to the 8 XCDs round-robin by hardware.  Renumber them so that each XCD

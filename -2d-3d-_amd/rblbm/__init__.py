@@ -23,8 +23,10 @@ T_CONST, T_ADIABATIC, T_PERIODIC = 0, 1, 2
 FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_ONE_PASS = 0, 1, 2
 LAYOUT_TWO_ARRAY, LAYOUT_AA, LAYOUT_AUTO = 0, 1, 2
-TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32, TILE_34X15 = 0, 1, 2, 3, 4
+TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32, TILE_34X15, TILE_66X15 = 0, 1, 2, 3, 4, 5
+PREFETCH_NONE, PREFETCH_REGS, PREFETCH_STAGED = 0, 1, 2
 KCHUNK_AUTO = -1
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_UNSUPPORTED, ERR_RHO = -1, -2, -3, -4, -5, -6
 
 EXPORTS = ("lbm_create", "lbm_destroy", "lbm_init_exact", "lbm_init_seeded", "lbm_step",
            "lbm_step_timed", "lbm_sync", "lbm_itc", "lbm_get_params", "lbm_macros", "lbm_check",
@@ -231,8 +233,9 @@ class Solver:
         self._chk(self.lib.lbm_set_kernel_variant(self._h, int(variant), int(kchunk)))
 
     def set_fused_tile(self, tile: int, prefetch: int = 0):
-        """TILE_32X16 / TILE_16X16 / TILE_32X8 / TILE_32X32 (FP32, FP16 only) thread tile of the
-        one-pass form; prefetch 0/1."""
+        """Thread tile of the one-pass form: TILE_32X16 / TILE_16X16 / TILE_32X8 / TILE_34X15, and
+        TILE_32X32 / TILE_66X15 (FP32, FP16 only); prefetch PREFETCH_NONE / PREFETCH_REGS /
+        PREFETCH_STAGED (FP32 with TILE_34X15 or TILE_66X15 only)."""
         self._chk(self.lib.lbm_set_fused_tile(self._h, int(tile), int(prefetch)))
 
     def kernel_form(self) -> dict:

@@ -62,17 +62,20 @@ def parse():
     ap.add_argument("--cpu-nz", type=int, default=64)
     ap.add_argument("--variant", choices=["auto", "two-pass", "one-pass"], default="auto",
                     help="kernel form of the step (auto: the library's choice, the one-pass form "
-                         "for FP64 and the two-pass form for FP32 / FP16 at 512^3)")
+                         "for FP64 and FP32 and the two-pass form for FP16 at 512^3)")
     ap.add_argument("--tile", type=int, default=-1,
-                    help="one-pass thread tile (0: 32x16, 1: 16x16, 2: 32x8, 3: 32x32, 4: 34x15; "
-                         "-1: the library's default, 34x15)")
+                    help="one-pass thread tile (0: 32x16, 1: 16x16, 2: 32x8, 3: 32x32, 4: 34x15, "
+                         "5: 66x15; -1: the library's default, 34x15 in FP64 and FP16, 66x15 in "
+                         "FP32)")
     ap.add_argument("--prefetch", type=int, default=-1,
-                    help="one-pass prefetching (0/1; -1: the library's default, 1)")
+                    help="one-pass prefetching (0: none, 1: into registers, 2: staged in LDS, "
+                         "FP32 only; -1: the library's default, 1 in FP64 and FP16, 2 in FP32)")
     ap.add_argument("--kchunk", type=int, default=0,
-                    help="z-planes per one-pass block (0: default, -1: from the resident blocks)")
+                    help="z-planes per one-pass block (0 or -1: the library's default, 32)")
     ap.add_argument("--layout", choices=["two-array", "aa", "auto", "tuned"], default="two-array",
-                    help="population layout (default two-array, with the default two-pass "
-                         "form); aa: one array per lattice updated in place (two-pass form only); "
+                    help="population layout (default two-array, with the library's default "
+                         "kernel form); aa: one array per lattice updated in place (two-pass form "
+                         "only); "
                          "tuned: an LBM_LAYOUT_AUTO domain whose layout and kernel form "
                          "lbm_autotune picks before the first step (--variant, --kchunk, --tile, "
                          "--prefetch are then ignored)")

@@ -193,8 +193,9 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 /* Kernel form of lbm_step (no reference counterpart: the reference runs eight sweeps).
  * TWO_PASS: a thermal and a flow launch per step.  ONE_PASS: one launch per step (3D without
  * periodic thermal walls only); it moves fewer bytes but recomputes a ring of nodes per tile.
- * AUTO (the default) runs ONE_PASS for FP64 domains of 2^24 nodes or more (the faster form at
- * 512^3 on MI355X) and TWO_PASS otherwise; lbm_autotune measures both per shape.  Both give
+ * AUTO (the default) runs ONE_PASS for FP64 and FP32-shift domains of 2^24 nodes or more (the
+ * faster form at 512^3 on MI355X) and TWO_PASS otherwise (FP16-shift: TWO_PASS at every size);
+ * lbm_autotune measures both per shape.  Both give
  * bitwise identical results in every precision form.  ONE_PASS needs the two-array layout.
  * kchunk > 0 sets the number of z-planes per block of the one-pass form, LBM_KCHUNK_AUTO the
  * default (32), 0 keeps the current value. */
@@ -205,9 +206,13 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 
 /* Thread tile of the one-pass form (no reference counterpart): 32x16 threads, 16x16 or 32x8
- * (256-thread blocks, more of them per CU), 32x32 (1024 threads, the smallest ring share;
- * FP32 / FP16 forms only, else LBM_ERR_UNSUPPORTED), or 34x15 (32 x 13 nodes, rows aligned to
- * 32 nodes); prefetch = 1 issues the next plane's loads before the current flow collision.
+ * (256-thread blocks, more of them per CU), 32x32 (1024 threads, the smallest ring share),
+ * 34x15 (32 x 13 nodes, rows aligned to 32 nodes) or 66x15 (64 x 13 nodes; the x-edges of a
+ * tile fetch half as many extra cache lines per row).  32x32 and 66x15: FP32 / FP16 forms only,
+ * else LBM_ERR_UNSUPPORTED.  prefetch = LBM_PREFETCH_REGS issues the next plane's loads into
+ * registers before the current flow collision; LBM_PREFETCH_STAGED loads them straight into
+ * LDS (FP32-shift form with the 34x15 or 66x15 tile only, else LBM_ERR_UNSUPPORTED).  Defaults:
+ * 34x15 with LBM_PREFETCH_REGS (FP64, FP16-shift), 66x15 staged (FP32-shift).
  * Results do not depend on the choice.
  * Batch domains reject this and lbm_set_kernel_variant (LBM_ERR_STATE): a batch is tuned with
  * lbm_batch_autotune or lbm_batch_set_kernel_variant. */
@@ -216,6 +221,10 @@ int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 #define LBM_TILE_32X8 2
 #define LBM_TILE_32X32 3
 #define LBM_TILE_34X15 4
+#define LBM_TILE_66X15 5
+#define LBM_PREFETCH_NONE 0
+#define LBM_PREFETCH_REGS 1
+#define LBM_PREFETCH_STAGED 2
 int lbm_set_fused_tile(lbm_handle *h, int32_t tile, int32_t prefetch);
 /* The population layout in effect (LBM_LAYOUT_TWO_ARRAY or LBM_LAYOUT_AA). */
 int lbm_get_layout(lbm_handle *h, int32_t *layout);

@@ -22,6 +22,21 @@ def test_header_declares_entry_points():
         assert n in names
 
 
+def test_python_constants_match_the_header():
+    """Every kernel-form, tile, prefetch, layout, precision and error constant of the ctypes
+    binding has the value its #define gives."""
+    import rblbm
+    text = open(HEADER).read()
+    defs = dict(re.findall(r"#define LBM_(\w+)\s+\(?(-?\d+)\)?", text))
+    names = [n for n in defs if n.split("_")[0] in ("KERNEL", "TILE", "PREFETCH", "LAYOUT", "ERR",
+                                                    "KCHUNK")]
+    assert len(names) >= 20
+    for n in names:
+        assert getattr(rblbm, n) == int(defs[n]), n
+    for n in ("FP64", "FP32_SHIFT", "FP16_SHIFT"):
+        assert getattr(rblbm, n) == int(defs[n]), n
+
+
 def test_library_exports_every_declared_symbol():
     import rblbm
     if not os.path.exists(rblbm.LIB_PATH):

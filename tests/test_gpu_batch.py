@@ -88,10 +88,12 @@ def test_batch_rejects_mixed_precision_and_owned_domain_destroy():
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
-def test_batch_fp16_table_launch_equals_per_domain(variant):
+@pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+def test_batch_lowprec_table_launch_equals_per_domain(variant, prec):
     # the table launches run the same node arithmetic as the per-domain kernels: bitwise equal
+    # (FP32 one-pass: the default 66x15 tile with staged loads, one launch for all domains)
     def run(mode):
-        b = rblbm.Batch(_cfgs(rblbm.FP16_SHIFT))
+        b = rblbm.Batch(_cfgs(prec))
         b.set_mode(mode)
         b.set_kernel_variant(variant)
         b.initial_seeded(9, 0.05)
