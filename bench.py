@@ -90,8 +90,8 @@ def parse():
                          "(counter calibration)")
     ap.add_argument("--traffic-summary", default="", metavar="DIR",
                     help="only write profiles/traffic_fp64_512.json from the rocprofv3 --pmc "
-                         "runs under DIR (fetch/, write/: the bench; copy_fetch/, copy_write/: "
-                         "the copy probe)")
+                         "runs under DIR (fetch/, write/: the bench; fetch_TAG/, write_TAG/: "
+                         "optional further bench runs; copy_fetch/, copy_write/: the copy probe)")
     return ap.parse_args()
 
 
@@ -167,24 +167,32 @@ def traffic_summary(d):
     kc = [k for k in cf if "k_copy" in k][0]
     f_fetch = nbytes / (cf[kc] * 1024.0)
     f_write = nbytes / (cw[kc] * 1024.0)
-    fetch, write = _pmc(os.path.join(d, "fetch"), "FETCH_SIZE"), _pmc(os.path.join(d, "write"), "WRITE_SIZE")
-    bench = [json.loads(l) for l in open(os.path.join(d, "fetch.log")) if l.startswith("{")][-1]
-    nodes = bench["config"]["nodes"]
-    out = {"kernel_sha": kernel_sha(), "nodes": nodes,
+    out = {"kernel_sha": kernel_sha(), "nodes": None,
            "copy_calibration": {"probe_bytes": nbytes, "fetch_kib": cf[kc], "write_kib": cw[kc],
                                 "fetch_factor": f_fetch, "write_factor": f_write},
            "forms": {}}
-    for name, f in bench["forms"].items():
-        kern = [k for k in fetch if any(p in k for p in STEP_KERNELS[(name, f["form"]["name"])])]
-        if not kern:
-            continue
-        fb = sum(fetch[k] for k in kern) * 1024.0 * f_fetch
-        wb = sum(write[k] for k in kern) * 1024.0 * f_write
-        out["forms"][f"{name}/{f['form']['name']}"] = {
-            "kernel": " + ".join(k.split("(")[0].replace("void ", "") for k in kern),
-            "fetch_bytes_per_step": fb, "write_bytes_per_step": wb, "hbm_bytes_per_step": fb + wb,
-            "algorithmic_bytes_per_step": f["bytes_per_update"] * nodes,
-            "ratio": (fb + wb) / (f["bytes_per_update"] * nodes)}
+    # fetch/, write/: the default bench run; fetch_<tag>/, write_<tag>/ (optional): further
+    # bench runs, e.g. with --variant two-pass
+    tags = [""] + sorted(n[len("fetch_"):] for n in os.listdir(d)
+                         if n.startswith("fetch_") and os.path.isdir(os.path.join(d, n)))
+    for tag in tags:
+        sfx = "_" + tag if tag else ""
+        fetch = _pmc(os.path.join(d, "fetch" + sfx), "FETCH_SIZE")
+        write = _pmc(os.path.join(d, "write" + sfx), "WRITE_SIZE")
+        bench = [json.loads(l) for l in open(os.path.join(d, f"fetch{sfx}.log")) if l.startswith("{")][-1]
+        nodes = out["nodes"] = bench["config"]["nodes"]
+        for name, f in bench["forms"].items():
+            key = f"{name}/{f['form']['name']}"
+            kern = [k for k in fetch if any(p in k for p in STEP_KERNELS[(name, f["form"]["name"])])]
+            if not kern or key in out["forms"]:
+                continue
+            fb = sum(fetch[k] for k in kern) * 1024.0 * f_fetch
+            wb = sum(write[k] for k in kern) * 1024.0 * f_write
+            out["forms"][key] = {
+                "kernel": " + ".join(k.split("(")[0].replace("void ", "") for k in kern),
+                "fetch_bytes_per_step": fb, "write_bytes_per_step": wb, "hbm_bytes_per_step": fb + wb,
+                "algorithmic_bytes_per_step": f["bytes_per_update"] * nodes,
+                "ratio": (fb + wb) / (f["bytes_per_update"] * nodes)}
     with open(TRAFFIC, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out), flush=True)
