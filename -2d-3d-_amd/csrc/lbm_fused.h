@@ -33,6 +33,13 @@
 //           no register holds them in flight.  4-byte storage (the FP32 form) only.  The
 //           flow stores of a plane are deferred past the next plane's wait, which waits for
 //           every vector memory operation of the wave.
+//   PF = 3  as PF = 1, with the flow stores of a plane deferred through LDS: the flow part
+//           writes its 19 post-collision populations to the lane's LDS slots, and the lane
+//           stores them to memory after the next plane's loads have arrived.  Vector memory
+//           operations complete in issue order, and the loop's wait for the prefetched
+//           registers otherwise also waits for the flow stores issued after those loads (the
+//           compiler takes the count of the loop's store-free path).  The slots take
+//           19 x 8 B per thread in FP64 (77.5 KB at 34x15), beside the 49 KB exchange buffers.
 // A tile row of 32 interior nodes spans one 128-byte line in FP32 and two in FP64; its two ring
 // columns touch the neighbouring lines, so the narrow tiles fetch up to 3x the FP32 rows they
 // use.  The 66x15 tile (64 interior columns, 16 waves, 128 registers per lane: reduced-precision
@@ -64,6 +71,11 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     constexpr int NTP = (NT + 63) / 64 * 64;
     __shared__ S stage[PF == 2 ? 26 : 1][PF == 2 ? NTP : 1];
     static_assert(PF != 2 || Form::can_stage, "staged loads need 4-byte storage");
+    // PF == 1 or 3: the next plane's loads go into registers; PF == 3: slot fdef[a][tid] holds
+    // the lane's post-collision flow population a until it is stored
+    constexpr bool REGS = PF == 1 || PF == 3;
+    constexpr bool DEFER_LDS = PF == 3;
+    __shared__ R fdef[DEFER_LDS ? 19 : 1][DEFER_LDS ? NT : 1];
 
     const unsigned tile_x = lb % fg.ntx;
     const unsigned rest = lb / fg.ntx;
@@ -91,48 +103,96 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     R g5pp = 0;
     bool have_prev = false, have_pp = false;
 
-    // Loads of the thermal part of plane kq: pulled g^n, pulled f^{n+1}, B_prev.
+    // Loads of the thermal part of plane kq: pulled g^n, pulled f^{n+1}, B_prev, as the stored
+    // elements (one load each at the pull's source address) and the thermal pulls' wall modes.
+    // Form::cook_f / cook_g turn them into the incoming values once they have arrived, so that
+    // no arithmetic on a loaded value sits between a prefetch and its use.
+    // A wave whose lanes are all interior nodes (their ring corners included, which load and
+    // discard) takes the interior loads: a scalar base per population and the lane's in-plane
+    // byte offset (which fits 32 bits when a plane of populations does).  Other waves choose
+    // each lane's source address with the wall rules (Form::raw_g / raw_f).  Either way every
+    // pull is one load with no branch between it and its use.
+    // Lanes outside `active` load too (their results are unused): lanes outside the domain at
+    // the clamped node, so that every lane writes the registers it prefetches into and no old
+    // value has to be kept for it.
     using B = typename Form::B;
-    auto load = [&](int kq, R(&lg)[7], R(&lf)[19], B(&lb)[3]) -> bool {
-        if (!active || kq < 0 || kq >= nz) return false;
-        const uint64_t c = col + plane * (uint64_t)kq;
-        const bool border = i == 0 || j == 0 || kq == 0 || i == nx - 1 || j == ny - 1 || kq == nz - 1;
-        Form::pull_g(Gin, N, i, j, kq, c, border, kp, fp, lg);
-        Form::pull_f(Fcur, N, i, j, kq, c, border, kp, fp, lf);
-        lb[0] = Bp[c];
-        lb[1] = Bp[N + c];
-        lb[2] = Bp[2 * N + c];
-        return true;
+    const bool col_inner = in_dom && i > 0 && j > 0 && i < nx - 1 && j < ny - 1 &&
+                           plane * sizeof(S) < ((uint64_t)1 << 32) && plane * sizeof(B) < ((uint64_t)1 << 32);
+    const uint32_t vo_s = (uint32_t)(col * sizeof(S)), vo_b = (uint32_t)(col * sizeof(B));
+    const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1);
+    const uint64_t colc = (uint64_t)ic + (uint64_t)nx * (uint64_t)jc;
+    auto load = [&](int kq, S(&lg)[7], S(&lf)[19], B(&lb)[3], unsigned &modes) -> bool {
+        if (kq < 0 || kq >= nz) return false;
+        const bool inner = col_inner && kq > 0 && kq < nz - 1;
+        if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+            modes = 0;
+            Form::raw_g_inner(Gin, N, kq, plane, vo_s, kp, lg);
+            Form::raw_f_inner(Fcur, N, kq, plane, vo_s, kp, lf);
+            const int64_t pk = (int64_t)kq * (int64_t)plane;
+            lb[0] = Form::ld_inner(Bp, pk, vo_b);
+            lb[1] = Form::ld_inner(Bp, (int64_t)N + pk, vo_b);
+            lb[2] = Form::ld_inner(Bp, 2 * (int64_t)N + pk, vo_b);
+        } else {
+            const uint64_t c = colc + plane * (uint64_t)kq;
+            modes = Form::raw_g(Gin, N, ic, jc, kq, c, true, kp, lg);
+            Form::raw_f(Fcur, N, ic, jc, kq, c, true, kp, lf);
+            lb[0] = Bp[c];
+            lb[1] = Bp[N + c];
+            lb[2] = Bp[2 * N + c];
+        }
+        return active;
     };
     // PF == 2: the same loads, staged.  The lane's pulls go to its LDS slots (the wave's slots
     // start at wbase); the wall mode of each thermal pull and the history stay in registers.
     const int wbase = tid & ~63;
+    // Interior waves and the other waves as in `load`.
     auto stage_load = [&](int kq, unsigned &modes, B(&lb)[3]) -> bool {
         if constexpr (PF == 2) {
-            if (!active || kq < 0 || kq >= nz) return false;
-            const uint64_t c = col + plane * (uint64_t)kq;
-            const bool border = i == 0 || j == 0 || kq == 0 || i == nx - 1 || j == ny - 1 || kq == nz - 1;
-            Form::stage_f(Fcur, N, i, j, kq, c, border, kp, &stage[0][wbase], NTP);
-            modes = Form::stage_g(Gin, N, i, j, kq, c, border, kp, &stage[19][wbase], NTP);
-            lb[0] = Bp[c];
-            lb[1] = Bp[N + c];
-            lb[2] = Bp[2 * N + c];
-            return true;
+            if (kq < 0 || kq >= nz) return false;
+            const bool inner = col_inner && kq > 0 && kq < nz - 1;
+            if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+                modes = 0;
+                Form::stage_f_inner(Fcur, N, kq, plane, vo_s, kp, &stage[0][wbase], NTP);
+                Form::stage_g_inner(Gin, N, kq, plane, vo_s, kp, &stage[19][wbase], NTP);
+                const int64_t pk = (int64_t)kq * (int64_t)plane;
+                lb[0] = Form::ld_inner(Bp, pk, vo_b);
+                lb[1] = Form::ld_inner(Bp, (int64_t)N + pk, vo_b);
+                lb[2] = Form::ld_inner(Bp, 2 * (int64_t)N + pk, vo_b);
+            } else {
+                const uint64_t c = colc + plane * (uint64_t)kq;
+                Form::stage_f(Fcur, N, ic, jc, kq, c, true, kp, &stage[0][wbase], NTP);
+                modes = Form::stage_g(Gin, N, ic, jc, kq, c, true, kp, &stage[19][wbase], NTP);
+                lb[0] = Bp[c];
+                lb[1] = Bp[N + c];
+                lb[2] = Bp[2 * N + c];
+            }
+            return active;
         } else {
             return false;
         }
     };
     // With PF the loads of plane kk + 1 are issued before the flow part of iteration kk, so
     // that their latency hides behind that collision.
-    R qg[7], qf[19];
-    B qb[3];
+    S qg[7] = {}, qf[19] = {};
+    B qb[3] = {};
     unsigned qmodes = 0;
-    bool q_have = PF == 1 ? load(k0 - 1, qg, qf, qb)
-                          : (PF == 2 ? stage_load(k0 - 1, qmodes, qb) : false);
-    // PF == 2: the flow stores of plane `fpend_c`, issued after the next wait
+    bool q_have = REGS ? load(k0 - 1, qg, qf, qb, qmodes)
+                       : (PF == 2 ? stage_load(k0 - 1, qmodes, qb) : false);
+    // PF == 2 / 3: the flow stores of plane `fpend_c`, issued after the next wait (PF == 2 holds
+    // them in registers, PF == 3 in LDS)
     R fout[PF == 2 ? 19 : 1];
     bool fpend = false;
     uint64_t fpend_c = 0;
+    auto store_deferred = [&]() {
+        if constexpr (DEFER_LDS) {
+            R o[19];
+#pragma unroll
+            for (int a = 0; a < 19; ++a) o[a] = fdef[a][tid];
+            Form::store_f(Fnext, N, fpend_c, o);
+        } else if constexpr (PF == 2) {
+            Form::store_f(Fnext, N, fpend_c, fout);
+        }
+    };
 
     for (int kk = k0 - 1; kk <= k1; ++kk) {
         R lg[7], fnew[19];
@@ -141,11 +201,13 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         if constexpr (PF == 2) {
             have = q_have;
             // every vector memory operation of this wave has completed, the staged loads of
-            // plane kk among them: the LDS slots hold this lane's pulls (the compiler does not
-            // track the staged loads)
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            // plane kk among them: the LDS slots hold this lane's pulls.  The builtin (vmcnt 0,
+            // the other counters at their maxima) also tells the compiler, which otherwise
+            // waits once more for the loads it saw issued into LDS at the next barrier.
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            asm volatile("" ::: "memory");
             if (fpend) {
-                Form::store_f(Fnext, N, fpend_c, fout);
+                store_deferred();
                 fpend = false;
             }
             if (have) {
@@ -154,16 +216,36 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
 #pragma unroll
                 for (int a = 0; a < 3; ++a) bp[a] = qb[a];
             }
-        } else if constexpr (PF == 1) {
+        } else if constexpr (REGS) {
             have = q_have;
+            if constexpr (DEFER_LDS) {
+                // An empty use of every prefetched register: the compiler places its waits for
+                // them here, before the deferred stores.  Placed at the first arithmetic use
+                // instead, after the stores, those waits would count the stores too (vector
+                // memory operations complete in order).  The memory clobber keeps the stores
+                // below.
 #pragma unroll
-            for (int a = 0; a < 7; ++a) lg[a] = qg[a];
+                for (int a = 0; a < 7; ++a) asm volatile("" ::"v"(qg[a]));
 #pragma unroll
-            for (int a = 0; a < 19; ++a) fnew[a] = qf[a];
+                for (int a = 0; a < 19; ++a) asm volatile("" ::"v"(qf[a]));
+#pragma unroll
+                for (int a = 0; a < 3; ++a) asm volatile("" ::"v"(qb[a]));
+                asm volatile("" ::: "memory");
+                if (fpend) {
+                    store_deferred();
+                    fpend = false;
+                }
+            }
+            Form::cook_g(qg, qmodes, fp, nd, lg);
+            Form::cook_f(qf, fnew);
 #pragma unroll
             for (int a = 0; a < 3; ++a) bp[a] = qb[a];
         } else {
-            have = load(kk, lg, fnew, bp);
+            S rg[7] = {}, rf[19] = {};
+            unsigned modes = 0;
+            have = load(kk, rg, rf, bp, modes);
+            Form::cook_g(rg, modes, fp, nd, lg);
+            Form::cook_f(rf, fnew);
         }
         R gnew[7] = {};
         R rho_n = 0, u_n = 0, v_n = 0, w_n = 0;
@@ -188,7 +270,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         lds[wb][2][tid] = gnew[3];
         lds[wb][3][tid] = gnew[4];
         __syncthreads();
-        if constexpr (PF == 1) q_have = kk < k1 && load(kk + 1, qg, qf, qb);
+        if constexpr (REGS) q_have = kk < k1 && load(kk + 1, qg, qf, qb, qmodes);
         if constexpr (PF == 2) q_have = kk < k1 && stage_load(kk + 1, qmodes, qb);
 
         const int p = kk - 1;
@@ -217,6 +299,11 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
                 for (int a = 0; a < 19; ++a) fout[a] = out[a];
                 fpend = true;
                 fpend_c = col + plane * (uint64_t)p;
+            } else if constexpr (DEFER_LDS) {
+#pragma unroll
+                for (int a = 0; a < 19; ++a) fdef[a][tid] = out[a];
+                fpend = true;
+                fpend_c = col + plane * (uint64_t)p;
             } else {
                 Form::store_f(Fnext, N, col + plane * (uint64_t)p, out);
             }
@@ -234,8 +321,8 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         w_p = w_n;
         have_prev = have;
     }
-    if constexpr (PF == 2)
-        if (fpend) Form::store_f(Fnext, N, fpend_c, fout);
+    if constexpr (PF == 2 || DEFER_LDS)
+        if (fpend) store_deferred();
 }
 
 // The launch's domains and their parameters come from a table in device memory: block b (after
@@ -287,6 +374,15 @@ hipError_t launch_fused_table(const FusedGeom &fg0, const void *inst, const uint
     k3_fused<Form, fused_tx(T), fused_ty(T), PF><<<nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
         d, owner, nblocks)
     constexpr bool lp = sizeof(typename Form::R) == 4;
+    if (fg0.pf == 3) {
+        // prefetching into registers, flow stores deferred through LDS: the 34x15 tile, and the
+        // 66x15 tile in the reduced-precision forms
+        if (fg0.tile == 4) RBLBM_FUSED_B(4, 3);
+        else if (lp && fg0.tile == 5) {
+            if constexpr (lp) RBLBM_FUSED_B(5, 3);
+        } else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (fg0.pf == 2) {
         // staged loads: 4-byte storage, the 34x15 and 66x15 tiles
         if constexpr (Form::can_stage) {

@@ -28,6 +28,7 @@
 
 #include "lbm_collide_ref.h"
 #include "lbm_internal.h"
+#include "lbm_pull.h"
 
 namespace rblbm {
 
@@ -1183,6 +1184,84 @@ struct FormF64 {
                                                   const P &, double *f)
     {
         pull_flow<D3Q19>(F, N, i, j, k, c, border, kp, f);
+    }
+    // prefetching into registers (lbm_fused.h, PF == 1 / 3): one load per pull at its source
+    // address (the same rules as pull_flow / heat_in), the thermal wall arithmetic applied after
+    // arrival from the returned modes (two bits per direction)
+    template <int... A>
+    static __device__ __forceinline__ void raw_f_seq(const double *F, uint64_t N, int i, int j,
+                                                     int k, uint64_t c, bool border,
+                                                     const KParams &kp, double *f,
+                                                     std::integer_sequence<int, A...>)
+    {
+        ((f[A] = F[flow_elem<D3Q19, A>(N, i, j, k, c, kp)]), ...);
+    }
+    static __device__ __forceinline__ void raw_f(const double *F, uint64_t N, int i, int j, int k,
+                                                 uint64_t c, bool border, const KParams &kp, double *f)
+    {
+        raw_f_seq(F, N, i, j, k, c, border, kp, f, std::make_integer_sequence<int, 19>{});
+    }
+    template <int... A>
+    static __device__ __forceinline__ unsigned raw_g_seq(const double *G, uint64_t N, int i, int j,
+                                                         int k, uint64_t c, bool border,
+                                                         const KParams &kp, double *g,
+                                                         std::integer_sequence<int, A...>)
+    {
+        unsigned modes = 0, m;
+        ((g[A] = G[heat_elem<D3Q7, A>(N, i, j, k, c, kp, m)], modes |= m << (2 * A)), ...);
+        return modes;
+    }
+    static __device__ __forceinline__ unsigned raw_g(const double *G, uint64_t N, int i, int j,
+                                                     int k, uint64_t c, bool border,
+                                                     const KParams &kp, double *g)
+    {
+        return raw_g_seq(G, N, i, j, k, c, border, kp, g, std::make_integer_sequence<int, 7>{});
+    }
+    // Interior waves (every lane an interior node of plane kq): population a of the node at
+    // in-plane byte offset vo, loaded through a buffer descriptor based at the population's
+    // upstream element for vo = 0 (buf_ld).  Thermal modes are all PULL_T0.
+    static __device__ __forceinline__ void raw_f_inner(const double *F, uint64_t N, int kq,
+                                                       uint64_t plane, uint32_t vo,
+                                                       const KParams &kp, double *f)
+    {
+        const int64_t sy = kp.nx, sz = (int64_t)kp.nx * kp.ny, pk = (int64_t)kq * (int64_t)plane;
+#pragma unroll
+        for (int a = 0; a < 19; ++a) {
+            const int64_t sh = D3Q19::ex[a] + D3Q19::ey[a] * sy + D3Q19::ez[a] * sz;
+            f[a] = buf_ld(F + ((int64_t)a * (int64_t)N + pk - sh), vo);
+        }
+    }
+    static __device__ __forceinline__ void raw_g_inner(const double *G, uint64_t N, int kq,
+                                                       uint64_t plane, uint32_t vo,
+                                                       const KParams &kp, double *g)
+    {
+        const int64_t sy = kp.nx, sz = (int64_t)kp.nx * kp.ny, pk = (int64_t)kq * (int64_t)plane;
+#pragma unroll
+        for (int a = 0; a < 7; ++a) {
+            const int64_t sh = D3Q7::ex[a] + D3Q7::ey[a] * sy + D3Q7::ez[a] * sz;
+            g[a] = buf_ld(G + ((int64_t)a * (int64_t)N + pk - sh), vo);
+        }
+    }
+    static __device__ __forceinline__ double ld_inner(const double *P, int64_t elem, uint32_t vo)
+    {
+        return buf_ld(P + elem, vo);
+    }
+    static __device__ __forceinline__ void cook_f(const double *raw, double *f)
+    {
+#pragma unroll
+        for (int a = 0; a < 19; ++a) f[a] = raw[a];
+    }
+    // as heat_in: adiabatic walls return the opposite population, isothermal walls
+    // -g_post(opp a) + abb T_wall (the hot wall at the low index)
+    static __device__ __forceinline__ void cook_g(const double *raw, unsigned modes, const P &p,
+                                                  const Node &, double *g)
+    {
+#pragma unroll
+        for (int a = 0; a < 7; ++a) {
+            const bool low_wall = (D3Q7::ex[a] + D3Q7::ey[a] + D3Q7::ez[a]) > 0;
+            const double tw = low_wall ? p.thot : p.tcold;
+            g[a] = ((modes >> (2 * a)) & 3u) == PULL_ABB ? -raw[a] + p.abb * tw : raw[a];
+        }
     }
     static __device__ __forceinline__ double tsum(const double *g)
     {

@@ -27,12 +27,14 @@
 #include <utility>
 
 #include "lbm_internal.h"
+#include "lbm_pull.h"
 
 namespace rblbm {
 
 // Storage policy: load/store one population of direction A, converting to/from FP32 deviations.
 template <typename S> struct Pop;
 template <> struct Pop<float> {
+    template <int Q, int A> static __device__ __forceinline__ float cvt(float v) { return v; }
     template <int Q, int A> static __device__ __forceinline__ float ld(const float *p) { return *p; }
     template <int Q, int A> static __device__ __forceinline__ void st(float *p, float v) { *p = v; }
 };
@@ -42,9 +44,13 @@ template <int Q> __host__ __device__ constexpr double wq(int a)
 }
 template <> struct Pop<_Float16> {
     // stored value = deviation / w_a
+    template <int Q, int A> static __device__ __forceinline__ float cvt(_Float16 v)
+    {
+        return (float)v * (float)wq<Q>(A);
+    }
     template <int Q, int A> static __device__ __forceinline__ float ld(const _Float16 *p)
     {
-        return (float)(*p) * (float)wq<Q>(A);
+        return cvt<Q, A>(*p);
     }
     template <int Q, int A> static __device__ __forceinline__ void st(_Float16 *p, float v)
     {
@@ -76,60 +82,6 @@ template <int A> __device__ __forceinline__ float t0_pull(const LPParams &lp)
 // -------------------------------------------------------------------------------------------
 // pulls (walls as in lbm_kernels.hip: last outside axis decides for the flow lattice)
 // -------------------------------------------------------------------------------------------
-// The stored element a pull of direction A reads: its upstream neighbour's, the periodic image,
-// or (walls) the node's own population of the opposite direction.  The thermal version also
-// says how the read value becomes the incoming one: PULL_T0 (plus the shift correction),
-// PULL_BACK (adiabatic wall: the opposite population as it is), PULL_ABB (isothermal wall:
-// anti-bounce-back).
-enum : unsigned { PULL_T0 = 0, PULL_BACK = 1, PULL_ABB = 2 };
-
-template <class L, typename S, int A>
-__device__ __forceinline__ const S *flow_src(const S *__restrict__ P, uint64_t N, int i, int j,
-                                             int k, uint64_t c, bool border, const KParams &kp)
-{
-    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
-    const int sy = kp.nx, sz = kp.nx * kp.ny;
-    if (!border) return P + A * N + c - (ex + ey * sy + ez * sz);
-    const bool ox = (unsigned)(i - ex) >= (unsigned)kp.nx;
-    const bool oy = (unsigned)(j - ey) >= (unsigned)kp.ny;
-    const bool oz = (unsigned)(k - ez) >= (unsigned)kp.nz;
-    if (oz) {
-        if (kp.vel_z == VEL_PERIODIC) return P + A * N + c + (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz;
-        return P + L::opp[A] * N + c;
-    }
-    if (oy) return P + L::opp[A] * N + c;
-    if (ox) {
-        if (kp.vel_x == VEL_PERIODIC) return P + A * N + c + (int64_t)(ex == 1 ? kp.nx - 1 - i : -i);
-        return P + L::opp[A] * N + c;
-    }
-    return P + A * N + c - (int64_t)(ex + ey * sy + ez * sz);
-}
-
-template <class L, typename S, int A>
-__device__ __forceinline__ const S *heat_src(const S *__restrict__ P, uint64_t N, int i, int j,
-                                             int k, uint64_t c, bool border, const KParams &kp,
-                                             unsigned &mode)
-{
-    constexpr int ex = L::ex[A], ey = L::ey[A], ez = L::ez[A];
-    const int sy = kp.nx, sz = kp.nx * kp.ny;
-    mode = PULL_T0;
-    if (!border) return P + A * N + c - (ex + ey * sy + ez * sz);
-    int kind = -1;
-    int64_t wrap = 0;
-    if (ex != 0) {
-        if ((unsigned)(i - ex) >= (unsigned)kp.nx) { kind = kp.temp_x; wrap = ex == 1 ? kp.nx - 1 - i : -i; }
-    } else if (ey != 0) {
-        if ((unsigned)(j - ey) >= (unsigned)kp.ny) { kind = kp.temp_y; wrap = (int64_t)(ey == 1 ? kp.ny - 1 - j : -j) * sy; }
-    } else if (ez != 0) {
-        if ((unsigned)(k - ez) >= (unsigned)kp.nz) { kind = kp.temp_z; wrap = (int64_t)(ez == 1 ? kp.nz - 1 - k : -k) * sz; }
-    }
-    if (kind < 0) return P + A * N + c - (int64_t)(ex + ey * sy + ez * sz);
-    // a periodic pair of walls is never the shift axis (it has two isothermal walls)
-    if (kind == T_PERIODIC) return P + A * N + c + wrap;
-    mode = kind == T_ADIABATIC ? PULL_BACK : PULL_ABB;
-    return P + L::opp[A] * N + c;
-}
-
 // Incoming thermal deviation of direction A from the value v read at heat_src (t0 = T0(i, j)).
 template <class L, int A>
 __device__ __forceinline__ float heat_fix(float v, unsigned mode, const LPParams &lp, float t0)
@@ -173,8 +125,8 @@ __device__ __forceinline__ void stage_flow(const S *P, uint64_t N, int i, int j,
                                            std::integer_sequence<int, A...>)
 {
     static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
-    (__builtin_amdgcn_global_load_lds(flow_src<L, S, A>(P, N, i, j, k, c, border, kp),
-                                      dst + A * pitch, 4, 0, 0), ...);
+    (__builtin_amdgcn_global_load_lds(P + flow_elem<L, A>(N, i, j, k, c, kp), dst + A * pitch, 4, 0,
+                                      0), ...);
 }
 template <class L, typename S, int... A>
 __device__ __forceinline__ unsigned stage_heat(const S *P, uint64_t N, int i, int j, int k,
@@ -183,10 +135,22 @@ __device__ __forceinline__ unsigned stage_heat(const S *P, uint64_t N, int i, in
 {
     static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
     unsigned modes = 0, m;
-    ((__builtin_amdgcn_global_load_lds(heat_src<L, S, A>(P, N, i, j, k, c, border, kp, m),
-                                       dst + A * pitch, 4, 0, 0),
+    ((__builtin_amdgcn_global_load_lds(P + heat_elem<L, A>(N, i, j, k, c, kp, m), dst + A * pitch, 4,
+                                       0, 0),
       modes |= m << (2 * A)), ...);
     return modes;
+}
+// Interior waves: the population's upstream element for vo = 0 as a scalar base, the lane's
+// in-plane byte offset vo as the only vector operand.
+template <class L, typename S, int... A>
+__device__ __forceinline__ void stage_inner(const S *P, uint64_t N, int kq, uint64_t plane,
+                                            uint32_t vo, const KParams &kp, S *dst, int pitch,
+                                            std::integer_sequence<int, A...>)
+{
+    static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
+    const int64_t sy = kp.nx, sz = (int64_t)kp.nx * kp.ny, pk = (int64_t)kq * (int64_t)plane;
+    (buf_ld_lds(P + ((int64_t)A * (int64_t)N + pk - (L::ex[A] + L::ey[A] * sy + L::ez[A] * sz)), vo,
+                dst + A * pitch), ...);
 }
 template <class L, int... A>
 __device__ __forceinline__ void unstage_heat(const float *slot, int pitch, unsigned modes,
@@ -194,6 +158,38 @@ __device__ __forceinline__ void unstage_heat(const float *slot, int pitch, unsig
                                              std::integer_sequence<int, A...>)
 {
     ((g[A] = heat_fix<L, A>(slot[A * pitch], (modes >> (2 * A)) & 3u, lp, t0)), ...);
+}
+
+// Pulls split at the load (the one-pass kernel's prefetching into registers): the stored
+// elements at the pulls' source addresses, and their conversion to the incoming FP32 values
+// once they have arrived.  raw + cook give what pull_flow_lp / pull_heat_lp give.
+template <class L, typename S, int... A>
+__device__ __forceinline__ void raw_flow(const S *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                         bool border, const KParams &kp, S *f,
+                                         std::integer_sequence<int, A...>)
+{
+    ((f[A] = P[flow_elem<L, A>(N, i, j, k, c, kp)]), ...);
+}
+template <class L, typename S, int... A>
+__device__ __forceinline__ unsigned raw_heat(const S *P, uint64_t N, int i, int j, int k,
+                                             uint64_t c, bool border, const KParams &kp, S *g,
+                                             std::integer_sequence<int, A...>)
+{
+    unsigned modes = 0, m;
+    ((g[A] = P[heat_elem<L, A>(N, i, j, k, c, kp, m)], modes |= m << (2 * A)), ...);
+    return modes;
+}
+template <class L, typename S, int... A>
+__device__ __forceinline__ void cook_flow(const S *raw, float *f, std::integer_sequence<int, A...>)
+{
+    ((f[A] = Pop<S>::template cvt<L::Q, A>(raw[A])), ...);
+}
+template <class L, typename S, int... A>
+__device__ __forceinline__ void cook_heat(const S *raw, unsigned modes, const LPParams &lp, float t0,
+                                          float *g, std::integer_sequence<int, A...>)
+{
+    ((g[A] = heat_fix<L, A>(Pop<S>::template cvt<L::Q, A>(raw[A]), (modes >> (2 * A)) & 3u, lp, t0)),
+     ...);
 }
 
 // In-place (A-A) layout (LAY_ODD / LAY_EVEN, lbm_lattice.h): the same incoming values as the
@@ -962,6 +958,57 @@ template <typename SS> struct FormLP {
     {
         store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
     }
+    // prefetching into registers (lbm_fused.h, PF == 1 / 3): raw loads at the pulls' source
+    // addresses (returning the thermal wall modes, two bits each), converted after arrival
+    static __device__ __forceinline__ void raw_f(const S *F, uint64_t N, int i, int j, int k,
+                                                 uint64_t c, bool border, const KParams &kp, S *f)
+    {
+        raw_flow<D3Q19, S>(F, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
+    }
+    static __device__ __forceinline__ unsigned raw_g(const S *G, uint64_t N, int i, int j, int k,
+                                                     uint64_t c, bool border, const KParams &kp, S *g)
+    {
+        return raw_heat<D3Q7, S>(G, N, i, j, k, c, border, kp, g, SEQ(D3Q7));
+    }
+    // interior waves (every lane an interior node of plane kq): global loads with the
+    // population's upstream element for vo = 0 as the scalar base and the lane's in-plane byte
+    // offset vo as the only vector operand; thermal modes all PULL_T0
+    static __device__ __forceinline__ void raw_f_inner(const S *F, uint64_t N, int kq,
+                                                       uint64_t plane, uint32_t vo,
+                                                       const KParams &kp, S *f)
+    {
+        const int64_t sy = kp.nx, sz = (int64_t)kp.nx * kp.ny, pk = (int64_t)kq * (int64_t)plane;
+#pragma unroll
+        for (int a = 0; a < 19; ++a) {
+            const int64_t sh = D3Q19::ex[a] + D3Q19::ey[a] * sy + D3Q19::ez[a] * sz;
+            f[a] = ld_inner(F, (int64_t)a * (int64_t)N + pk - sh, vo);
+        }
+    }
+    static __device__ __forceinline__ void raw_g_inner(const S *G, uint64_t N, int kq,
+                                                       uint64_t plane, uint32_t vo,
+                                                       const KParams &kp, S *g)
+    {
+        const int64_t sy = kp.nx, sz = (int64_t)kp.nx * kp.ny, pk = (int64_t)kq * (int64_t)plane;
+#pragma unroll
+        for (int a = 0; a < 7; ++a) {
+            const int64_t sh = D3Q7::ex[a] + D3Q7::ey[a] * sy + D3Q7::ez[a] * sz;
+            g[a] = ld_inner(G, (int64_t)a * (int64_t)N + pk - sh, vo);
+        }
+    }
+    template <typename T>
+    static __device__ __forceinline__ T ld_inner(const T *P, int64_t elem, uint32_t vo)
+    {
+        return buf_ld(P + elem, vo);
+    }
+    static __device__ __forceinline__ void cook_f(const S *raw, float *f)
+    {
+        cook_flow<D3Q19, S>(raw, f, SEQ(D3Q19));
+    }
+    static __device__ __forceinline__ void cook_g(const S *raw, unsigned modes, const P &lp,
+                                                  const Node &nd, float *g)
+    {
+        cook_heat<D3Q7, S>(raw, modes, lp, nd.t0, g, SEQ(D3Q7));
+    }
     // staged loads of the one-pass kernel (lbm_fused.h, PF == 2): 4-byte storage only
     static constexpr bool can_stage = sizeof(S) == 4;
     static __device__ __forceinline__ void stage_f(const S *F, uint64_t N, int i, int j, int k,
@@ -969,6 +1016,18 @@ template <typename SS> struct FormLP {
                                                    S *dst, int pitch)
     {
         if constexpr (can_stage) stage_flow<D3Q19, S>(F, N, i, j, k, c, border, kp, dst, pitch, SEQ(D3Q19));
+    }
+    static __device__ __forceinline__ void stage_f_inner(const S *F, uint64_t N, int kq,
+                                                         uint64_t plane, uint32_t vo,
+                                                         const KParams &kp, S *dst, int pitch)
+    {
+        if constexpr (can_stage) stage_inner<D3Q19, S>(F, N, kq, plane, vo, kp, dst, pitch, SEQ(D3Q19));
+    }
+    static __device__ __forceinline__ void stage_g_inner(const S *G, uint64_t N, int kq,
+                                                         uint64_t plane, uint32_t vo,
+                                                         const KParams &kp, S *dst, int pitch)
+    {
+        if constexpr (can_stage) stage_inner<D3Q7, S>(G, N, kq, plane, vo, kp, dst, pitch, SEQ(D3Q7));
     }
     // returns the wall modes of the seven thermal pulls, two bits each
     static __device__ __forceinline__ unsigned stage_g(const S *G, uint64_t N, int i, int j, int k,
