@@ -746,7 +746,7 @@ static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefet
 {
     if (variant < LBM_KERNEL_AUTO || variant > LBM_KERNEL_ONE_PASS || kchunk < LBM_KCHUNK_AUTO ||
         tile < LBM_TILE_32X16 || tile > LBM_TILE_66X15 || prefetch < LBM_PREFETCH_NONE ||
-        prefetch > LBM_PREFETCH_DEFERRED)
+        prefetch > LBM_PREFETCH_STAGED)
         return LBM_ERR_ARG;
     if ((tile == LBM_TILE_32X32 || tile == LBM_TILE_66X15) && !h->lp)
         return fail(h, LBM_ERR_UNSUPPORTED, "the 32x32 and 66x15 tiles need a reduced-precision form");
@@ -754,8 +754,6 @@ static int set_form(lbm_handle *h, int variant, int kchunk, int tile, int prefet
         (h->cfg.precision != LBM_FP32_SHIFT || (tile != LBM_TILE_34X15 && tile != LBM_TILE_66X15)))
         return fail(h, LBM_ERR_UNSUPPORTED,
                     "staged prefetching needs the FP32-shift form and the 34x15 or 66x15 tile");
-    if (prefetch == LBM_PREFETCH_DEFERRED && tile != LBM_TILE_34X15 && tile != LBM_TILE_66X15)
-        return fail(h, LBM_ERR_UNSUPPORTED, "deferred flow stores need the 34x15 or 66x15 tile");
     if (variant == LBM_KERNEL_ONE_PASS && !h->fused_ok)
         return fail(h, LBM_ERR_UNSUPPORTED,
                     "one-pass form needs 3D, non-periodic thermal walls and two population arrays");
@@ -928,13 +926,11 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
         cands.push_back({LBM_KERNEL_TWO_PASS, h->fg.kchunk, h->fg.tile, h->fg.pf, aa ? 1 : 0});
         if (t->fused_ok)
             for (int tile = 0; tile < FUSED_TILES; ++tile)
-                for (int pf = LBM_PREFETCH_NONE; pf <= LBM_PREFETCH_DEFERRED; ++pf) {
+                for (int pf = LBM_PREFETCH_NONE; pf <= LBM_PREFETCH_STAGED; ++pf) {
                     const bool lp_tile = tile == LBM_TILE_32X32 || tile == LBM_TILE_66X15;
-                    const bool wide = tile == LBM_TILE_34X15 || tile == LBM_TILE_66X15;
-                    const bool staged_ok = h->cfg.precision == LBM_FP32_SHIFT && wide;
-                    if ((lp_tile && !h->lp) || (pf == LBM_PREFETCH_STAGED && !staged_ok) ||
-                        (pf == LBM_PREFETCH_DEFERRED && !wide))
-                        continue;
+                    const bool staged_ok = h->cfg.precision == LBM_FP32_SHIFT &&
+                                           (tile == LBM_TILE_34X15 || tile == LBM_TILE_66X15);
+                    if ((lp_tile && !h->lp) || (pf == LBM_PREFETCH_STAGED && !staged_ok)) continue;
                     for (int kc : {8, 16, 32, 64, 128})
                         if (kc <= sc.nz) cands.push_back({LBM_KERNEL_ONE_PASS, kc, tile, pf, 0});
                 }

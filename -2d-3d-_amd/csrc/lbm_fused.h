@@ -33,13 +33,6 @@
 //           no register holds them in flight.  4-byte storage (the FP32 form) only.  The
 //           flow stores of a plane are deferred past the next plane's wait, which waits for
 //           every vector memory operation of the wave.
-//   PF = 3  as PF = 1, with the flow stores of a plane deferred through LDS: the flow part
-//           writes its 19 post-collision populations to the lane's LDS slots, and the lane
-//           stores them to memory after the next plane's loads have arrived.  Vector memory
-//           operations complete in issue order, and the loop's wait for the prefetched
-//           registers otherwise also waits for the flow stores issued after those loads (the
-//           compiler takes the count of the loop's store-free path).  The slots take
-//           19 x 8 B per thread in FP64 (77.5 KB at 34x15), beside the 49 KB exchange buffers.
 // A tile row of 32 interior nodes spans one 128-byte line in FP32 and two in FP64; its two ring
 // columns touch the neighbouring lines, so the narrow tiles fetch up to 3x the FP32 rows they
 // use.  The 66x15 tile (64 interior columns, 16 waves, 128 registers per lane: reduced-precision
@@ -71,11 +64,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     constexpr int NTP = (NT + 63) / 64 * 64;
     __shared__ S stage[PF == 2 ? 26 : 1][PF == 2 ? NTP : 1];
     static_assert(PF != 2 || Form::can_stage, "staged loads need 4-byte storage");
-    // PF == 1 or 3: the next plane's loads go into registers; PF == 3: slot fdef[a][tid] holds
-    // the lane's post-collision flow population a until it is stored
-    constexpr bool REGS = PF == 1 || PF == 3;
-    constexpr bool DEFER_LDS = PF == 3;
-    __shared__ R fdef[DEFER_LDS ? 19 : 1][DEFER_LDS ? NT : 1];
+    constexpr bool REGS = PF == 1;   // the next plane's loads go into registers
 
     const unsigned tile_x = lb % fg.ntx;
     const unsigned rest = lb / fg.ntx;
@@ -178,21 +167,10 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     unsigned qmodes = 0;
     bool q_have = REGS ? load(k0 - 1, qg, qf, qb, qmodes)
                        : (PF == 2 ? stage_load(k0 - 1, qmodes, qb) : false);
-    // PF == 2 / 3: the flow stores of plane `fpend_c`, issued after the next wait (PF == 2 holds
-    // them in registers, PF == 3 in LDS)
+    // PF == 2: the flow stores of plane `fpend_c`, issued after the next wait
     R fout[PF == 2 ? 19 : 1];
     bool fpend = false;
     uint64_t fpend_c = 0;
-    auto store_deferred = [&]() {
-        if constexpr (DEFER_LDS) {
-            R o[19];
-#pragma unroll
-            for (int a = 0; a < 19; ++a) o[a] = fdef[a][tid];
-            Form::store_f(Fnext, N, fpend_c, o);
-        } else if constexpr (PF == 2) {
-            Form::store_f(Fnext, N, fpend_c, fout);
-        }
-    };
 
     for (int kk = k0 - 1; kk <= k1; ++kk) {
         R lg[7], fnew[19];
@@ -207,7 +185,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
             __builtin_amdgcn_s_waitcnt(0x0F70);
             asm volatile("" ::: "memory");
             if (fpend) {
-                store_deferred();
+                Form::store_f(Fnext, N, fpend_c, fout);
                 fpend = false;
             }
             if (have) {
@@ -218,24 +196,6 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
             }
         } else if constexpr (REGS) {
             have = q_have;
-            if constexpr (DEFER_LDS) {
-                // An empty use of every prefetched register: the compiler places its waits for
-                // them here, before the deferred stores.  Placed at the first arithmetic use
-                // instead, after the stores, those waits would count the stores too (vector
-                // memory operations complete in order).  The memory clobber keeps the stores
-                // below.
-#pragma unroll
-                for (int a = 0; a < 7; ++a) asm volatile("" ::"v"(qg[a]));
-#pragma unroll
-                for (int a = 0; a < 19; ++a) asm volatile("" ::"v"(qf[a]));
-#pragma unroll
-                for (int a = 0; a < 3; ++a) asm volatile("" ::"v"(qb[a]));
-                asm volatile("" ::: "memory");
-                if (fpend) {
-                    store_deferred();
-                    fpend = false;
-                }
-            }
             Form::cook_g(qg, qmodes, fp, nd, lg);
             Form::cook_f(qf, fnew);
 #pragma unroll
@@ -299,11 +259,6 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
                 for (int a = 0; a < 19; ++a) fout[a] = out[a];
                 fpend = true;
                 fpend_c = col + plane * (uint64_t)p;
-            } else if constexpr (DEFER_LDS) {
-#pragma unroll
-                for (int a = 0; a < 19; ++a) fdef[a][tid] = out[a];
-                fpend = true;
-                fpend_c = col + plane * (uint64_t)p;
             } else {
                 Form::store_f(Fnext, N, col + plane * (uint64_t)p, out);
             }
@@ -321,8 +276,8 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         w_p = w_n;
         have_prev = have;
     }
-    if constexpr (PF == 2 || DEFER_LDS)
-        if (fpend) store_deferred();
+    if constexpr (PF == 2)
+        if (fpend) Form::store_f(Fnext, N, fpend_c, fout);
 }
 
 // The launch's domains and their parameters come from a table in device memory: block b (after
@@ -374,15 +329,6 @@ hipError_t launch_fused_table(const FusedGeom &fg0, const void *inst, const uint
     k3_fused<Form, fused_tx(T), fused_ty(T), PF><<<nblocks, fused_tx(T) * fused_ty(T), 0, s>>>( \
         d, owner, nblocks)
     constexpr bool lp = sizeof(typename Form::R) == 4;
-    if (fg0.pf == 3) {
-        // prefetching into registers, flow stores deferred through LDS: the 34x15 tile, and the
-        // 66x15 tile in the reduced-precision forms
-        if (fg0.tile == 4) RBLBM_FUSED_B(4, 3);
-        else if (lp && fg0.tile == 5) {
-            if constexpr (lp) RBLBM_FUSED_B(5, 3);
-        } else return hipErrorInvalidValue;
-        return hipGetLastError();
-    }
     if (fg0.pf == 2) {
         // staged loads: 4-byte storage, the 34x15 and 66x15 tiles
         if constexpr (Form::can_stage) {

@@ -24,7 +24,7 @@ FP64, FP32_SHIFT, FP16_SHIFT = 0, 1, 2
 KERNEL_AUTO, KERNEL_TWO_PASS, KERNEL_ONE_PASS = 0, 1, 2
 LAYOUT_TWO_ARRAY, LAYOUT_AA, LAYOUT_AUTO = 0, 1, 2
 TILE_32X16, TILE_16X16, TILE_32X8, TILE_32X32, TILE_34X15, TILE_66X15 = 0, 1, 2, 3, 4, 5
-PREFETCH_NONE, PREFETCH_REGS, PREFETCH_STAGED, PREFETCH_DEFERRED = 0, 1, 2, 3
+PREFETCH_NONE, PREFETCH_REGS, PREFETCH_STAGED = 0, 1, 2
 KCHUNK_AUTO = -1
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_UNSUPPORTED, ERR_RHO = -1, -2, -3, -4, -5, -6
 
@@ -235,9 +235,7 @@ class Solver:
     def set_fused_tile(self, tile: int, prefetch: int = 0):
         """Thread tile of the one-pass form: TILE_32X16 / TILE_16X16 / TILE_32X8 / TILE_34X15, and
         TILE_32X32 / TILE_66X15 (FP32, FP16 only); prefetch PREFETCH_NONE / PREFETCH_REGS /
-        PREFETCH_STAGED (FP32 with TILE_34X15 or TILE_66X15 only) / PREFETCH_DEFERRED (into
-        registers, flow stores held in LDS until the next plane's loads arrive; TILE_34X15 or
-        TILE_66X15 only)."""
+        PREFETCH_STAGED (FP32 with TILE_34X15 or TILE_66X15 only)."""
         self._chk(self.lib.lbm_set_fused_tile(self._h, int(tile), int(prefetch)))
 
     def kernel_form(self) -> dict:

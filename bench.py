@@ -69,8 +69,7 @@ def parse():
                          "FP32)")
     ap.add_argument("--prefetch", type=int, default=-1,
                     help="one-pass prefetching (0: none, 1: into registers, 2: staged in LDS, "
-                         "FP32 only, 3: into registers with the flow stores deferred through "
-                         "LDS; -1: the library's default, 1 in FP64 and FP16, 2 in FP32)")
+                         "FP32 only; -1: the library's default, 1 in FP64 and FP16, 2 in FP32)")
     ap.add_argument("--kchunk", type=int, default=0,
                     help="z-planes per one-pass block (0 or -1: the library's default, 32)")
     ap.add_argument("--layout", choices=["two-array", "aa", "auto", "tuned"], default="two-array",

@@ -214,10 +214,7 @@ int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
  * tile fetch half as many extra cache lines per row).  32x32 and 66x15: FP32 / FP16 forms only,
  * else LBM_ERR_UNSUPPORTED.  prefetch = LBM_PREFETCH_REGS issues the next plane's loads into
  * registers before the current flow collision; LBM_PREFETCH_STAGED loads them straight into
- * LDS (FP32-shift form with the 34x15 or 66x15 tile only, else LBM_ERR_UNSUPPORTED);
- * LBM_PREFETCH_DEFERRED prefetches into registers and holds each plane's flow populations in
- * LDS until the next plane's loads have arrived, then stores them (the 34x15 tile, and 66x15 in
- * the FP32 / FP16 forms, else LBM_ERR_UNSUPPORTED).  Defaults:
+ * LDS (FP32-shift form with the 34x15 or 66x15 tile only, else LBM_ERR_UNSUPPORTED).  Defaults:
  * 34x15 with LBM_PREFETCH_REGS (FP64, FP16-shift), 66x15 staged (FP32-shift).
  * Results do not depend on the choice.
  * Batch domains reject this and lbm_set_kernel_variant (LBM_ERR_STATE): a batch is tuned with
@@ -231,7 +228,6 @@ int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
 #define LBM_PREFETCH_NONE 0
 #define LBM_PREFETCH_REGS 1
 #define LBM_PREFETCH_STAGED 2
-#define LBM_PREFETCH_DEFERRED 3
 int lbm_set_fused_tile(lbm_handle *h, int32_t tile, int32_t prefetch);
 /* The population layout in effect (LBM_LAYOUT_TWO_ARRAY or LBM_LAYOUT_AA). */
 int lbm_get_layout(lbm_handle *h, int32_t *layout);

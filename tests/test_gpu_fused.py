@@ -50,8 +50,7 @@ def test_one_pass_fp64_bitwise(name, kchunk):
 
 TILES = [rblbm.TILE_32X16, rblbm.TILE_16X16, rblbm.TILE_32X8, rblbm.TILE_32X32,
          rblbm.TILE_34X15, rblbm.TILE_66X15]
-PREFETCH = [rblbm.PREFETCH_NONE, rblbm.PREFETCH_REGS, rblbm.PREFETCH_STAGED,
-            rblbm.PREFETCH_DEFERRED]
+PREFETCH = [rblbm.PREFETCH_NONE, rblbm.PREFETCH_REGS, rblbm.PREFETCH_STAGED]
 
 
 def _form_supported(tile, prefetch, prec):
@@ -59,8 +58,6 @@ def _form_supported(tile, prefetch, prec):
         return False                                 # 128 registers per lane: FP64 does not fit
     if prefetch == rblbm.PREFETCH_STAGED:
         return prec == rblbm.FP32_SHIFT and tile in (rblbm.TILE_34X15, rblbm.TILE_66X15)
-    if prefetch == rblbm.PREFETCH_DEFERRED:
-        return tile in (rblbm.TILE_34X15, rblbm.TILE_66X15)
     return True
 
 
