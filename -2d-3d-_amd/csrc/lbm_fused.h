@@ -133,29 +133,21 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     };
     // PF == 2: the same loads, staged.  The lane's pulls go to its LDS slots (the wave's slots
     // start at wbase); the wall mode of each thermal pull and the history stay in registers.
+    // Their destination is LDS, so a branch around them holds no register, and they keep one
+    // address per active lane: loading every lane, interior waves through buffer descriptors
+    // as in `load`, read 18% more from memory (26.9 instead of 22.6 GB per 512^3 FP32 step).
     const int wbase = tid & ~63;
-    // Interior waves and the other waves as in `load`.
     auto stage_load = [&](int kq, unsigned &modes, B(&lb)[3]) -> bool {
         if constexpr (PF == 2) {
-            if (kq < 0 || kq >= nz) return false;
-            const bool inner = col_inner && kq > 0 && kq < nz - 1;
-            if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
-                modes = 0;
-                Form::stage_f_inner(Fcur, N, kq, plane, vo_s, kp, &stage[0][wbase], NTP);
-                Form::stage_g_inner(Gin, N, kq, plane, vo_s, kp, &stage[19][wbase], NTP);
-                const int64_t pk = (int64_t)kq * (int64_t)plane;
-                lb[0] = Form::ld_inner(Bp, pk, vo_b);
-                lb[1] = Form::ld_inner(Bp, (int64_t)N + pk, vo_b);
-                lb[2] = Form::ld_inner(Bp, 2 * (int64_t)N + pk, vo_b);
-            } else {
-                const uint64_t c = colc + plane * (uint64_t)kq;
-                Form::stage_f(Fcur, N, ic, jc, kq, c, true, kp, &stage[0][wbase], NTP);
-                modes = Form::stage_g(Gin, N, ic, jc, kq, c, true, kp, &stage[19][wbase], NTP);
-                lb[0] = Bp[c];
-                lb[1] = Bp[N + c];
-                lb[2] = Bp[2 * N + c];
-            }
-            return active;
+            if (!active || kq < 0 || kq >= nz) return false;
+            const uint64_t c = col + plane * (uint64_t)kq;
+            const bool border = i == 0 || j == 0 || kq == 0 || i == nx - 1 || j == ny - 1 || kq == nz - 1;
+            Form::stage_f(Fcur, N, i, j, kq, c, border, kp, &stage[0][wbase], NTP);
+            modes = Form::stage_g(Gin, N, i, j, kq, c, border, kp, &stage[19][wbase], NTP);
+            lb[0] = Bp[c];
+            lb[1] = Bp[N + c];
+            lb[2] = Bp[2 * N + c];
+            return true;
         } else {
             return false;
         }

@@ -125,8 +125,8 @@ __device__ __forceinline__ void stage_flow(const S *P, uint64_t N, int i, int j,
                                            std::integer_sequence<int, A...>)
 {
     static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
-    (__builtin_amdgcn_global_load_lds(P + flow_elem<L, A>(N, i, j, k, c, kp), dst + A * pitch, 4, 0,
-                                      0), ...);
+    (__builtin_amdgcn_global_load_lds(flow_src<L, S, A>(P, N, i, j, k, c, border, kp),
+                                      dst + A * pitch, 4, 0, 0), ...);
 }
 template <class L, typename S, int... A>
 __device__ __forceinline__ unsigned stage_heat(const S *P, uint64_t N, int i, int j, int k,
@@ -135,22 +135,10 @@ __device__ __forceinline__ unsigned stage_heat(const S *P, uint64_t N, int i, in
 {
     static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
     unsigned modes = 0, m;
-    ((__builtin_amdgcn_global_load_lds(P + heat_elem<L, A>(N, i, j, k, c, kp, m), dst + A * pitch, 4,
-                                       0, 0),
+    ((__builtin_amdgcn_global_load_lds(heat_src<L, S, A>(P, N, i, j, k, c, border, kp, m),
+                                       dst + A * pitch, 4, 0, 0),
       modes |= m << (2 * A)), ...);
     return modes;
-}
-// Interior waves: the population's upstream element for vo = 0 as a scalar base, the lane's
-// in-plane byte offset vo as the only vector operand.
-template <class L, typename S, int... A>
-__device__ __forceinline__ void stage_inner(const S *P, uint64_t N, int kq, uint64_t plane,
-                                            uint32_t vo, const KParams &kp, S *dst, int pitch,
-                                            std::integer_sequence<int, A...>)
-{
-    static_assert(sizeof(S) == 4, "staged loads move 4-byte elements");
-    const int64_t sy = kp.nx, sz = (int64_t)kp.nx * kp.ny, pk = (int64_t)kq * (int64_t)plane;
-    (buf_ld_lds(P + ((int64_t)A * (int64_t)N + pk - (L::ex[A] + L::ey[A] * sy + L::ez[A] * sz)), vo,
-                dst + A * pitch), ...);
 }
 template <class L, int... A>
 __device__ __forceinline__ void unstage_heat(const float *slot, int pitch, unsigned modes,
@@ -1016,18 +1004,6 @@ template <typename SS> struct FormLP {
                                                    S *dst, int pitch)
     {
         if constexpr (can_stage) stage_flow<D3Q19, S>(F, N, i, j, k, c, border, kp, dst, pitch, SEQ(D3Q19));
-    }
-    static __device__ __forceinline__ void stage_f_inner(const S *F, uint64_t N, int kq,
-                                                         uint64_t plane, uint32_t vo,
-                                                         const KParams &kp, S *dst, int pitch)
-    {
-        if constexpr (can_stage) stage_inner<D3Q19, S>(F, N, kq, plane, vo, kp, dst, pitch, SEQ(D3Q19));
-    }
-    static __device__ __forceinline__ void stage_g_inner(const S *G, uint64_t N, int kq,
-                                                         uint64_t plane, uint32_t vo,
-                                                         const KParams &kp, S *dst, int pitch)
-    {
-        if constexpr (can_stage) stage_inner<D3Q7, S>(G, N, kq, plane, vo, kp, dst, pitch, SEQ(D3Q7));
     }
     // returns the wall modes of the seven thermal pulls, two bits each
     static __device__ __forceinline__ unsigned stage_g(const S *G, uint64_t N, int i, int j, int k,

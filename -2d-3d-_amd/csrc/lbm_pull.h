@@ -30,14 +30,6 @@ template <typename T> __device__ __forceinline__ T buf_ld(const T *base, uint32_
     else
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(base), vo, 0, 0));
 }
-// The same element staged into LDS: the wave's 64 lanes fill 64 consecutive 4-byte slots from
-// `dst` (4-byte elements).
-template <typename T> __device__ __forceinline__ void buf_ld_lds(const T *base, uint32_t vo, T *dst)
-{
-    static_assert(sizeof(T) == 4, "staged loads move 4-byte elements");
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc_at(base), dst, 4, vo, 0, 0, 0);
-}
-
 // The stored element a pull of direction A reads: its upstream neighbour's, the periodic image,
 // or (walls) the node's own population of the opposite direction.  The thermal version also
 // says how the read value becomes the incoming one: PULL_T0 (plus the shift correction),
