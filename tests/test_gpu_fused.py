@@ -172,3 +172,36 @@ def test_one_pass_round_schedule_covers_every_tile(prec, shape):
     a, b = one.gather(), two.gather()
     for k in ("f", "g", "bx", "by", "bz"):
         np.testing.assert_array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP64, rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+@pytest.mark.parametrize("tile, prefetch", [(rblbm.TILE_34X15, rblbm.PREFETCH_REGS),
+                                            (rblbm.TILE_34X15, rblbm.PREFETCH_NONE),
+                                            (rblbm.TILE_66X15, rblbm.PREFETCH_REGS)])
+def test_one_pass_interior_waves(prec, tile, prefetch):
+    """A grid on which whole waves hold interior nodes only, so that their pulls take the
+    buffer-descriptor loads (34x15 tiles 1-2 cover x = 31..96 and tile row 1 covers
+    y = 12..26; the 66x15 tile 1 covers x = 63..128), beside waves with wall and out-of-domain
+    lanes: FP64 bitwise against the oracle, FP32 / FP16 bitwise against the two-pass form."""
+    if tile == rblbm.TILE_66X15 and prec == rblbm.FP64:
+        pytest.skip("the 66x15 tile is a reduced-precision tile")
+    dims = dict(dims=3, nx=140, ny=44, nz=24, rayleigh=1e5, mach=0.3, precision=prec)
+    sol = rblbm.Solver(rblbm.Config(**dims))
+    sol.set_kernel_variant(rblbm.KERNEL_ONE_PASS, 8)
+    sol.set_fused_tile(tile, prefetch)
+    sol.initial_seeded(6, 0.2)
+    sol.step(9)
+    if prec == rblbm.FP64:
+        ora = orc.Oracle(orc.Config(**{k: v for k, v in dims.items() if k != "precision"}))
+        ora.init_seeded(6, 0.2)
+        ora.step(9)
+        s = ora.state()
+    else:
+        ref = rblbm.Solver(rblbm.Config(**dims))
+        ref.set_kernel_variant(rblbm.KERNEL_TWO_PASS)
+        ref.initial_seeded(6, 0.2)
+        ref.step(9)
+        s = ref.gather()
+    a = sol.gather()
+    for k in ("f", "g", "bx", "by", "bz"):
+        np.testing.assert_array_equal(a[k], s[k])
