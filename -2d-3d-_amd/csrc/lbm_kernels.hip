@@ -1173,18 +1173,6 @@ struct FormF64 {
     // a neighbour's post-collision population of direction a pulled into this node: unchanged
     static __device__ __forceinline__ double pulled(int, double v, const P &) { return v; }
     static __device__ __forceinline__ void round_g(double *) {}
-    static __device__ __forceinline__ void pull_g(const double *G, uint64_t N, int i, int j, int k,
-                                                  uint64_t c, bool border, const KParams &kp,
-                                                  const P &, double *g)
-    {
-        pull_heat<D3Q7>(G, N, i, j, k, c, border, kp, g);
-    }
-    static __device__ __forceinline__ void pull_f(const double *F, uint64_t N, int i, int j, int k,
-                                                  uint64_t c, bool border, const KParams &kp,
-                                                  const P &, double *f)
-    {
-        pull_flow<D3Q19>(F, N, i, j, k, c, border, kp, f);
-    }
     // prefetching into registers (lbm_fused.h, PF == 1 / 3): one load per pull at its source
     // address (the same rules as pull_flow / heat_in), the thermal wall arithmetic applied after
     // arrival from the returned modes (two bits per direction)

@@ -150,7 +150,7 @@ __device__ __forceinline__ void unstage_heat(const float *slot, int pitch, unsig
 
 // Pulls split at the load (the one-pass kernel's prefetching into registers): the stored
 // elements at the pulls' source addresses, and their conversion to the incoming FP32 values
-// once they have arrived.  raw + cook give what pull_flow_lp / pull_heat_lp give.
+// once they have arrived.  raw + cook give what flow_in / heat_in give.
 template <class L, typename S, int... A>
 __device__ __forceinline__ void raw_flow(const S *P, uint64_t N, int i, int j, int k, uint64_t c,
                                          bool border, const KParams &kp, S *f,
@@ -274,13 +274,6 @@ __device__ __forceinline__ void out_lp(S *P, uint64_t N, int i, int j, int k, ui
 }
 constexpr int lay_flip_lp(int lay) { return lay == LAY_PULL ? LAY_PULL : 3 - lay; }
 
-template <class L, typename S, int... A>
-__device__ __forceinline__ void pull_flow_lp(const S *__restrict__ P, uint64_t N, int i, int j, int k,
-                                             uint64_t c, bool border, const KParams &kp, float *f,
-                                             std::integer_sequence<int, A...>)
-{
-    ((f[A] = flow_in<L, S, A>(P, N, i, j, k, c, border, kp)), ...);
-}
 template <class L, typename S, int... A>
 __device__ __forceinline__ void pull_heat_lp(const S *__restrict__ P, uint64_t N, int i, int j, int k,
                                              uint64_t c, bool border, const KParams &kp,
@@ -894,18 +887,6 @@ template <typename SS> struct FormLP {
     {
         const int e = lp.t0axis == 1 ? D3Q7::ex[a] : (lp.t0axis == 2 ? D3Q7::ey[a] : 0);
         return v + -(float)w7(a) * ((float)e * lp.t0s);
-    }
-    static __device__ __forceinline__ void pull_g(const S *G, uint64_t N, int i, int j, int k,
-                                                  uint64_t c, bool border, const KParams &kp,
-                                                  const P &lp, float *g)
-    {
-        pull_heat_lp<D3Q7, S>(G, N, i, j, k, c, border, kp, lp, g, SEQ(D3Q7));
-    }
-    static __device__ __forceinline__ void pull_f(const S *F, uint64_t N, int i, int j, int k,
-                                                  uint64_t c, bool border, const KParams &kp,
-                                                  const P &, float *f)
-    {
-        pull_flow_lp<D3Q19, S>(F, N, i, j, k, c, border, kp, f, SEQ(D3Q19));
     }
     // temperature deviation dT = T - T0(x)
     static __device__ __forceinline__ float tsum(const float *g)
