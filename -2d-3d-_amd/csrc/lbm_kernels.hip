@@ -1165,6 +1165,7 @@ struct FormF64 {
     using S = double;
     using B = double;
     static constexpr bool can_stage = false;   // staged loads move 4-byte elements
+    static constexpr bool ring_last = false;   // one-pass thread order: row-major tile
     struct P {
         double thot, tcold, abb;
     };

@@ -978,6 +978,8 @@ template <typename SS> struct FormLP {
     {
         cook_heat<D3Q7, S>(raw, modes, lp, nd.t0, g, SEQ(D3Q7));
     }
+    // one-pass thread order (lbm_fused.h): interior rows first, ring nodes in waves of their own
+    static constexpr bool ring_last = true;
     // staged loads of the one-pass kernel (lbm_fused.h, PF == 2): 4-byte storage only
     static constexpr bool can_stage = sizeof(S) == 4;
     static __device__ __forceinline__ void stage_f(const S *F, uint64_t N, int i, int j, int k,
