@@ -345,9 +345,10 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC && !h->aa;
     h->variant = LBM_KERNEL_AUTO;
     // the one-pass form's default shape, the fastest one-pass form at 512^3: FP64 the 34x15 tile
-    // with prefetching into registers, FP32-shift the 66x15 tile with staged prefetching (the
-    // FP16-shift form runs the two-pass form under AUTO)
-    if (c.precision == LBM_FP32_SHIFT) set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_66X15, LBM_PREFETCH_STAGED);
+    // with prefetching into registers; FP32-shift the 66x15 tile without prefetching (95
+    // registers, no spills: 6.72-6.94 ms per step against 6.93-7.58 ms staged, alternating
+    // processes on one box); the FP16-shift form runs the two-pass form under AUTO
+    if (c.precision == LBM_FP32_SHIFT) set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_66X15, LBM_PREFETCH_NONE);
     else set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_34X15, LBM_PREFETCH_REGS);
 
     int rc = LBM_OK;

@@ -21,7 +21,10 @@ Prints ONE JSON line:
              and WRITE_SIZE, each scaled by its reading for a copy kernel of known size;
              profiles/traffic_fp64_512.json), null unless that summary was measured on the
              kernel sources of this tree; copy_gbs / read_gbs = the hand-written streaming copy
-             and read kernels of hbm_probe.hip in the same run (the practical ceiling)
+             and read kernels of hbm_probe.hip in the same run (the practical ceiling);
+             tile_copy_gbs = a copy of 29 FP64 arrays of the same grid in the one-pass step's
+             access pattern (32 x 13 tiles marching 32-plane chunks), which moves the FP64
+             step's algorithmic bytes with no arithmetic
   configs    the other benchmark configurations' rates: C5 (1024^3 Ra=1e10 FP16-shift) and C4
              (the seeded 64-cavity FP32-shift ensemble: HIP-graph table launches against one
              launch per cavity)
@@ -110,6 +113,19 @@ def hbm_probe(gib=4.0, reps=5):
     if rc != 0:
         raise RuntimeError(f"hbm_probe failed: HIP error {rc}")
     return cp.value, rd.value
+
+
+def hbm_probe_tile(n, reps=3):
+    """The step's access pattern without arithmetic (hbm_probe_tile): GB/s over an n^3 grid."""
+    import ctypes as C
+
+    lib = C.CDLL(PROBE_LIB)
+    lib.hbm_probe_tile.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    g = C.c_double()
+    rc = lib.hbm_probe_tile(0, n, n, n, reps, C.byref(g))
+    if rc != 0:
+        raise RuntimeError(f"hbm_probe_tile failed: HIP error {rc}")
+    return g.value
 
 
 def copy_probe(gib):
@@ -434,6 +450,7 @@ def main():
         f["nu_vol"] = f.pop("diag")["nu_vol"]
 
     copy_gbs, read_gbs = hbm_probe()
+    tile_gbs = hbm_probe_tile(n)
     traffic, traffic_src = (None, "not measured at this size / form")
     if n == 512:
         traffic, traffic_src = measured_traffic(main_name, main_form["form"]["name"])
@@ -468,6 +485,7 @@ def main():
                      "bytes_per_update": bpu, "algorithmic_bytes_per_step": bpu * nodes,
                      "kernel": main_form["form"]["kernels"], "copy_gbs": copy_gbs,
                      "read_gbs": read_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "tile_copy_gbs": tile_gbs, "frac_of_tile_copy": achieved / tile_gbs,
                      "kernel_sha": kernel_sha()},
         "configs": configs,
         "cpu_baseline": cpu,

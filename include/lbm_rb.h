@@ -215,7 +215,7 @@ int lbm_set_kernel_variant(lbm_handle *h, int32_t variant, int32_t kchunk);
  * else LBM_ERR_UNSUPPORTED.  prefetch = LBM_PREFETCH_REGS issues the next plane's loads into
  * registers before the current flow collision; LBM_PREFETCH_STAGED loads them straight into
  * LDS (FP32-shift form with the 34x15 or 66x15 tile only, else LBM_ERR_UNSUPPORTED).  Defaults:
- * 34x15 with LBM_PREFETCH_REGS (FP64, FP16-shift), 66x15 staged (FP32-shift).
+ * 34x15 with LBM_PREFETCH_REGS (FP64, FP16-shift), 66x15 with LBM_PREFETCH_NONE (FP32-shift).
  * Results do not depend on the choice.
  * Batch domains reject this and lbm_set_kernel_variant (LBM_ERR_STATE): a batch is tuned with
  * lbm_batch_autotune or lbm_batch_set_kernel_variant. */

@@ -91,7 +91,7 @@ def test_batch_rejects_mixed_precision_and_owned_domain_destroy():
 @pytest.mark.parametrize("prec", [rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
 def test_batch_lowprec_table_launch_equals_per_domain(variant, prec):
     # the table launches run the same node arithmetic as the per-domain kernels: bitwise equal
-    # (FP32 one-pass: the default 66x15 tile with staged loads, one launch for all domains)
+    # (FP32 one-pass: the default 66x15 tile without prefetching, one launch for all domains)
     def run(mode):
         b = rblbm.Batch(_cfgs(prec))
         b.set_mode(mode)

@@ -91,17 +91,21 @@ def test_c3_512_fp64_forms_mass_and_mirror_symmetry():
         gc.collect()
 
 
-def test_c3_512_fp32_staged_one_pass_matches_two_pass():
-    """At 512^3 the FP32-shift form runs the one-pass kernel with the 66x15 tile and staged
-    loads by default; 100 steps of it equal 100 steps of the two-pass form bit for bit."""
+def test_c3_512_fp32_one_pass_forms_match_two_pass():
+    """At 512^3 the FP32-shift form runs the one-pass kernel with the 66x15 tile and no
+    prefetching by default; 100 steps of it, and of the staged-load form, equal 100 steps of the
+    two-pass form bit for bit."""
     ref = None
-    for variant in (rblbm.KERNEL_AUTO, rblbm.KERNEL_TWO_PASS):
+    for variant, prefetch in ((rblbm.KERNEL_AUTO, None), (rblbm.KERNEL_ONE_PASS, rblbm.PREFETCH_STAGED),
+                              (rblbm.KERNEL_TWO_PASS, None)):
         sol = rblbm.Solver(_cavity(N3, 1e8, precision=rblbm.FP32_SHIFT))
         sol.set_kernel_variant(variant)
         if variant == rblbm.KERNEL_AUTO:
             assert sol.kernel_form() == dict(variant=rblbm.KERNEL_ONE_PASS, kchunk=32,
                                              tile=rblbm.TILE_66X15,
-                                             prefetch=rblbm.PREFETCH_STAGED)
+                                             prefetch=rblbm.PREFETCH_NONE)
+        if prefetch is not None:
+            sol.set_fused_tile(rblbm.TILE_66X15, prefetch)
         sol.initial_seeded(SEED, 1e-3)
         sol.step(100)
         m = sol.macros()

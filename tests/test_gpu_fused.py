@@ -61,17 +61,20 @@ def _form_supported(tile, prefetch, prec):
     return True
 
 
-@pytest.mark.parametrize("tile", TILES)
-@pytest.mark.parametrize("prefetch", PREFETCH)
-@pytest.mark.parametrize("prec", [rblbm.FP64, rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
+# FP16 runs the tiles of the default and wide shapes only (the others share its code)
+TILE_CASES = [(tile, prefetch, prec)
+              for prec in (rblbm.FP64, rblbm.FP32_SHIFT, rblbm.FP16_SHIFT)
+              for prefetch in PREFETCH for tile in TILES
+              if prec != rblbm.FP16_SHIFT or tile in (rblbm.TILE_34X15, rblbm.TILE_66X15)]
+
+
+@pytest.mark.parametrize("tile, prefetch, prec", TILE_CASES)
 def test_one_pass_tiles_and_prefetch(tile, prefetch, prec):
     """Every tile shape and prefetching mode gives the default kernel's results: bitwise
     against the oracle in FP64, bitwise against the two-pass form in FP32 and FP16 (the same
     per-node arithmetic, products fused only inside a source expression, and the exchanged
     thermal populations rounded as the storage rounds them).  Unsupported combinations are
     refused with LBM_ERR_UNSUPPORTED."""
-    if prec == rblbm.FP16_SHIFT and tile not in (rblbm.TILE_34X15, rblbm.TILE_66X15):
-        pytest.skip("FP16: the tiles of the default and wide shapes (the others share its code)")
     dims = dict(dims=3, nx=45, ny=37, nz=29, rayleigh=1e5, mach=0.3, precision=prec)
     sol = rblbm.Solver(rblbm.Config(**dims))
     sol.set_kernel_variant(rblbm.KERNEL_ONE_PASS, 7)
@@ -174,17 +177,18 @@ def test_one_pass_round_schedule_covers_every_tile(prec, shape):
         np.testing.assert_array_equal(a[k], b[k])
 
 
-@pytest.mark.parametrize("prec", [rblbm.FP64, rblbm.FP32_SHIFT, rblbm.FP16_SHIFT])
-@pytest.mark.parametrize("tile, prefetch", [(rblbm.TILE_34X15, rblbm.PREFETCH_REGS),
-                                            (rblbm.TILE_34X15, rblbm.PREFETCH_NONE),
-                                            (rblbm.TILE_66X15, rblbm.PREFETCH_REGS)])
+# the 66x15 tile is a reduced-precision tile
+@pytest.mark.parametrize("prec, tile, prefetch", [
+    (prec, tile, prefetch)
+    for tile, prefetch in ((rblbm.TILE_34X15, rblbm.PREFETCH_REGS), (rblbm.TILE_34X15, rblbm.PREFETCH_NONE),
+                           (rblbm.TILE_66X15, rblbm.PREFETCH_REGS))
+    for prec in (rblbm.FP64, rblbm.FP32_SHIFT, rblbm.FP16_SHIFT)
+    if not (tile == rblbm.TILE_66X15 and prec == rblbm.FP64)])
 def test_one_pass_interior_waves(prec, tile, prefetch):
     """A grid on which whole waves hold interior nodes only, so that their pulls take the
     buffer-descriptor loads (34x15 tiles 1-2 cover x = 31..96 and tile row 1 covers
     y = 12..26; the 66x15 tile 1 covers x = 63..128), beside waves with wall and out-of-domain
     lanes: FP64 bitwise against the oracle, FP32 / FP16 bitwise against the two-pass form."""
-    if tile == rblbm.TILE_66X15 and prec == rblbm.FP64:
-        pytest.skip("the 66x15 tile is a reduced-precision tile")
     dims = dict(dims=3, nx=140, ny=44, nz=24, rayleigh=1e5, mach=0.3, precision=prec)
     sol = rblbm.Solver(rblbm.Config(**dims))
     sol.set_kernel_variant(rblbm.KERNEL_ONE_PASS, 8)
