@@ -123,8 +123,9 @@ def hbm_probe_tile(n, reps=3):
     lib.hbm_probe_tile.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
     g = C.c_double()
     rc = lib.hbm_probe_tile(0, n, n, n, reps, C.byref(g))
-    if rc != 0:
-        raise RuntimeError(f"hbm_probe_tile failed: HIP error {rc}")
+    if rc != 0:   # e.g. 2 x 29 arrays of n^3 doubles do not fit the device
+        print(f"bench: tile-pattern copy not measured (HIP error {rc})", file=sys.stderr, flush=True)
+        return None
     return g.value
 
 
@@ -485,7 +486,8 @@ def main():
                      "bytes_per_update": bpu, "algorithmic_bytes_per_step": bpu * nodes,
                      "kernel": main_form["form"]["kernels"], "copy_gbs": copy_gbs,
                      "read_gbs": read_gbs, "frac_of_copy": achieved / copy_gbs,
-                     "tile_copy_gbs": tile_gbs, "frac_of_tile_copy": achieved / tile_gbs,
+                     "tile_copy_gbs": tile_gbs,
+                     "frac_of_tile_copy": achieved / tile_gbs if tile_gbs else None,
                      "kernel_sha": kernel_sha()},
         "configs": configs,
         "cpu_baseline": cpu,

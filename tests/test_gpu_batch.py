@@ -183,3 +183,35 @@ def test_batch_domain_refuses_state_changes_of_its_own():
         np.testing.assert_array_equal(dd.gather()["f"], o.state()["f"])
         np.testing.assert_array_equal(dd.gather()["g"], o.state()["g"])
     b.close()
+
+
+def test_batch_autotune_per_shape_cache_and_bitwise_state(tmp_path):
+    # lbm_batch_autotune tunes each distinct shape once (one cache line per shape), gives the
+    # domains of one shape the same form, gives every domain domain 0's tile and prefetch (one
+    # launch has one block shape), reads the cache on a second call, and leaves the results
+    # bitwise those of the untuned batch (every form runs the same node arithmetic)
+    shapes = [(24, 20, 16, 1e5, 0.71), (32, 32, 32, 1e6, 7.0), (24, 20, 16, 3e4, 7.0)]
+    cfgs = [rblbm.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=ra, prandtl=pr, precision=rblbm.FP32_SHIFT)
+            for a, b, c, ra, pr in shapes]
+    cache = str(tmp_path / "batch_tune.txt")
+    tuned = rblbm.Batch(cfgs)
+    tuned.initial_seeded(23, 0.05)
+    tuned.autotune(cache, steps=2)
+    lines = [l for l in open(cache).read().split("\n") if l]
+    assert sorted(l.split()[0].split("_p")[0] for l in lines) == ["d3_24x20x16", "d3_32x32x32"]
+    forms = [d.kernel_form() for d in tuned.domains]
+    assert forms[0] == forms[2]
+    assert all((f["tile"], f["prefetch"]) == (forms[0]["tile"], forms[0]["prefetch"]) for f in forms)
+    tuned.autotune(cache, steps=2)                      # from the cache: nothing appended
+    assert len([l for l in open(cache).read().split("\n") if l]) == len(lines)
+    plain = rblbm.Batch(cfgs)
+    plain.initial_seeded(23, 0.05)
+    for b in (tuned, plain):
+        b.set_mode(rblbm.BATCH_ONE_LAUNCH)
+        b.step(9)
+    for d1, d2 in zip(tuned.domains, plain.domains):
+        s1, s2 = d1.gather(), d2.gather()
+        for k in ("f", "g", "bx", "by", "bz"):
+            np.testing.assert_array_equal(s1[k], s2[k])
+    tuned.close()
+    plain.close()
