@@ -171,10 +171,20 @@ static double node_bytes(const lbm_config &c, bool aa, bool fused)
     return (aa ? 1 : 2) * q * es + (fused ? 6 : 3) * ea + (c.dims == 2 ? 2 : 1) * ea + 40.0;
 }
 
-// Default z-chunk of the one-pass form.  Measured at 512^3 FP64 with the 34x15 tile (XCD-remapped
-// order: each XCD's blocks a contiguous z-slab): 16.8 ms per step with chunks of 8 planes, 14.9 at
-// 32, 15.2 at 64, 16.1 at 128, 18.6 at 256.
-static int default_kchunk(int nz) { return nz >= 32 ? 32 : nz; }
+// Default z-chunk of the one-pass form: the largest power of two up to nz / 16, within 8..32
+// planes (nz itself below 8).  Measured with the default tiles, ms per step:
+//   512^3 FP64: 16.8 with chunks of 8 planes, 14.3-15.0 at 16, 14.1-14.9 at 32, 15.1-15.2 at 64;
+//   512^3 FP32: 7.1-7.2 at 16, 6.7-6.9 at 32, 6.9-7.8 at 64;
+//   256^3 FP64: 2.14 at 8, 1.91 at 16, 1.99 at 32; FP32: 0.93, 0.89, 0.96;
+//   192^3 FP64: 0.90 at 8, 0.92 at 16, 1.00 at 32; FP32: 0.467, 0.469, 0.563.
+// Shorter chunks give a smaller domain enough blocks to fill the device (the 66x15 FP32 tile
+// at 256^3 has 80 tiles a chunk) at the price of more halo planes.
+static int default_kchunk(int nz)
+{
+    int k = 8;
+    while (k < 32 && 2 * k <= nz / 16) k *= 2;
+    return k > nz ? nz : k;
+}
 
 // One-pass launch geometry for z-chunks of `kchunk` planes (LBM_KCHUNK_AUTO: default_kchunk)
 // and thread tile shape `tile`.
@@ -490,14 +500,20 @@ extern "C" int lbm_init_seeded(lbm_handle *h, uint64_t seed, double amp)
 }
 
 // The kernel form a step of domain h runs (after its first step, which is always two-pass):
-// ONE_PASS as requested; under AUTO the one-pass form for FP64 and FP32-shift domains of 2^24
-// nodes or more (1.1x faster than the two-pass form at 512^3), else the two-pass form: the
-// FP16-shift one-pass kernel is about 1.7x slower at 512^3, and a smaller domain has too few
-// one-pass tiles to fill the device.  The one-pass form needs the configuration to admit it.
-constexpr uint64_t kOnePassNodes = (uint64_t)1 << 24;
+// ONE_PASS as requested; under AUTO the one-pass form for FP64 domains of 2^22 nodes or more
+// and FP32-shift domains of 2^24 nodes or more, else the two-pass form.  Measured, ms per step
+// (two-pass against one-pass in its default shape): FP64 512^3 16.4 / 14.2, 256^3 2.28 / 1.91,
+// 192^3 0.98 / 0.90, 128^3 0.301 / 0.302; FP32 512^3 8.1 / 6.8, 256^3 1.11 / 0.89, 192^3
+// 0.455 / 0.467, 128^3 0.143 / 0.174.  The FP16-shift one-pass kernel is 1.2x slower at 512^3.
+// The one-pass form needs the configuration to admit it.
+static uint64_t one_pass_min_nodes(int precision)
+{
+    return precision == LBM_FP64 ? (uint64_t)1 << 22 : (uint64_t)1 << 24;
+}
 static bool auto_one_pass(const lbm_handle *h)
 {
-    return h->fused_ok && h->cfg.precision != LBM_FP16_SHIFT && h->N >= kOnePassNodes;
+    return h->fused_ok && h->cfg.precision != LBM_FP16_SHIFT &&
+           h->N >= one_pass_min_nodes(h->cfg.precision);
 }
 static bool runs_one_pass(const lbm_handle *h)
 {
@@ -922,7 +938,8 @@ static int autotune_impl(lbm_handle *h, const char *cache_path, int32_t steps, d
         std::vector<TuneChoice> cands;
         // the incumbent: the form AUTO runs in this layout (its default one-pass shape for large
         // FP64 and FP32-shift domains; the scratch domain is judged by the main domain's size)
-        if (!aa && t->fused_ok && h->cfg.precision != LBM_FP16_SHIFT && h->N >= kOnePassNodes)
+        if (!aa && t->fused_ok && h->cfg.precision != LBM_FP16_SHIFT &&
+            h->N >= one_pass_min_nodes(h->cfg.precision))
             cands.push_back({LBM_KERNEL_ONE_PASS, t->fg.kchunk, t->fg.tile, t->fg.pf, 0});
         cands.push_back({LBM_KERNEL_TWO_PASS, h->fg.kchunk, h->fg.tile, h->fg.pf, aa ? 1 : 0});
         if (t->fused_ok)

@@ -26,7 +26,8 @@ Prints ONE JSON line:
              access pattern (32 x 13 tiles marching 32-plane chunks), which moves the FP64
              step's algorithmic bytes with no arithmetic
   configs    the other benchmark configurations' rates: C5 (1024^3 Ra=1e10 FP16-shift) and C4
-             (the seeded 64-cavity FP32-shift ensemble: HIP-graph table launches against one
+             (the seeded 64-cavity FP32-shift ensemble: HIP-graph table launches, in the
+             default forms and in the forms lbm_batch_autotune picks per shape, against one
              launch per cavity)
   cpu_baseline  the FP64 OpenMP oracle (a restatement of the reference's CPU path) timed on
              this host over a bounded sample of the same workload
@@ -393,8 +394,20 @@ def c4_rate(rblbm, steps):
         b.step(4)
         ms = b.step_timed(steps)
         out[name] = {"glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms / steps}
-    out["glups"] = out["graph"]["glups"]
-    out["graph_vs_per_cavity"] = out["graph"]["glups"] / out["per_cavity"]["glups"]
+    # the autotuner's choice per shape (lbm_batch_autotune, no cache), then graph replay again
+    t0 = time.perf_counter()
+    b.autotune(None, steps=4)
+    tune_s = time.perf_counter() - t0
+    b.set_mode(rblbm.BATCH_GRAPH)
+    b.step(4)
+    ms = b.step_timed(steps)
+    forms = [d.kernel_form() for d in b.domains]
+    out["graph_autotuned"] = {
+        "glups": nodes * steps / (ms / 1e3) / 1e9, "ms_per_step": ms / steps, "tune_seconds": tune_s,
+        "one_pass_domains": sum(f["variant"] == rblbm.KERNEL_ONE_PASS for f in forms),
+        "tile": forms[0]["tile"], "prefetch": forms[0]["prefetch"]}
+    out["glups"] = out["graph_autotuned"]["glups"]
+    out["graph_vs_per_cavity"] = out["graph_autotuned"]["glups"] / out["per_cavity"]["glups"]
     b.close()
     return out
 

@@ -196,12 +196,13 @@ int lbm_bytes_per_update(lbm_handle *h, double *bytes);
 /* Kernel form of lbm_step (no reference counterpart: the reference runs eight sweeps).
  * TWO_PASS: a thermal and a flow launch per step.  ONE_PASS: one launch per step (3D without
  * periodic thermal walls only); it moves fewer bytes but recomputes a ring of nodes per tile.
- * AUTO (the default) runs ONE_PASS for FP64 and FP32-shift domains of 2^24 nodes or more (the
- * faster form at 512^3 on MI355X) and TWO_PASS otherwise (FP16-shift: TWO_PASS at every size);
- * lbm_autotune measures both per shape.  Both give
+ * AUTO (the default) runs ONE_PASS for FP64 domains of 2^22 nodes or more and FP32-shift
+ * domains of 2^24 nodes or more (the faster form there on MI355X) and TWO_PASS otherwise
+ * (FP16-shift: TWO_PASS at every size); lbm_autotune measures both per shape.  Both give
  * bitwise identical results in every precision form.  ONE_PASS needs the two-array layout.
  * kchunk > 0 sets the number of z-planes per block of the one-pass form, LBM_KCHUNK_AUTO the
- * default (32), 0 keeps the current value. */
+ * default (the largest power of two up to nz / 16 within 8..32, nz below 8), 0 keeps the
+ * current value. */
 #define LBM_KERNEL_AUTO 0
 #define LBM_KERNEL_TWO_PASS 1
 #define LBM_KERNEL_ONE_PASS 2

@@ -209,3 +209,16 @@ def test_one_pass_interior_waves(prec, tile, prefetch):
     a = sol.gather()
     for k in ("f", "g", "bx", "by", "bz"):
         np.testing.assert_array_equal(a[k], s[k])
+
+
+@pytest.mark.parametrize("n, prec, variant, kchunk", [
+    (128, rblbm.FP64, rblbm.KERNEL_TWO_PASS, 8), (192, rblbm.FP64, rblbm.KERNEL_ONE_PASS, 8),
+    (256, rblbm.FP64, rblbm.KERNEL_ONE_PASS, 16), (192, rblbm.FP32_SHIFT, rblbm.KERNEL_TWO_PASS, 8),
+    (256, rblbm.FP32_SHIFT, rblbm.KERNEL_ONE_PASS, 16), (256, rblbm.FP16_SHIFT, rblbm.KERNEL_TWO_PASS, 16)])
+def test_auto_form_by_domain_size(n, prec, variant, kchunk):
+    """AUTO runs the one-pass form for FP64 domains of 2^22 nodes or more and FP32-shift domains
+    of 2^24 or more, with z-chunks of the largest power of two up to nz / 16 within 8..32."""
+    sol = rblbm.Solver(rblbm.Config(dims=3, nx=n, ny=n, nz=n, rayleigh=1e6, precision=prec))
+    f = sol.kernel_form()
+    assert (f["variant"], f["kchunk"]) == (variant, kchunk)
+    sol.close()
