@@ -186,6 +186,13 @@ static int default_kchunk(int nz)
     return k > nz ? nz : k;
 }
 
+// The one-pass form's default tile and prefetching per precision form (see create_impl).
+static void default_tile(int precision, int *tile, int *prefetch)
+{
+    *tile = precision == LBM_FP32_SHIFT ? LBM_TILE_66X15 : LBM_TILE_34X15;
+    *prefetch = precision == LBM_FP32_SHIFT ? LBM_PREFETCH_NONE : LBM_PREFETCH_REGS;
+}
+
 // One-pass launch geometry for z-chunks of `kchunk` planes (LBM_KCHUNK_AUTO: default_kchunk)
 // and thread tile shape `tile`.
 static void set_fused_geom(lbm_handle *h, int kchunk, int tile, int prefetch)
@@ -358,8 +365,9 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
     // with prefetching into registers; FP32-shift the 66x15 tile without prefetching (95
     // registers, no spills: 6.72-6.94 ms per step against 6.93-7.58 ms staged, alternating
     // processes on one box); the FP16-shift form runs the two-pass form under AUTO
-    if (c.precision == LBM_FP32_SHIFT) set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_66X15, LBM_PREFETCH_NONE);
-    else set_fused_geom(h, LBM_KCHUNK_AUTO, LBM_TILE_34X15, LBM_PREFETCH_REGS);
+    int tile0, pf0;
+    default_tile(c.precision, &tile0, &pf0);
+    set_fused_geom(h, LBM_KCHUNK_AUTO, tile0, pf0);
 
     int rc = LBM_OK;
     hipError_t e = hipSetDevice(c.device);
@@ -2118,6 +2126,70 @@ extern "C" int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp)
     return LBM_OK;
 }
 
+// The per-shape choices of lbm_batch_autotune time each shape alone, where a small cavity
+// cannot fill the device with one-pass tiles; a one-launch step of the whole batch can.  This
+// times the batch's one-launch step with those choices against every domain in the one-pass
+// form (domain 0's one-pass tile and prefetch when it chose the one-pass form, else the
+// precision form's default; each domain's one-pass z-chunk, else the default for its depth) on
+// a scratch batch of the same configurations, and switches to the one-pass form when that is
+// more than 3% faster.  Skipped when a domain cannot run the one-pass form, when every domain
+// already does, or when free device memory cannot hold the scratch batch.
+static int batch_tune_table_form(lbm_batch *b, int steps)
+{
+    lbm_handle *h0 = b->dom[0];
+    bool all_one = true;
+    size_t need = 0;
+    for (int i = 0; i < b->n; ++i) {
+        if (!b->dom[i]->fused_ok) return LBM_OK;
+        all_one = all_one && b->dom[i]->variant == LBM_KERNEL_ONE_PASS;
+        need += b->dom[i]->dev_bytes;
+    }
+    if (all_one) return LBM_OK;
+    size_t free_b = 0, all_b = 0;
+    if (hipMemGetInfo(&free_b, &all_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return LBM_OK;
+    }
+    if ((double)need > 0.8 * (double)free_b) return LBM_OK;
+    int tile0, pf0;
+    if (h0->variant == LBM_KERNEL_ONE_PASS) {
+        tile0 = h0->fg.tile;
+        pf0 = h0->fg.pf;
+    } else {
+        default_tile(b->precision, &tile0, &pf0);
+    }
+    auto one_pass_kchunk = [](const lbm_handle *h) {
+        return h->variant == LBM_KERNEL_ONE_PASS ? h->fg.kchunk : default_kchunk(h->cfg.nz);
+    };
+    std::vector<lbm_config> cf(b->n);
+    for (int i = 0; i < b->n; ++i) cf[i] = b->dom[i]->cfg;
+    lbm_batch *t = nullptr;
+    int rc = lbm_batch_create(cf.data(), b->n, &t);
+    if (rc == LBM_ERR_NOMEM) return LBM_OK;   // no room for the scratch batch after all
+    if (rc) return fail(h0, rc, "batch autotune: scratch batch: %s", lbm_error_string(rc));
+    t->mode = LBM_BATCH_ONE_LAUNCH;
+    rc = lbm_batch_init_seeded(t, 1, 1e-3);
+    double ms[2] = {0.0, 0.0};
+    for (int c = 0; c < 2 && !rc; ++c) {
+        for (int i = 0; i < b->n && !rc; ++i) {
+            const lbm_handle *h = b->dom[i];
+            rc = c == 0 ? set_form(t->dom[i], h->variant, h->fg.kchunk, h->fg.tile, h->fg.pf)
+                        : set_form(t->dom[i], LBM_KERNEL_ONE_PASS, one_pass_kchunk(h), tile0, pf0);
+        }
+        batch_invalidate(t);
+        if (!rc) rc = lbm_batch_step(t, 2);   // the first leaves the plain state
+        if (!rc) rc = lbm_batch_step_timed(t, steps, &ms[c]);
+    }
+    lbm_batch_destroy(t);
+    if (rc) return fail(h0, rc, "batch autotune: timing failed");
+    if (ms[1] < 0.97 * ms[0])
+        for (int i = 0; i < b->n && !rc; ++i) {
+            lbm_handle *h = b->dom[i];
+            rc = set_form(h, LBM_KERNEL_ONE_PASS, one_pass_kchunk(h), tile0, pf0);
+        }
+    return rc;
+}
+
 extern "C" int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t steps)
 {
     if (!b || steps <= 0) return LBM_ERR_ARG;
@@ -2142,8 +2214,9 @@ extern "C" int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t 
         int rc = set_form(h, h->variant, h->fg.kchunk, b->dom[0]->fg.tile, b->dom[0]->fg.pf);
         if (rc) return rc;
     }
+    int rc = batch_tune_table_form(b, steps);
     batch_invalidate(b);
-    return LBM_OK;
+    return rc;
 }
 
 extern "C" int lbm_batch_set_kernel_variant(lbm_batch *b, int32_t variant)
@@ -2166,6 +2239,25 @@ extern "C" int lbm_batch_set_mode(lbm_batch *b, int32_t mode)
     return LBM_OK;
 }
 
+// The form of a one-launch step: one-pass when every domain admits it and none is set to
+// TWO_PASS, and either every domain is set to ONE_PASS or, under AUTO, the batch as a whole has
+// the node count at which a single domain runs the one-pass form: the combined launch fills the
+// device although each cavity alone would not.  The seeded 64-cavity ensemble (edges 32-128,
+// 44.6 M nodes), graph replay: FP32-shift 2.77 ms per step one-pass against 2.99 two-pass, FP64
+// 5.29 against 6.25.
+static bool batch_table_one_pass(const lbm_batch *b)
+{
+    uint64_t total = 0;
+    bool all_one = true;
+    for (int i = 0; i < b->n; ++i) {
+        const lbm_handle *h = b->dom[i];
+        if (!h->fused_ok || h->variant == LBM_KERNEL_TWO_PASS) return false;
+        all_one = all_one && h->variant == LBM_KERNEL_ONE_PASS;
+        total += h->N;
+    }
+    return all_one || (b->precision != LBM_FP16_SHIFT && total >= one_pass_min_nodes(b->precision));
+}
+
 // Blocks of one domain in the batch's form.
 static unsigned batch_blocks(const lbm_batch *b, const lbm_handle *h)
 {
@@ -2178,9 +2270,7 @@ static unsigned batch_blocks(const lbm_batch *b, const lbm_handle *h)
 static int batch_tables(lbm_batch *b)
 {
     lbm_handle *h0 = b->dom[0];
-    b->twopass = false;
-    for (int i = 0; i < b->n; ++i)
-        if (b->dom[i]->variant != LBM_KERNEL_ONE_PASS) b->twopass = true;
+    b->twopass = !batch_table_one_pass(b);
     const size_t isz = !b->twopass ? fused_entry_bytes(h0)
                        : h0->lp    ? twopass_entry_bytes_lp(b->precision)
                                    : twopass_entry_bytes_f64();

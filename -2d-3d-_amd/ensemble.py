@@ -4,12 +4,15 @@
 Ensemble: 64 cubic cavities, edge drawn uniformly (seeded) from {32, 48, 64, 96, 128},
 Ra log-uniform in [1e4, 1e7], Pr in {0.71, 7.0}, FP32-shifted storage, seeded initial noise
 (instance index in the noise key).  Aggregate GLUPS = sum of nodes * steps / device seconds,
-timed for each kernel form (two_pass, one_pass) and stepping mode of the batch:
+timed for each kernel form (auto: the library's default, which runs the one-launch modes in
+the one-pass form once the batch's total node count reaches 2^22 FP64 or 2^24 FP32-shift, and
+each cavity in its own form in per_domain mode; two_pass; one_pass) and stepping mode of the
+batch:
   per_domain   one launch per cavity per pass
   one_launch   one launch per pass for all cavities (per-cavity parameter/offset table)
   graph        one_launch, two steps captured in a HIP graph and replayed
-and, with the diagnostics of every cavity every --every steps (two-pass form, wall-clock time
-including the diagnostics' host work):
+and, with the diagnostics of every cavity every --every steps (the default form, wall-clock
+time including the diagnostics' host work):
   diag_eager   lbm_batch_step + lbm_diagnostics per cavity (one synchronisation per cavity)
   diag_graph   lbm_batch_run_diagnostics (steps and reductions of a period replayed from a
                HIP graph; one synchronisation per run)
@@ -56,7 +59,8 @@ def main():
     out = {"workload": f"{a.n} cavities, edges {sorted(set(c.nx for c in cfgs))}, "
                        f"Ra log-uniform [1e4,1e7], Pr in {{0.71,7}}, {a.precision}",
            "nodes": nodes, "steps": a.steps, "modes": {}}
-    for form, variant in (("two_pass", rblbm.KERNEL_TWO_PASS), ("one_pass", rblbm.KERNEL_ONE_PASS)):
+    for form, variant in (("auto", rblbm.KERNEL_AUTO), ("two_pass", rblbm.KERNEL_TWO_PASS),
+                          ("one_pass", rblbm.KERNEL_ONE_PASS)):
         b.set_kernel_variant(variant)
         for name, mode in (("per_domain", rblbm.BATCH_PER_DOMAIN),
                            ("one_launch", rblbm.BATCH_ONE_LAUNCH),
@@ -68,7 +72,7 @@ def main():
             out["modes"][key] = {"ms_per_step": ms / a.steps,
                                  "glups": nodes * a.steps / (ms / 1e3) / 1e9}
             print(f"{key}: {out['modes'][key]}", file=sys.stderr, flush=True)
-    b.set_kernel_variant(rblbm.KERNEL_TWO_PASS)
+    b.set_kernel_variant(rblbm.KERNEL_AUTO)
     b.set_mode(rblbm.BATCH_GRAPH)
     b.step(a.warmup)
     nsamp = max(1, a.steps // a.every)
@@ -90,7 +94,7 @@ def main():
     out["domain0_nu_vol"] = d["nu_vol"]
     out["graph_vs_per_domain"] = {
         f: out["modes"][f + "/graph"]["glups"] / out["modes"][f + "/per_domain"]["glups"]
-        for f in ("two_pass", "one_pass")}
+        for f in ("auto", "two_pass", "one_pass")}
     b.close()
     print(json.dumps(out), flush=True)
 

@@ -255,10 +255,13 @@ int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *m
  * lbm_step_timed, lbm_init_exact, lbm_init_seeded, lbm_scatter, lbm_set_kernel_variant,
  * lbm_autotune, lbm_run_diagnostics, lbm_destroy) return LBM_ERR_STATE for a batch domain: the
  * batch's launch tables describe its buffers, and only the lbm_batch_* calls keep them current.
- * The one-launch modes run the two-pass form (two launches a step over 256-node blocks of all
- * domains) unless every domain is set to LBM_KERNEL_ONE_PASS (lbm_batch_set_kernel_variant or
- * lbm_batch_autotune), then the one-pass form (one launch a step; the tile shape and prefetch
- * of domain 0 for all, each domain's own z-chunk). */
+ * The one-launch modes run the one-pass form (one launch a step; the tile shape and prefetch of
+ * domain 0 for all, each domain's own z-chunk) when every domain admits it, none is set to
+ * LBM_KERNEL_TWO_PASS, and either every domain is set to LBM_KERNEL_ONE_PASS
+ * (lbm_batch_set_kernel_variant or lbm_batch_autotune) or, under AUTO, the batch's total node
+ * count reaches the size at which AUTO runs a single domain in the one-pass form (2^22 FP64,
+ * 2^24 FP32-shift; never FP16-shift); otherwise the two-pass form (two launches a step over
+ * 256-node blocks of all domains).  One launch per domain runs each domain's own form. */
 typedef struct lbm_batch lbm_batch;
 #define LBM_BATCH_PER_DOMAIN 0    /* one launch per domain per step */
 #define LBM_BATCH_ONE_LAUNCH 1    /* one launch per step for all domains (per-domain table) */
@@ -277,7 +280,12 @@ int lbm_batch_nodes(lbm_batch *b, int64_t *nodes);
  * period holds the batch's steps in its mode (table launches, or per-domain launches in
  * LBM_BATCH_PER_DOMAIN) and every domain's diagnostics reductions. */
 int lbm_batch_run_diagnostics(lbm_batch *b, int32_t nsteps, int32_t every, lbm_diag *out);
-/* lbm_autotune for each distinct domain shape (the one-launch modes use the tuned z-chunks). */
+/* lbm_autotune for each distinct domain shape (the one-launch modes use the tuned z-chunks),
+ * then, unless every domain already runs the one-pass form, the batch's one-launch step with
+ * those choices is timed against every domain in the one-pass form on a scratch batch of the
+ * same configurations (when free device memory holds it), and the one-pass form is taken if it
+ * is more than 3% faster: a small cavity alone cannot fill the device with one-pass tiles, the
+ * whole batch can.  The per-shape choices go to the cache; the batch-level choice does not. */
 int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t steps);
 
 const char *lbm_last_error(lbm_handle *h);

@@ -215,3 +215,26 @@ def test_batch_autotune_per_shape_cache_and_bitwise_state(tmp_path):
             np.testing.assert_array_equal(s1[k], s2[k])
     tuned.close()
     plain.close()
+
+
+@pytest.mark.parametrize("prec", [rblbm.FP64, rblbm.FP32_SHIFT])
+def test_batch_auto_table_form_by_total_size(prec):
+    # under AUTO a one-launch step runs the one-pass table form once the batch as a whole has
+    # 2^22 (FP64) or 2^24 (FP32-shift) nodes, though each cavity alone runs the two-pass form:
+    # bitwise the per-domain steps (FP64 also the oracle's)
+    shapes = [(128, 128, 128, 1e6, 0.71), (128, 128, 128, 3e5, 7.0)] if prec == rblbm.FP64 else \
+             [(256, 256, 128, 1e6, 0.71), (256, 256, 128, 3e5, 7.0)]
+    cfgs = [rblbm.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=ra, prandtl=pr, precision=prec)
+            for a, b, c, ra, pr in shapes]
+    out = []
+    for mode in (rblbm.BATCH_GRAPH, rblbm.BATCH_PER_DOMAIN):
+        b = rblbm.Batch(cfgs)
+        assert all(d.kernel_form()["variant"] == rblbm.KERNEL_TWO_PASS for d in b.domains)
+        b.set_mode(mode)
+        b.initial_seeded(41, 0.05)
+        b.step(11)
+        out.append([d.gather() for d in b.domains])
+        b.close()
+    for s1, s2 in zip(*out):
+        for k in ("f", "g", "bx", "by", "bz"):
+            np.testing.assert_array_equal(s1[k], s2[k])
