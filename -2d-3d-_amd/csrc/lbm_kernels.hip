@@ -247,13 +247,29 @@ __device__ __forceinline__ double ld_off(const double *P, int64_t off, uint32_t 
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(row_rsrc(P, off), vo, 0, 0));
 }
+template <int X, int... A>
+__device__ __forceinline__ void redo_x_flow(const double *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                            const KParams &kp, double *f, std::integer_sequence<int, A...>)
+{
+    ((f[A] = D3Q19::ex[A] == X ? P[flow_elem<D3Q19, A>(N, i, j, k, c, kp)] : f[A]), ...);
+}
+template <int X, int... A>
+__device__ __forceinline__ void redo_x_heat(const double *P, uint64_t N, int i, int j, int k, uint64_t c,
+                                            const KParams &kp, double *g, std::integer_sequence<int, A...>)
+{
+    ((g[A] = D3Q7::ex[A] == X ? heat_in<D3Q7, A>(P, N, i, j, k, c, kp) : g[A]), ...);
+}
 __device__ __forceinline__ void offset_flow_in(const double *P, const PullOffsets &po, uint32_t vo,
                                                int i, int j, int k, uint64_t c, const KParams &kp,
                                                double *f)
 {
 #pragma unroll
     for (int a = 0; a < 19; ++a) f[a] = ld_off(P, po.f[a], vo);
-    if (i == 0 || i == kp.nx - 1) pull_flow<D3Q19>(P, kp.P, i, j, k, c, true, kp, f);
+    // x-wall nodes redo the pulls whose source lies beyond their wall (ex = +1 at i = 0,
+    // ex = -1 at i = nx - 1) with the wall rules (flow_elem: the rules of pull_flow); a wave
+    // holds one of the two walls
+    if (i == 0) redo_x_flow<1>(P, kp.P, i, j, k, c, kp, f, std::make_integer_sequence<int, 19>{});
+    if (i == kp.nx - 1) redo_x_flow<-1>(P, kp.P, i, j, k, c, kp, f, std::make_integer_sequence<int, 19>{});
 }
 __device__ __forceinline__ void offset_heat_in(const double *P, const PullOffsets &po, uint32_t vo,
                                                int i, int j, int k, uint64_t c, const KParams &kp,
@@ -261,8 +277,8 @@ __device__ __forceinline__ void offset_heat_in(const double *P, const PullOffset
 {
 #pragma unroll
     for (int a = 0; a < 7; ++a) g[a] = ld_off(P, po.g[a], vo);
-    if (i == 0 || i == kp.nx - 1)
-        heat_in_all<D3Q7>(P, kp.P, i, j, k, c, kp, g, std::make_integer_sequence<int, 7>{});
+    if (i == 0) redo_x_heat<1>(P, kp.P, i, j, k, c, kp, g, std::make_integer_sequence<int, 7>{});
+    if (i == kp.nx - 1) redo_x_heat<-1>(P, kp.P, i, j, k, c, kp, g, std::make_integer_sequence<int, 7>{});
 }
 template <int Q>
 __device__ __forceinline__ void offset_out(double *P, const PullOffsets &po, uint32_t vo,

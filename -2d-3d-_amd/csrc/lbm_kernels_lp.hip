@@ -313,12 +313,17 @@ __device__ __forceinline__ void fast_flow_in(const S *P, const PullOffsets &pl, 
                                              std::integer_sequence<int, A...>)
 {
     ((f[A] = ld_at<S, 19, A>(P, LAY == LAY_PULL ? pl.f[A] : (LAY == LAY_ODD ? pl.s[A] : pl.fe[A]), vo)), ...);
-    if (i == 0 || i == kp.nx - 1) {
+    // x-wall nodes redo the pulls whose source lies beyond their wall (ex = +1 at i = 0,
+    // ex = -1 at i = nx - 1) with the wall rules; a wave holds one of the two walls
+    auto redo = [&](auto sgn) {
+        constexpr int X = decltype(sgn)::value;
         if constexpr (LAY == LAY_PULL)
-            ((f[A] = D3Q19::ex[A] != 0 ? flow_in<D3Q19, S, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
+            ((f[A] = D3Q19::ex[A] == X ? flow_in<D3Q19, S, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
         else
-            ((f[A] = D3Q19::ex[A] != 0 ? flow_in_aa<D3Q19, S, LAY, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
-    }
+            ((f[A] = D3Q19::ex[A] == X ? flow_in_aa<D3Q19, S, LAY, A>(P, kp.P, i, j, k, c, true, kp) : f[A]), ...);
+    };
+    if (i == 0) redo(std::integral_constant<int, 1>{});
+    if (i == kp.nx - 1) redo(std::integral_constant<int, -1>{});
 }
 template <typename S, int LAY, int... A>
 __device__ __forceinline__ void fast_heat_in(const S *P, const PullOffsets &pl, uint32_t vo, int i, int j,
@@ -328,12 +333,15 @@ __device__ __forceinline__ void fast_heat_in(const S *P, const PullOffsets &pl, 
 {
     ((g[A] = ld_at<S, 7, A>(P, LAY == LAY_PULL ? pl.g[A] : (LAY == LAY_ODD ? pl.s[A] : pl.ge[A]), vo) +
              t0_pull<A>(lp)), ...);
-    if (i == 0 || i == kp.nx - 1) {
+    auto redo = [&](auto sgn) {   // as in fast_flow_in
+        constexpr int X = decltype(sgn)::value;
         if constexpr (LAY == LAY_PULL)
-            ((g[A] = D3Q7::ex[A] != 0 ? heat_in<D3Q7, S, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
+            ((g[A] = D3Q7::ex[A] == X ? heat_in<D3Q7, S, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
         else
-            ((g[A] = D3Q7::ex[A] != 0 ? heat_in_aa<D3Q7, S, LAY, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
-    }
+            ((g[A] = D3Q7::ex[A] == X ? heat_in_aa<D3Q7, S, LAY, A>(P, kp.P, i, j, k, c, true, kp, lp) : g[A]), ...);
+    };
+    if (i == 0) redo(std::integral_constant<int, 1>{});
+    if (i == kp.nx - 1) redo(std::integral_constant<int, -1>{});
 }
 template <class L, typename S, int LAY, int... A>
 __device__ __forceinline__ void fast_out(S *P, const PullOffsets &pl, uint32_t vo, int i, int j, int k,
