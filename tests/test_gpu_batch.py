@@ -226,15 +226,21 @@ def test_batch_auto_table_form_by_total_size(prec):
              [(256, 256, 128, 1e6, 0.71), (256, 256, 128, 3e5, 7.0)]
     cfgs = [rblbm.Config(dims=3, nx=a, ny=b, nz=c, rayleigh=ra, prandtl=pr, precision=prec)
             for a, b, c, ra, pr in shapes]
-    out = []
-    for mode in (rblbm.BATCH_GRAPH, rblbm.BATCH_PER_DOMAIN):
+    # a run switching modes mid-way passes between the one-pass table form and each cavity's
+    # two-pass form in both directions
+    runs = {"graph": [(rblbm.BATCH_GRAPH, 11)], "per_domain": [(rblbm.BATCH_PER_DOMAIN, 11)],
+            "mixed": [(rblbm.BATCH_GRAPH, 5), (rblbm.BATCH_PER_DOMAIN, 3), (rblbm.BATCH_ONE_LAUNCH, 3)]}
+    out = {}
+    for name, plan in runs.items():
         b = rblbm.Batch(cfgs)
         assert all(d.kernel_form()["variant"] == rblbm.KERNEL_TWO_PASS for d in b.domains)
-        b.set_mode(mode)
         b.initial_seeded(41, 0.05)
-        b.step(11)
-        out.append([d.gather() for d in b.domains])
+        for mode, n in plan:
+            b.set_mode(mode)
+            b.step(n)
+        out[name] = [d.gather() for d in b.domains]
         b.close()
-    for s1, s2 in zip(*out):
-        for k in ("f", "g", "bx", "by", "bz"):
-            np.testing.assert_array_equal(s1[k], s2[k])
+    for name in ("graph", "mixed"):
+        for s1, s2 in zip(out[name], out["per_domain"]):
+            for k in ("f", "g", "bx", "by", "bz"):
+                np.testing.assert_array_equal(s1[k], s2[k], err_msg=f"{name} {k}")
