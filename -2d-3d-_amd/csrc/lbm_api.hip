@@ -186,6 +186,14 @@ static int default_kchunk(int nz)
     return k > nz ? nz : k;
 }
 
+// The one-pass kernel stores a tile node's values through buffer descriptors based at its plane
+// with the node's in-plane byte offset, a 32-bit value: the form needs nx * ny * 8 < 2^32 (every
+// plane of populations and history in every precision form).
+static bool fused_plane_fits(const lbm_config &c)
+{
+    return (uint64_t)c.nx * (uint64_t)c.ny * 8u < ((uint64_t)1 << 32);
+}
+
 // The one-pass form's default tile and prefetching per precision form (see create_impl).
 static void default_tile(int precision, int *tile, int *prefetch)
 {
@@ -359,7 +367,8 @@ static int create_impl(const lbm_config *cfg, hipStream_t shared, lbm_handle **o
         free(h);
         return LBM_ERR_ARG;
     }
-    h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC && !h->aa;
+    h->fused_ok = c.dims == 3 && c.temp_x != LBM_T_PERIODIC && c.temp_z != LBM_T_PERIODIC && !h->aa &&
+                  fused_plane_fits(c);
     h->variant = LBM_KERNEL_AUTO;
     // the one-pass form's default shape, the fastest one-pass form at 512^3: FP64 the 34x15 tile
     // with prefetching into registers; FP32-shift the 66x15 tile without prefetching (95
@@ -873,7 +882,7 @@ static int convert_layout(lbm_handle *h, bool aa)
     h->Bp_alt = b1;
     h->dev_bytes += fb + gb + N * 3 * ea;
     h->aa = false;
-    h->fused_ok = true;   // aa_ok implies the one-pass conditions
+    h->fused_ok = fused_plane_fits(h->cfg);   // aa_ok implies the wall conditions
     return LBM_OK;
 }
 

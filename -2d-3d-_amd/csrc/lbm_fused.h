@@ -131,6 +131,12 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     const bool col_inner = in_dom && i > 0 && j > 0 && i < nx - 1 && j < ny - 1 &&
                            plane * sizeof(S) < ((uint64_t)1 << 32) && plane * sizeof(B) < ((uint64_t)1 << 32);
     const uint32_t vo_s = (uint32_t)(col * sizeof(S)), vo_b = (uint32_t)(col * sizeof(B));
+    // Tile nodes store through buffer descriptors based at their plane's element, with the
+    // lane's in-plane byte offset vo_s / vo_b: no 64-bit address arithmetic per store.  The host
+    // runs the one-pass form only when a plane's bytes fit 32 bits (fused_plane_fits).
+    auto store_f_at = [&](int q, const R *v) {
+        Form::store_f_inner(Fnext, N, (int64_t)q * (int64_t)plane, vo_s, v);
+    };
     const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1);
     const uint64_t colc = (uint64_t)ic + (uint64_t)nx * (uint64_t)jc;
     auto load = [&](int kq, S(&lg)[7], S(&lf)[19], B(&lb)[3], unsigned &modes) -> bool {
@@ -182,10 +188,10 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
     unsigned qmodes = 0;
     bool q_have = REGS ? load(k0 - 1, qg, qf, qb, qmodes)
                        : (PF == 2 ? stage_load(k0 - 1, qmodes, qb) : false);
-    // PF == 2: the flow stores of plane `fpend_c`, issued after the next wait
+    // PF == 2: the flow stores of plane `fpend_p`, issued after the next wait
     R fout[PF == 2 ? 19 : 1];
     bool fpend = false;
-    uint64_t fpend_c = 0;
+    int fpend_p = 0;
 
     for (int kk = k0 - 1; kk <= k1; ++kk) {
         R lg[7], fnew[19];
@@ -200,7 +206,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
             __builtin_amdgcn_s_waitcnt(0x0F70);
             asm volatile("" ::: "memory");
             if (fpend) {
-                Form::store_f(Fnext, N, fpend_c, fout);
+                store_f_at(fpend_p, fout);
                 fpend = false;
             }
             if (have) {
@@ -225,17 +231,17 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         R gnew[7] = {};
         R rho_n = 0, u_n = 0, v_n = 0, w_n = 0;
         if (have) {
-            const uint64_t c = col + plane * (uint64_t)kk;
             const R Tn = Form::tsum(lg);
             Form::macro(fnew, Tn, kp, fp, nd, rho_n, u_n, v_n, w_n);
             Form::thermal(lg, gnew, Tn, u_n, v_n, w_n, bp, kp, fp, nd);
             Form::round_g(gnew);   // exchanged and stored as the storage rounds them
             if (tile && kk >= k0 && kk < k1) {
-                Form::store_g(Gout, N, c, gnew);
+                const int64_t pk = (int64_t)kk * (int64_t)plane;
+                Form::store_g_inner(Gout, N, pk, vo_s, gnew);
                 if (kp.use_g) {
-                    Bpout[c] = bp[0];
-                    Bpout[N + c] = bp[1];
-                    Bpout[2 * N + c] = bp[2];
+                    buf_st(Bpout + pk, vo_b, bp[0]);
+                    buf_st(Bpout + ((int64_t)N + pk), vo_b, bp[1]);
+                    buf_st(Bpout + (2 * (int64_t)N + pk), vo_b, bp[2]);
                 }
             }
         }
@@ -273,9 +279,9 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
 #pragma unroll
                 for (int a = 0; a < 19; ++a) fout[a] = out[a];
                 fpend = true;
-                fpend_c = col + plane * (uint64_t)p;
+                fpend_p = p;
             } else {
-                Form::store_f(Fnext, N, col + plane * (uint64_t)p, out);
+                store_f_at(p, out);
             }
         }
         // shift the plane pipeline
@@ -292,7 +298,7 @@ fused_tile(const KParams &kp, const typename Form::P &fp, const FusedGeom &fg, u
         have_prev = have;
     }
     if constexpr (PF == 2)
-        if (fpend) Form::store_f(Fnext, N, fpend_c, fout);
+        if (fpend) store_f_at(fpend_p, fout);
 }
 
 // The launch's domains and their parameters come from a table in device memory: block b (after

@@ -1304,6 +1304,20 @@ struct FormF64 {
 #pragma unroll
         for (int a = 0; a < 19; ++a) F[a * N + c] = f[a];
     }
+    // stores of plane element pk at the lane's in-plane byte offset vo (buffer descriptors: no
+    // 64-bit address arithmetic per store)
+    static __device__ __forceinline__ void store_g_inner(double *G, uint64_t N, int64_t pk, uint32_t vo,
+                                                         const double *g)
+    {
+#pragma unroll
+        for (int a = 0; a < 7; ++a) buf_st(G + ((int64_t)a * (int64_t)N + pk), vo, g[a]);
+    }
+    static __device__ __forceinline__ void store_f_inner(double *F, uint64_t N, int64_t pk, uint32_t vo,
+                                                         const double *f)
+    {
+#pragma unroll
+        for (int a = 0; a < 19; ++a) buf_st(F + ((int64_t)a * (int64_t)N + pk), vo, f[a]);
+    }
     // Wall value of thermal direction a at the node itself (adiabatic or constant T).
     static __device__ __forceinline__ double wall_g(int a, const double *gp, int kind, const P &p,
                                                     const Node &)

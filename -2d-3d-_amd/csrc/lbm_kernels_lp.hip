@@ -935,6 +935,26 @@ template <typename SS> struct FormLP {
     {
         store_all<D3Q19, S>(F, N, c, f, SEQ(D3Q19));
     }
+    // as store_g / store_f at plane element pk and in-plane byte offset vo (FP32: buffer
+    // descriptors; FP16 keeps the converting stores at c = pk + vo / 2)
+    template <int... A>
+    static __device__ __forceinline__ void store_inner_seq(S *P, uint64_t N, int64_t pk, uint32_t vo,
+                                                           const float *v, std::integer_sequence<int, A...>)
+    {
+        (buf_st(P + ((int64_t)A * (int64_t)N + pk), vo, v[A]), ...);
+    }
+    static __device__ __forceinline__ void store_g_inner(S *G, uint64_t N, int64_t pk, uint32_t vo,
+                                                         const float *g)
+    {
+        if constexpr (sizeof(S) == 4) store_inner_seq(G, N, pk, vo, g, SEQ(D3Q7));
+        else store_g(G, N, (uint64_t)pk + vo / sizeof(S), g);
+    }
+    static __device__ __forceinline__ void store_f_inner(S *F, uint64_t N, int64_t pk, uint32_t vo,
+                                                         const float *f)
+    {
+        if constexpr (sizeof(S) == 4) store_inner_seq(F, N, pk, vo, f, SEQ(D3Q19));
+        else store_f(F, N, (uint64_t)pk + vo / sizeof(S), f);
+    }
     // prefetching into registers (lbm_fused.h, PF == 1 / 3): raw loads at the pulls' source
     // addresses (returning the thermal wall modes, two bits each), converted after arrival
     static __device__ __forceinline__ void raw_f(const S *F, uint64_t N, int i, int j, int k,

@@ -30,6 +30,20 @@ template <typename T> __device__ __forceinline__ T buf_ld(const T *base, uint32_
     else
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(base), vo, 0, 0));
 }
+// Stores of interior nodes, addressed as buf_ld's loads.
+template <typename T> __device__ __forceinline__ void buf_st(T *base, uint32_t vo, T v)
+{
+    const auto r = rsrc_at(base);
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r, vo, 0, 0);
+    else if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0)), v), r, vo, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b16(r, 0, 0, 0)), v), r, vo, 0, 0);
+}
 // The stored element a pull of direction A reads: its upstream neighbour's, the periodic image,
 // or (walls) the node's own population of the opposite direction.  The thermal version also
 // says how the read value becomes the incoming one: PULL_T0 (plus the shift correction),
