@@ -47,6 +47,7 @@ struct lbm_handle {
     int fpar, gpar;         // LAY_ODD / LAY_EVEN of F[0] and G[0] (in-place layout, phase 1)
     bool capturing;         // the handle's stream is being captured into a HIP graph
     FusedGeom fg;
+    FusedGeom bfg;          // geometry of the domain's entry in its batch's one-pass table
     int64_t itc;
     bool owns_stream;       // false for the domains of a batch (they share the batch's stream)
     bool in_batch;          // destroyed with its batch, not by lbm_destroy
@@ -186,6 +187,14 @@ static int default_kchunk(int nz)
     return k > nz ? nz : k;
 }
 
+// Default z-chunk of a domain in its batch's one-pass table: min(nz, 32).  The table launch
+// fills the device with the tiles of all its domains, so fewer halo planes win over more
+// blocks.  The seeded 64-cavity ensemble (edges 32-128) in the one-pass table form, graph
+// replay, two processes each: FP32-shift 2.83-2.84 ms per step with the single-domain default
+// (8 planes for these depths), 2.70-2.76 with 16, 2.53-2.76 with 32; FP64 4.86-5.34, 4.98-5.15,
+// 4.62-5.15.  A domain set to the one-pass form keeps its own z-chunk (lbm_batch_autotune).
+static int batch_kchunk(int nz) { return nz < 32 ? nz : 32; }
+
 // The one-pass kernel stores a tile node's values through buffer descriptors based at its plane
 // with the node's in-plane byte offset, a 32-bit value: the form needs nx * ny * 8 < 2^32 (every
 // plane of populations and history in every precision form).
@@ -201,21 +210,26 @@ static void default_tile(int precision, int *tile, int *prefetch)
     *prefetch = precision == LBM_FP32_SHIFT ? LBM_PREFETCH_NONE : LBM_PREFETCH_REGS;
 }
 
-// One-pass launch geometry for z-chunks of `kchunk` planes (LBM_KCHUNK_AUTO: default_kchunk)
-// and thread tile shape `tile`.
-static void set_fused_geom(lbm_handle *h, int kchunk, int tile, int prefetch)
+// One-pass launch geometry of configuration c for z-chunks of `kchunk` planes
+// (LBM_KCHUNK_AUTO: default_kchunk) and thread tile shape `tile`.
+static FusedGeom make_fused_geom(const lbm_config &c, int kchunk, int tile, int prefetch)
 {
     int tx, ty;
     fused_tile_dims(tile, &tx, &ty);
-    FusedGeom &g = h->fg;
+    FusedGeom g;
     g.tile = tile;
     g.pf = prefetch;
-    g.ntx = (unsigned)((h->cfg.nx + tx - 3) / (tx - 2));
-    g.nty = (unsigned)((h->cfg.ny + ty - 3) / (ty - 2));
-    if (kchunk <= 0) kchunk = default_kchunk(h->cfg.nz);
-    g.kchunk = kchunk > h->cfg.nz ? h->cfg.nz : kchunk;
-    g.nzc = (unsigned)((h->cfg.nz + g.kchunk - 1) / g.kchunk);
+    g.ntx = (unsigned)((c.nx + tx - 3) / (tx - 2));
+    g.nty = (unsigned)((c.ny + ty - 3) / (ty - 2));
+    if (kchunk <= 0) kchunk = default_kchunk(c.nz);
+    g.kchunk = kchunk > c.nz ? c.nz : kchunk;
+    g.nzc = (unsigned)((c.nz + g.kchunk - 1) / g.kchunk);
     g.nblocks = g.ntx * g.nty * g.nzc;
+    return g;
+}
+static void set_fused_geom(lbm_handle *h, int kchunk, int tile, int prefetch)
+{
+    h->fg = make_fused_geom(h->cfg, kchunk, tile, prefetch);
     h->ftab_ok = false;
 }
 
@@ -239,10 +253,11 @@ static size_t fused_entry_bytes(const lbm_handle *h)
     return h->lp ? fused_instance_bytes_lp(h->cfg.precision) : fused_instance_bytes_f64();
 }
 
-static void fused_fill(const lbm_handle *h, void *dst, const FusedPtrs &p, unsigned first_block)
+static void fused_fill(const lbm_handle *h, void *dst, const FusedPtrs &p, unsigned first_block,
+                       const FusedGeom &g)
 {
-    if (h->lp) fused_instance_fill_lp(h->cfg.precision, dst, h->kp, h->fg, p, first_block);
-    else fused_instance_fill_f64(dst, h->kp, h->fg, p, first_block);
+    if (h->lp) fused_instance_fill_lp(h->cfg.precision, dst, h->kp, g, p, first_block);
+    else fused_instance_fill_f64(dst, h->kp, g, p, first_block);
 }
 
 // One-pass step of a single domain: launch from the handle's two-entry table, rebuilding it
@@ -271,7 +286,7 @@ static int launch_fused_single(lbm_handle *h)
         HIPCHK(h, hipStreamSynchronize(h->stream));
         std::vector<unsigned char> host(2 * isz);
         for (int q = 0; q < 2; ++q)
-            fused_fill(h, host.data() + q * isz, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, q), 0);
+            fused_fill(h, host.data() + q * isz, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, q), 0, h->fg);
         HIPCHK(h, hipMemcpy(h->ftab, host.data(), 2 * isz, hipMemcpyHostToDevice));
         h->ftab_ok = true;
         h->ftab_fcur = h->fcur;
@@ -2139,7 +2154,7 @@ extern "C" int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp)
 // cannot fill the device with one-pass tiles; a one-launch step of the whole batch can.  This
 // times the batch's one-launch step with those choices against every domain in the one-pass
 // form (domain 0's one-pass tile and prefetch when it chose the one-pass form, else the
-// precision form's default; each domain's one-pass z-chunk, else the default for its depth) on
+// precision form's default; each domain's one-pass z-chunk, else the batch default) on
 // a scratch batch of the same configurations, and switches to the one-pass form when that is
 // more than 3% faster.  Skipped when a domain cannot run the one-pass form, when every domain
 // already does, or when free device memory cannot hold the scratch batch.
@@ -2168,7 +2183,7 @@ static int batch_tune_table_form(lbm_batch *b, int steps)
         default_tile(b->precision, &tile0, &pf0);
     }
     auto one_pass_kchunk = [](const lbm_handle *h) {
-        return h->variant == LBM_KERNEL_ONE_PASS ? h->fg.kchunk : default_kchunk(h->cfg.nz);
+        return h->variant == LBM_KERNEL_ONE_PASS ? h->fg.kchunk : batch_kchunk(h->cfg.nz);
     };
     std::vector<lbm_config> cf(b->n);
     for (int i = 0; i < b->n; ++i) cf[i] = b->dom[i]->cfg;
@@ -2270,8 +2285,9 @@ static bool batch_table_one_pass(const lbm_batch *b)
 // Blocks of one domain in the batch's form.
 static unsigned batch_blocks(const lbm_batch *b, const lbm_handle *h)
 {
-    return b->twopass ? (unsigned)((h->N + 255) / 256) : h->fg.nblocks;
+    return b->twopass ? (unsigned)((h->N + 255) / 256) : h->bfg.nblocks;
 }
+
 
 // Tables for both buffer parities from the domains' current pointers: parity p is the current
 // state, parity p^1 the state after one step (F and G swapped; in the one-pass form also
@@ -2280,6 +2296,12 @@ static int batch_tables(lbm_batch *b)
 {
     lbm_handle *h0 = b->dom[0];
     b->twopass = !batch_table_one_pass(b);
+    for (int i = 0; i < b->n; ++i) {
+        lbm_handle *h = b->dom[i];
+        h->bfg = h->variant == LBM_KERNEL_ONE_PASS
+                     ? h->fg
+                     : make_fused_geom(h->cfg, batch_kchunk(h->cfg.nz), h->fg.tile, h->fg.pf);
+    }
     const size_t isz = !b->twopass ? fused_entry_bytes(h0)
                        : h0->lp    ? twopass_entry_bytes_lp(b->precision)
                                    : twopass_entry_bytes_f64();
@@ -2304,7 +2326,7 @@ static int batch_tables(lbm_batch *b)
             lbm_handle *h = b->dom[i];
             void *dst = host.data() + isz * i;
             if (!b->twopass)
-                fused_fill(h, dst, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, par), first);
+                fused_fill(h, dst, fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp_alt, par), first, h->bfg);
             else if (h->lp)
                 twopass_entry_fill_lp(b->precision, dst, h->kp,
                                       fused_ptrs(h, h->fcur, h->gcur, h->Bp, h->Bp, par),

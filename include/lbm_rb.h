@@ -261,7 +261,9 @@ int lbm_autotune(lbm_handle *h, const char *cache_path, int32_t steps, double *m
  * (lbm_batch_set_kernel_variant or lbm_batch_autotune) or, under AUTO, the batch's total node
  * count reaches the size at which AUTO runs a single domain in the one-pass form (2^22 FP64,
  * 2^24 FP32-shift; never FP16-shift); otherwise the two-pass form (two launches a step over
- * 256-node blocks of all domains).  One launch per domain runs each domain's own form. */
+ * 256-node blocks of all domains).  One launch per domain runs each domain's own form.  In the
+ * one-pass table a domain not set to LBM_KERNEL_ONE_PASS takes z-chunks of min(nz, 32) planes
+ * (the table launch fills the device, so fewer halo planes win); one set to it keeps its own. */
 typedef struct lbm_batch lbm_batch;
 #define LBM_BATCH_PER_DOMAIN 0    /* one launch per domain per step */
 #define LBM_BATCH_ONE_LAUNCH 1    /* one launch per step for all domains (per-domain table) */
