@@ -2155,19 +2155,25 @@ extern "C" int lbm_batch_init_seeded(lbm_batch *b, uint64_t seed, double amp)
 // times the batch's one-launch step with those choices against every domain in the one-pass
 // form (domain 0's one-pass tile and prefetch when it chose the one-pass form, else the
 // precision form's default; each domain's one-pass z-chunk, else the batch default) on
-// a scratch batch of the same configurations, and switches to the one-pass form when that is
-// more than 3% faster.  Skipped when a domain cannot run the one-pass form, when every domain
-// already does, or when free device memory cannot hold the scratch batch.
+// a scratch batch of the same configurations.  The incumbent is the form the batch runs under
+// AUTO (the one-pass table once the batch's total node count reaches the one-pass size, see
+// batch_table_one_pass); the other is taken only when it is more than 3% faster, since
+// placement alone moves a form's time by several per cent.  Skipped when a domain cannot run
+// the one-pass form, when every domain already does, or when free device memory cannot hold
+// the scratch batch.
 static int batch_tune_table_form(lbm_batch *b, int steps)
 {
     lbm_handle *h0 = b->dom[0];
     bool all_one = true;
     size_t need = 0;
+    uint64_t total = 0;
     for (int i = 0; i < b->n; ++i) {
         if (!b->dom[i]->fused_ok) return LBM_OK;
         all_one = all_one && b->dom[i]->variant == LBM_KERNEL_ONE_PASS;
         need += b->dom[i]->dev_bytes;
+        total += b->dom[i]->N;
     }
+    const bool auto_one = b->precision != LBM_FP16_SHIFT && total >= one_pass_min_nodes(b->precision);
     if (all_one) return LBM_OK;
     size_t free_b = 0, all_b = 0;
     if (hipMemGetInfo(&free_b, &all_b) != hipSuccess) {
@@ -2206,7 +2212,7 @@ static int batch_tune_table_form(lbm_batch *b, int steps)
     }
     lbm_batch_destroy(t);
     if (rc) return fail(h0, rc, "batch autotune: timing failed");
-    if (ms[1] < 0.97 * ms[0])
+    if (auto_one ? !(ms[0] < 0.97 * ms[1]) : ms[1] < 0.97 * ms[0])
         for (int i = 0; i < b->n && !rc; ++i) {
             lbm_handle *h = b->dom[i];
             rc = set_form(h, LBM_KERNEL_ONE_PASS, one_pass_kchunk(h), tile0, pf0);

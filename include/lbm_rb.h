@@ -285,9 +285,10 @@ int lbm_batch_run_diagnostics(lbm_batch *b, int32_t nsteps, int32_t every, lbm_d
 /* lbm_autotune for each distinct domain shape (the one-launch modes use the tuned z-chunks),
  * then, unless every domain already runs the one-pass form, the batch's one-launch step with
  * those choices is timed against every domain in the one-pass form on a scratch batch of the
- * same configurations (when free device memory holds it), and the one-pass form is taken if it
- * is more than 3% faster: a small cavity alone cannot fill the device with one-pass tiles, the
- * whole batch can.  The per-shape choices go to the cache; the batch-level choice does not. */
+ * same configurations (when free device memory holds it): a small cavity alone cannot fill the
+ * device with one-pass tiles, the whole batch can.  The form AUTO would run for the batch is
+ * kept unless the other is more than 3% faster.  The per-shape choices go to the cache; the
+ * batch-level choice does not. */
 int lbm_batch_autotune(lbm_batch *b, const char *cache_path, int32_t steps);
 
 const char *lbm_last_error(lbm_handle *h);
