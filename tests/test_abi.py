@@ -119,3 +119,18 @@ def test_fortran_binding_types_match_the_header(tmp_path):
         assert got[(t, "sizeof")] == ctypes.sizeof(py), t
         for field, _ in py._fields_:
             assert got[(t, field)] == getattr(py, field).offset, (t, field)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """The product path has no fallback: a missing librblbm.so raises before any computation."""
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import rblbm\n"
+            "try:\n"
+            "    rblbm.library(%r)\n"
+            "except FileNotFoundError as e:\n"
+            "    print('raised:', e)\n"
+            "else:\n"
+            "    print('loaded')\n") % (os.path.join(ROOT, "-2d-3d-_amd"), str(tmp_path / "librblbm.so"))
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("raised:") and "no fallback" in r.stdout
